@@ -1,0 +1,26 @@
+"""MI355X-native distance-driven fan-beam reconstruction path.
+
+Drop-in replacement for the native hot path of
+Angus1996/AAPM_LowDoseCTImageRecon_competition: the multi-slice DD projector
+and backprojector (MultiSlicesProjGPU.cu, MultiSlicesBackGPU.cu), the
+helical-to-fan rebinning (HelicalToFanFunc_mex.cpp) and the OS-SART driver
+(OSSART_AAPM.m), implemented as hand-written HIP kernels for gfx950 behind
+the C ABI in include/ddfan.h (libddfan.so).
+"""
+from . import _native
+from ._native import DDFanError, DeviceBuffer, device_count
+from .dd2 import dd2_multi_slices
+from .geometry import Conf, detector_arc, detector_geometry, recon_mask, subset_views
+from .ossart import ossart_aapm
+from .plan import Plan, plan_footprint
+from .rebin import HelicalCfg, helical_to_fan_func, helical_to_fan_routine
+
+__all__ = [
+    "Conf", "DDFanError", "DeviceBuffer", "HelicalCfg", "Plan", "dd2_multi_slices",
+    "detector_arc", "detector_geometry", "device_count", "helical_to_fan_func",
+    "helical_to_fan_routine", "ossart_aapm", "plan_footprint", "recon_mask", "subset_views",
+]
+
+
+def native_library_path() -> str:
+    return _native.LIB_PATH

@@ -1,0 +1,885 @@
+// ddfan_api.hip -- C ABI: drop-in entry points, plans, OS-SART drivers.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <atomic>
+#include <string>
+#include <vector>
+
+#include "ddfan.h"
+#include "ddfan_host.h"
+#include "ddfan_internal.h"
+
+using namespace ddf;
+
+namespace {
+
+thread_local std::string t_err;
+std::atomic<int> g_default_sat{DDFAN_SAT_F32};
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    t_err = buf;
+    return code;
+}
+
+#define HIPCALL(x)                                                                           \
+    do {                                                                                     \
+        hipError_t e_ = (x);                                                                 \
+        if (e_ != hipSuccess)                                                                \
+            return fail(e_ == hipErrorOutOfMemory ? DDFAN_ENOMEM : DDFAN_EDEVICE,            \
+                        "%s: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, __LINE__);    \
+    } while (0)
+
+#define CHECK(cond, code, ...)                    \
+    do {                                          \
+        if (!(cond)) return fail(code, __VA_ARGS__); \
+    } while (0)
+
+#define TRY(x)                       \
+    do {                             \
+        int rc_ = (x);               \
+        if (rc_ != DDFAN_OK) return rc_; \
+    } while (0)
+
+// Owning device allocation.
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t reserve(size_t b) {
+        if (p && b <= bytes) return hipSuccess;
+        release();
+        hipError_t e = hipMalloc(&p, b ? b : 16);
+        if (e == hipSuccess)
+            bytes = b;
+        else
+            p = nullptr;
+        return e;
+    }
+    template <class T>
+    T* as() const {
+        return reinterpret_cast<T*>(p);
+    }
+};
+
+int check_device(int* dev) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return fail(DDFAN_ENODEV, "no GPU visible");
+    HIPCALL(hipGetDevice(dev));
+    return DDFAN_OK;
+}
+
+int check_subset(int PN, int first, int stride, int count) {
+    CHECK(count >= 0 && stride >= 1 && first >= 0, DDFAN_EINVAL,
+          "bad view subset (first %d stride %d count %d)", first, stride, count);
+    if (count > 0)
+        CHECK((long long)first + (long long)(count - 1) * stride < PN, DDFAN_EINVAL,
+              "view subset (first %d stride %d count %d) exceeds %d views", first, stride, count,
+              PN);
+    return DDFAN_OK;
+}
+
+// One slice split per GPU: startIdx[k] is the first slice of shard k
+// (DD2MutiSlices.m:95-101).  On one device the shards are the identity.
+int check_start_idx(const int* startIdx, int nStart, int SLN) {
+    if (!startIdx || nStart <= 0) return DDFAN_OK;
+    CHECK(startIdx[0] == 0, DDFAN_EINVAL, "startIdx[0] must be 0 (got %d)", startIdx[0]);
+    for (int k = 1; k < nStart; ++k)
+        CHECK(startIdx[k] >= startIdx[k - 1] && startIdx[k] <= SLN, DDFAN_EINVAL,
+              "startIdx must be non-decreasing within [0, SLN]");
+    return DDFAN_OK;
+}
+
+size_t sat_vol_bytes(int mode, int nzc, int ZT, int XN, int YN) {
+    return sat_elem_bytes(mode) * (size_t)nzc * ZT * (size_t)(XN + 1) * YN;
+}
+size_t sat_sino_bytes(int mode, int nzc, int ZT, int DNU, int views) {
+    return sat_elem_bytes(mode) * (size_t)nzc * ZT * (size_t)(DNU + 1) * views;
+}
+
+}  // namespace
+
+// =================================================================== plan
+struct ddfan_plan {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    GeomDev g{};
+    int PN = 0, mode = 0, ZT = 1, nzc = 1;
+    std::vector<float> h_hangs, h_cs;
+    DBuf cs, bx, by, xds, yds, mask;
+    DBuf satX, satY, satP;
+    // OS-SART state
+    DBuf rbuf, rowSum, colSum, xprev, momTab, counter, cs0, onesV, onesP, satX0, satY0, satP0;
+    bool timing = false;
+    struct Ev {
+        int kind;
+        hipEvent_t a, b;
+    };
+    std::vector<Ev> events;
+
+    ~ddfan_plan() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        clear_events();
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    void clear_events() {
+        for (auto& e : events) {
+            (void)hipEventDestroy(e.a);
+            (void)hipEventDestroy(e.b);
+        }
+        events.clear();
+    }
+    template <class F>
+    hipError_t timed(int kind, F&& f) {
+        if (!timing) return f();
+        hipEvent_t a, b;
+        hipError_t e = hipEventCreate(&a);
+        if (e != hipSuccess) return e;
+        e = hipEventCreate(&b);
+        if (e != hipSuccess) {
+            (void)hipEventDestroy(a);
+            return e;
+        }
+        (void)hipEventRecord(a, stream);
+        e = f();
+        (void)hipEventRecord(b, stream);
+        events.push_back({kind, a, b});
+        return e;
+    }
+};
+
+namespace {
+
+void fill_geom(GeomDev& g, const ddfan_geom* G, int SLN) {
+    g.x0 = G->x0;
+    g.y0 = G->y0;
+    g.dx = G->dx;
+    g.invdx = 1.0f / G->dx;  // MultiSlicesProjGPU.cu:149
+    g.ocxP = (G->XN - 1.0f) * 0.5f - G->imgXCenter / G->dx;  // :225-226
+    g.ocyP = (G->YN - 1.0f) * 0.5f - G->imgYCenter / G->dx;
+    g.ocxB = (float)((G->XN - 1.0) * 0.5 - (double)(G->imgXCenter / G->dx));  // Back :198-200
+    g.ocyB = (float)((G->YN - 1.0) * 0.5 - (double)(G->imgYCenter / G->dx));
+    g.inv_sid = 1.0f / sqrtf(G->x0 * G->x0 + G->y0 * G->y0);  // Back :141
+    float S2D, dbeta;
+    back_scalars(G->x0, G->y0, G->xds, G->yds, &S2D, &dbeta);
+    g.invdb = (float)(1.0 / (double)dbeta);  // Back :150
+    g.detCntIdx = G->detCntIdx;
+    g.XN = G->XN;
+    g.YN = G->YN;
+    g.SLN = SLN;
+    g.DNU = G->DNU;
+}
+
+int validate_geom(const ddfan_geom* G) {
+    CHECK(G, DDFAN_EINVAL, "geometry is NULL");
+    CHECK(G->DNU >= 2, DDFAN_EINVAL, "DNU must be >= 2 (got %d)", G->DNU);
+    CHECK(G->XN >= 1 && G->YN >= 1 && G->SLN >= 1, DDFAN_EINVAL, "XN, YN, SLN must be >= 1");
+    CHECK(G->xds && G->yds, DDFAN_EINVAL, "xds/yds are NULL");
+    CHECK(G->dx > 0.0f && isfinite(G->dx), DDFAN_EINVAL, "dx must be positive");
+    CHECK((long long)G->XN * G->YN * G->SLN < (1LL << 40), DDFAN_EINVAL, "volume too large");
+    return DDFAN_OK;
+}
+
+int upload(DBuf& b, const void* src, size_t bytes, hipStream_t st) {
+    HIPCALL(b.reserve(bytes));
+    HIPCALL(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, st));
+    return DDFAN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ddfan_status_string(int s) {
+    switch (s) {
+        case DDFAN_OK: return "ok";
+        case DDFAN_EINVAL: return "invalid argument";
+        case DDFAN_ENOMEM: return "out of memory";
+        case DDFAN_EBOUNDS: return "input out of bounds";
+        case DDFAN_EDEVICE: return "device error";
+        case DDFAN_ENODEV: return "no device";
+        case DDFAN_EBUSY: return "rejected by memory budget";
+        default: return "unknown status";
+    }
+}
+
+const char* ddfan_last_error(void) { return t_err.c_str(); }
+
+int ddfan_version(void) { return 1; }
+
+int ddfan_device_count(int* count) {
+    CHECK(count, DDFAN_EINVAL, "count is NULL");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+    return DDFAN_OK;
+}
+
+int ddfan_set_default_sat_mode(int m) {
+    CHECK(m >= 0 && m <= 2, DDFAN_EINVAL, "sat_mode must be 0, 1 or 2");
+    g_default_sat.store(m);
+    return DDFAN_OK;
+}
+
+// ------------------------------------------------------------ tables
+int ddfan_detector_arc(int DNU, float detCntIdx, float detCellWidth, float sdd, float sid,
+                       float* xds, float* yds) {
+    CHECK(DNU >= 1 && xds && yds, DDFAN_EINVAL, "bad detector arguments");
+    detector_arc(DNU, detCntIdx, detCellWidth, sdd, sid, xds, yds);
+    return DDFAN_OK;
+}
+
+int ddfan_recon_mask(int XN, int YN, unsigned char* mask) {
+    CHECK(XN >= 1 && YN >= 1 && mask, DDFAN_EINVAL, "bad mask arguments");
+    recon_mask(XN, YN, mask);
+    return DDFAN_OK;
+}
+
+int ddfan_subset_views(int numViews, int numOS, int k, int* out, int* count) {
+    CHECK(numViews >= 0 && numOS >= 1 && k >= 0 && k < numOS && count, DDFAN_EINVAL,
+          "bad subset arguments");
+    *count = subset_views(numViews, numOS, k, out);
+    return DDFAN_OK;
+}
+
+int ddfan_rebin_starts(const float* zPos, int SLN, float SD, float SO, int DetHeight,
+                       float PerDetW, float PerDetH, int DefTimes, float pitch, int* start) {
+    CHECK(zPos && start && SLN >= 0 && DefTimes >= 1 && DetHeight >= 1, DDFAN_EINVAL,
+          "bad rebin arguments");
+    const RebinParams rp = rebin_params(SD, SO, DetHeight, PerDetW, PerDetH, DefTimes, pitch);
+    for (int s = 0; s < SLN; ++s) start[s] = rebin_start(zPos[s], rp.h, rp.deltaZ);
+    return DDFAN_OK;
+}
+
+// ------------------------------------------------------------ memory
+int ddfan_dev_alloc(size_t bytes, void** ptr) {
+    CHECK(ptr, DDFAN_EINVAL, "ptr is NULL");
+    int dev;
+    TRY(check_device(&dev));
+    HIPCALL(hipMalloc(ptr, bytes ? bytes : 16));
+    return DDFAN_OK;
+}
+int ddfan_dev_free(void* ptr) {
+    if (ptr) HIPCALL(hipFree(ptr));
+    return DDFAN_OK;
+}
+int ddfan_h2d(void* dst, const void* src, size_t bytes) {
+    HIPCALL(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return DDFAN_OK;
+}
+int ddfan_d2h(void* dst, const void* src, size_t bytes) {
+    HIPCALL(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return DDFAN_OK;
+}
+int ddfan_d2d(void* dst, const void* src, size_t bytes) {
+    HIPCALL(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    return DDFAN_OK;
+}
+int ddfan_dev_mem_info(size_t* fb, size_t* tb) {
+    int dev;
+    TRY(check_device(&dev));
+    HIPCALL(hipMemGetInfo(fb, tb));
+    return DDFAN_OK;
+}
+
+// ------------------------------------------------------------ plan
+int ddfan_plan_footprint(const ddfan_geom* G, int PN, int sat_mode, int slice_chunk, int numOS,
+                         size_t* bytes) {
+    TRY(validate_geom(G));
+    CHECK(bytes && PN >= 1 && sat_mode >= 0 && sat_mode <= 2, DDFAN_EINVAL, "bad arguments");
+    const int ZT = slice_chunk > 0 ? slice_chunk : 16;
+    const int nzc = (G->SLN + ZT - 1) / ZT;
+    size_t b = sizeof(float) * 2 * (size_t)PN * G->SLN + sizeof(float) * 4 * (G->DNU + 1) +
+               (size_t)G->XN * G->YN;
+    b += 2 * sat_vol_bytes(sat_mode, nzc, ZT, G->XN > G->YN ? G->XN : G->YN,
+                           G->XN > G->YN ? G->XN : G->YN);
+    if (numOS > 0) {
+        const int cnt = (PN + numOS - 1) / numOS;
+        const size_t nvox = (size_t)G->XN * G->YN * G->SLN;
+        b += sat_sino_bytes(sat_mode, nzc, ZT, G->DNU, cnt);
+        b += sizeof(float) * ((size_t)cnt * G->DNU * G->SLN + nvox + (size_t)numOS * cnt * G->DNU +
+                              (size_t)numOS * G->XN * G->YN);
+        b += sat_sino_bytes(sat_mode, 1, 1, G->DNU, cnt) + 2 * sat_vol_bytes(sat_mode, 1, 1, G->XN, G->YN);
+    } else {
+        b += sat_sino_bytes(sat_mode, nzc, ZT, G->DNU, PN);
+    }
+    *bytes = b;
+    return DDFAN_OK;
+}
+
+int ddfan_plan_create(const ddfan_geom* G, const float* hangs, int PN, const unsigned char* mask,
+                      int sat_mode, int slice_chunk, ddfan_plan** out) {
+    CHECK(out, DDFAN_EINVAL, "out is NULL");
+    *out = nullptr;
+    TRY(validate_geom(G));
+    CHECK(hangs && PN >= 1, DDFAN_EINVAL, "need PN >= 1 angles");
+    // regularizeAngle subtracts 2*pi until the angle is in range; bounded
+    // magnitudes keep that loop short (and finite).
+    for (size_t i = 0; i < (size_t)PN * G->SLN; ++i)
+        CHECK(isfinite(hangs[i]) && fabsf(hangs[i]) < 1.0e6f, DDFAN_EINVAL,
+              "angle %zu is not finite or exceeds 1e6 rad", i);
+    CHECK(sat_mode >= 0 && sat_mode <= 2, DDFAN_EINVAL, "sat_mode must be 0, 1 or 2");
+    CHECK(slice_chunk == 0 || slice_chunk == 1 || slice_chunk == 2 || slice_chunk == 4 ||
+              slice_chunk == 8 || slice_chunk == 16,
+          DDFAN_EINVAL, "slice_chunk must be 0, 1, 2, 4, 8 or 16");
+    int dev;
+    TRY(check_device(&dev));
+    ddfan_plan* P = new (std::nothrow) ddfan_plan();
+    CHECK(P, DDFAN_ENOMEM, "host allocation failed");
+    P->device = dev;
+    int rc = [&]() -> int {
+        HIPCALL(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
+        const int SLN = G->SLN, DNU = G->DNU;
+        fill_geom(P->g, G, SLN);
+        P->PN = PN;
+        P->mode = sat_mode;
+        P->ZT = choose_slice_chunk(hangs, PN, SLN, slice_chunk);
+        P->nzc = (SLN + P->ZT - 1) / P->ZT;
+        P->h_hangs.assign(hangs, hangs + (size_t)PN * SLN);
+        P->h_cs.resize((size_t)PN * SLN * 2);
+        cossin_table(hangs, (size_t)PN * SLN, P->h_cs.data());
+        std::vector<float> hbx(DNU + 1), hby(DNU + 1);
+        dd_boundaries(DNU + 1, G->xds, hbx.data());
+        dd_boundaries(DNU + 1, G->yds, hby.data());
+        std::vector<uint8_t> hm;
+        if (!mask) hm.assign((size_t)G->XN * G->YN, 1);
+        TRY(upload(P->cs, P->h_cs.data(), sizeof(float) * P->h_cs.size(), P->stream));
+        TRY(upload(P->bx, hbx.data(), sizeof(float) * (DNU + 1), P->stream));
+        TRY(upload(P->by, hby.data(), sizeof(float) * (DNU + 1), P->stream));
+        TRY(upload(P->xds, G->xds, sizeof(float) * DNU, P->stream));
+        TRY(upload(P->yds, G->yds, sizeof(float) * DNU, P->stream));
+        TRY(upload(P->mask, mask ? mask : hm.data(), (size_t)G->XN * G->YN, P->stream));
+        HIPCALL(P->satX.reserve(sat_vol_bytes(sat_mode, P->nzc, P->ZT, G->XN, G->YN)));
+        HIPCALL(P->satY.reserve(sat_vol_bytes(sat_mode, P->nzc, P->ZT, G->YN, G->XN)));
+        HIPCALL(hipStreamSynchronize(P->stream));
+        return DDFAN_OK;
+    }();
+    if (rc != DDFAN_OK) {
+        delete P;
+        return rc;
+    }
+    *out = P;
+    return DDFAN_OK;
+}
+
+int ddfan_plan_destroy(ddfan_plan* P) {
+    if (!P) return DDFAN_OK;
+    (void)hipSetDevice(P->device);
+    delete P;
+    return DDFAN_OK;
+}
+
+int ddfan_plan_query(const ddfan_plan* P, int* zt, int* nzc, int* mode) {
+    CHECK(P, DDFAN_EINVAL, "plan is NULL");
+    if (zt) *zt = P->ZT;
+    if (nzc) *nzc = P->nzc;
+    if (mode) *mode = P->mode;
+    return DDFAN_OK;
+}
+
+int ddfan_plan_sync(ddfan_plan* P) {
+    CHECK(P, DDFAN_EINVAL, "plan is NULL");
+    HIPCALL(hipSetDevice(P->device));
+    HIPCALL(hipStreamSynchronize(P->stream));
+    return DDFAN_OK;
+}
+
+int ddfan_plan_proj(ddfan_plan* P, const float* d_vol, float* d_prj, int first, int stride,
+                    int count) {
+    CHECK(P && d_vol && d_prj, DDFAN_EINVAL, "NULL argument");
+    TRY(check_subset(P->PN, first, stride, count));
+    if (count == 0) return DDFAN_OK;
+    HIPCALL(hipSetDevice(P->device));
+    const Subset sub{first, stride, count};
+    HIPCALL(P->timed(0, [&] {
+        return launch_sat_vol(P->mode, P->ZT, P->g, P->nzc, d_vol, P->mask.as<uint8_t>(), P->satX.p,
+                              P->satY.p, P->stream);
+    }));
+    HIPCALL(P->timed(1, [&] {
+        return launch_proj(P->mode, P->ZT, PROJ_PLAIN, P->g, P->nzc, P->cs.as<float2>(),
+                           P->bx.as<float>(), P->by.as<float>(), P->xds.as<float>(),
+                           P->yds.as<float>(), P->satX.p, P->satY.p, sub, d_prj, nullptr, nullptr,
+                           P->stream);
+    }));
+    return DDFAN_OK;
+}
+
+int ddfan_plan_back(ddfan_plan* P, const float* d_prj, float* d_vol, int first, int stride,
+                    int count) {
+    CHECK(P && d_vol && d_prj, DDFAN_EINVAL, "NULL argument");
+    TRY(check_subset(P->PN, first, stride, count));
+    HIPCALL(hipSetDevice(P->device));
+    const Subset sub{first, stride, count};
+    const size_t need = sat_sino_bytes(P->mode, P->nzc, P->ZT, P->g.DNU, std::max(count, 1));
+    if (need > P->satP.bytes) {
+        HIPCALL(hipStreamSynchronize(P->stream));
+        HIPCALL(P->satP.reserve(need));
+    }
+    HIPCALL(P->timed(2, [&] {
+        return launch_sat_sino(P->mode, P->ZT, P->g.DNU, P->g.SLN, P->nzc, count, d_prj, P->satP.p,
+                               P->stream);
+    }));
+    HIPCALL(P->timed(3, [&] {
+        return launch_back(P->mode, P->ZT, BACK_PLAIN, P->g, P->nzc, P->cs.as<float2>(),
+                           P->mask.as<uint8_t>(), P->satP.p, sub, d_vol, nullptr, nullptr, nullptr,
+                           nullptr, P->stream);
+    }));
+    return DDFAN_OK;
+}
+
+int ddfan_plan_timing(ddfan_plan* P, int enable) {
+    CHECK(P, DDFAN_EINVAL, "plan is NULL");
+    HIPCALL(hipSetDevice(P->device));
+    HIPCALL(hipStreamSynchronize(P->stream));
+    P->clear_events();
+    P->timing = enable != 0;
+    return DDFAN_OK;
+}
+
+int ddfan_plan_timing_read(ddfan_plan* P, int kind, double* total_ms, int* launches) {
+    CHECK(P && total_ms && launches, DDFAN_EINVAL, "NULL argument");
+    HIPCALL(hipSetDevice(P->device));
+    HIPCALL(hipStreamSynchronize(P->stream));
+    double t = 0.0;
+    int n = 0;
+    for (auto& e : P->events) {
+        if (e.kind != kind) continue;
+        float ms = 0.0f;
+        HIPCALL(hipEventElapsedTime(&ms, e.a, e.b));
+        t += ms;
+        ++n;
+    }
+    *total_ms = t;
+    *launches = n;
+    return DDFAN_OK;
+}
+
+int ddfan_plan_trace_proj(ddfan_plan* P, int first, int stride, int count, int32_t* d_trace) {
+    CHECK(P && d_trace, DDFAN_EINVAL, "NULL argument");
+    TRY(check_subset(P->PN, first, stride, count));
+    HIPCALL(hipSetDevice(P->device));
+    HIPCALL(launch_proj_trace(P->g, P->cs.as<float2>(), P->bx.as<float>(), P->by.as<float>(),
+                              Subset{first, stride, count}, d_trace, P->stream));
+    HIPCALL(hipStreamSynchronize(P->stream));
+    return DDFAN_OK;
+}
+
+int ddfan_plan_trace_back(ddfan_plan* P, int first, int stride, int count, int32_t* d_trace) {
+    CHECK(P && d_trace, DDFAN_EINVAL, "NULL argument");
+    TRY(check_subset(P->PN, first, stride, count));
+    HIPCALL(hipSetDevice(P->device));
+    HIPCALL(launch_back_trace(P->g, P->cs.as<float2>(), Subset{first, stride, count}, d_trace,
+                              P->stream));
+    HIPCALL(hipStreamSynchronize(P->stream));
+    return DDFAN_OK;
+}
+
+// ------------------------------------------------------------ OS-SART (device-resident)
+int ddfan_ossart(ddfan_plan* P, const float* d_meas, float* d_x, int numIter, int numOS,
+                 int useFISTA, int flags) {
+    CHECK(P && d_meas && d_x, DDFAN_EINVAL, "NULL argument");
+    CHECK(numIter >= 0, DDFAN_EINVAL, "numIter must be >= 0");
+    CHECK(numOS >= 1 && numOS <= P->PN, DDFAN_EINVAL, "numOS must be in [1, %d]", P->PN);
+    HIPCALL(hipSetDevice(P->device));
+    hipStream_t st = P->stream;
+    const GeomDev& g = P->g;
+    const int DNU = g.DNU, XN = g.XN, YN = g.YN, SLN = g.SLN, mode = P->mode;
+    const size_t nvox = (size_t)XN * YN * SLN, npix = (size_t)XN * YN;
+    std::vector<int> cnt(numOS);
+    for (int k = 0; k < numOS; ++k) cnt[k] = subset_views(P->PN, numOS, k, nullptr);
+    const int maxc = cnt[0];
+    HIPCALL(hipStreamSynchronize(st));
+    HIPCALL(P->rowSum.reserve(sizeof(float) * (size_t)numOS * maxc * DNU));
+    HIPCALL(P->colSum.reserve(sizeof(float) * (size_t)numOS * npix));
+    HIPCALL(P->rbuf.reserve(sizeof(float) * (size_t)maxc * DNU * SLN));
+    HIPCALL(P->satP.reserve(std::max(P->satP.bytes, sat_sino_bytes(mode, P->nzc, P->ZT, DNU, maxc))));
+    HIPCALL(P->momTab.reserve(sizeof(float) * std::max(numIter, 1)));
+    HIPCALL(P->counter.reserve(sizeof(int)));
+    if (useFISTA) HIPCALL(P->xprev.reserve(sizeof(float) * nvox));
+
+    // OS-SART weights from slice 0, replicated to every slice (OSSART_AAPM.m:59-66):
+    // rowSum_k = A_k(mask*1), colSum_k = A_k^T(1), both on a one-slice geometry
+    // with the slice-0 angles.
+    std::vector<float> h_cs0((size_t)P->PN * 2);  // outlives the async upload below
+    {
+        GeomDev g0 = g;
+        g0.SLN = 1;
+        for (int v = 0; v < P->PN; ++v) {
+            h_cs0[2 * v] = P->h_cs[2 * ((size_t)v * SLN)];
+            h_cs0[2 * v + 1] = P->h_cs[2 * ((size_t)v * SLN) + 1];
+        }
+        TRY(upload(P->cs0, h_cs0.data(), sizeof(float) * h_cs0.size(), st));
+        HIPCALL(P->onesV.reserve(sizeof(float) * npix));
+        HIPCALL(P->onesP.reserve(sizeof(float) * (size_t)maxc * DNU));
+        HIPCALL(P->satX0.reserve(sat_vol_bytes(mode, 1, 1, XN, YN)));
+        HIPCALL(P->satY0.reserve(sat_vol_bytes(mode, 1, 1, YN, XN)));
+        HIPCALL(P->satP0.reserve(sat_sino_bytes(mode, 1, 1, DNU, maxc)));
+        HIPCALL(launch_fill(P->onesV.as<float>(), 1.0f, npix, st));
+        HIPCALL(launch_fill(P->onesP.as<float>(), 1.0f, (size_t)maxc * DNU, st));
+        HIPCALL(launch_sat_vol(mode, 1, g0, 1, P->onesV.as<float>(), P->mask.as<uint8_t>(),
+                               P->satX0.p, P->satY0.p, st));
+        for (int k = 0; k < numOS; ++k) {
+            const Subset sub{k, numOS, cnt[k]};
+            HIPCALL(launch_proj(mode, 1, PROJ_PLAIN, g0, 1, P->cs0.as<float2>(), P->bx.as<float>(),
+                                P->by.as<float>(), P->xds.as<float>(), P->yds.as<float>(),
+                                P->satX0.p, P->satY0.p, sub,
+                                P->rowSum.as<float>() + (size_t)k * maxc * DNU, nullptr, nullptr,
+                                st));
+            HIPCALL(launch_sat_sino(mode, 1, DNU, 1, 1, cnt[k], P->onesP.as<float>(), P->satP0.p, st));
+            HIPCALL(launch_back(mode, 1, BACK_PLAIN, g0, 1, P->cs0.as<float2>(),
+                                P->mask.as<uint8_t>(), P->satP0.p, sub,
+                                P->colSum.as<float>() + (size_t)k * npix, nullptr, nullptr, nullptr,
+                                nullptr, st));
+        }
+    }
+    // FISTA momenta (OSSART_AAPM.m:111-115): MATLAB double scalars, applied to
+    // single arrays as single((t0-1)/t1).
+    std::vector<float> mom(std::max(numIter, 1), 0.0f);
+    {
+        double t0 = 1.0;
+        for (int it = 0; it < numIter; ++it) {
+            const double t1 = (1.0 + sqrt(1.0 + 4.0 * t0 * t0)) / 2.0;
+            mom[it] = (float)((t0 - 1.0) / t1);
+            t0 = t1;
+        }
+    }
+    HIPCALL(hipMemcpyAsync(P->momTab.p, mom.data(), sizeof(float) * mom.size(),
+                           hipMemcpyHostToDevice, st));
+    const int zero = 0;
+    HIPCALL(hipMemcpyAsync(P->counter.p, &zero, sizeof(int), hipMemcpyHostToDevice, st));
+    if (useFISTA)
+        HIPCALL(hipMemcpyAsync(P->xprev.p, d_x, sizeof(float) * nvox, hipMemcpyDeviceToDevice, st));
+    HIPCALL(hipStreamSynchronize(st));  // host staging buffers above go out of scope
+
+    const bool timing = P->timing && !(flags & DDFAN_OSSART_GRAPH);
+    auto enqueue_iteration = [&]() -> hipError_t {
+        for (int k = 0; k < numOS; ++k) {
+            const Subset sub{k, numOS, cnt[k]};
+            hipError_t e;
+            auto T = [&](int kind, auto&& f) { return timing ? P->timed(kind, f) : f(); };
+            e = T(0, [&] {
+                return launch_sat_vol(mode, P->ZT, g, P->nzc, d_x, P->mask.as<uint8_t>(), P->satX.p,
+                                      P->satY.p, st);
+            });
+            if (e != hipSuccess) return e;
+            e = T(1, [&] {
+                return launch_proj(mode, P->ZT, PROJ_RATIO, g, P->nzc, P->cs.as<float2>(),
+                                   P->bx.as<float>(), P->by.as<float>(), P->xds.as<float>(),
+                                   P->yds.as<float>(), P->satX.p, P->satY.p, sub,
+                                   P->rbuf.as<float>(), d_meas,
+                                   P->rowSum.as<float>() + (size_t)k * maxc * DNU, st);
+            });
+            if (e != hipSuccess) return e;
+            e = T(2, [&] {
+                return launch_sat_sino(mode, P->ZT, DNU, SLN, P->nzc, cnt[k], P->rbuf.as<float>(),
+                                       P->satP.p, st);
+            });
+            if (e != hipSuccess) return e;
+            const int epi = (useFISTA && k == numOS - 1) ? BACK_UPDATE_FISTA : BACK_UPDATE;
+            e = T(3, [&] {
+                return launch_back(mode, P->ZT, epi, g, P->nzc, P->cs.as<float2>(),
+                                   P->mask.as<uint8_t>(), P->satP.p, sub, d_x,
+                                   P->colSum.as<float>() + (size_t)k * npix, P->xprev.as<float>(),
+                                   P->momTab.as<float>(), P->counter.as<int>(), st);
+            });
+            if (e != hipSuccess) return e;
+        }
+        return launch_counter_inc(P->counter.as<int>(), st);
+    };
+
+    if ((flags & DDFAN_OSSART_GRAPH) && numIter > 0) {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        HIPCALL(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        hipError_t ce = enqueue_iteration();
+        hipError_t ee = hipStreamEndCapture(st, &graph);
+        if (ce != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            HIPCALL(ce);
+        }
+        HIPCALL(ee);
+        hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            HIPCALL(ie);
+        }
+        hipError_t le = hipSuccess;
+        for (int it = 0; it < numIter && le == hipSuccess; ++it) le = hipGraphLaunch(exec, st);
+        hipError_t se = hipStreamSynchronize(st);
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+        HIPCALL(le);
+        HIPCALL(se);
+    } else {
+        for (int it = 0; it < numIter; ++it) HIPCALL(enqueue_iteration());
+        HIPCALL(hipStreamSynchronize(st));
+    }
+    return DDFAN_OK;
+}
+
+// ------------------------------------------------------------ drop-in entry points
+int ddfan_proj(float x0, float y0, int DNU, const float* xds, const float* yds, float imgXCenter,
+               float imgYCenter, const float* hangs, int PN, int XN, int YN, int SLN,
+               const float* hvol, float* hprj, float dx, const unsigned char* mask,
+               const int* startIdx, int nStart) {
+    CHECK(hvol && hprj && hangs, DDFAN_EINVAL, "NULL buffer");
+    TRY(check_start_idx(startIdx, nStart, SLN));
+    if (PN == 0) return DDFAN_OK;
+    ddfan_geom G{x0, y0, DNU, xds, yds, 0.0f, imgXCenter, imgYCenter, XN, YN, SLN, dx};
+    ddfan_plan* P = nullptr;
+    TRY(ddfan_plan_create(&G, hangs, PN, mask, g_default_sat.load(), 0, &P));
+    const size_t nv = (size_t)XN * YN * SLN, np = (size_t)PN * DNU * SLN;
+    int rc = [&]() -> int {
+        DBuf dv, dp;
+        HIPCALL(dv.reserve(sizeof(float) * nv));
+        HIPCALL(dp.reserve(sizeof(float) * np));
+        HIPCALL(hipMemcpyAsync(dv.p, hvol, sizeof(float) * nv, hipMemcpyHostToDevice, P->stream));
+        TRY(ddfan_plan_proj(P, dv.as<float>(), dp.as<float>(), 0, 1, PN));
+        HIPCALL(hipMemcpyAsync(hprj, dp.p, sizeof(float) * np, hipMemcpyDeviceToHost, P->stream));
+        HIPCALL(hipStreamSynchronize(P->stream));
+        return DDFAN_OK;
+    }();
+    ddfan_plan_destroy(P);
+    return rc;
+}
+
+int ddfan_back(float x0, float y0, int DNU, const float* xds, const float* yds, float detCntIdx,
+               float imgXCenter, float imgYCenter, const float* hangs, int PN, int XN, int YN,
+               int SLN, float* hvol, const float* hprj, float dx, const unsigned char* mask,
+               const int* startIdx, int nStart) {
+    CHECK(hvol && hprj && hangs, DDFAN_EINVAL, "NULL buffer");
+    TRY(check_start_idx(startIdx, nStart, SLN));
+    const size_t nv = (size_t)XN * YN * SLN, np = (size_t)PN * DNU * SLN;
+    if (PN == 0) {
+        memset(hvol, 0, sizeof(float) * nv);
+        return DDFAN_OK;
+    }
+    ddfan_geom G{x0, y0, DNU, xds, yds, detCntIdx, imgXCenter, imgYCenter, XN, YN, SLN, dx};
+    ddfan_plan* P = nullptr;
+    TRY(ddfan_plan_create(&G, hangs, PN, mask, g_default_sat.load(), 0, &P));
+    int rc = [&]() -> int {
+        DBuf dv, dp;
+        HIPCALL(dv.reserve(sizeof(float) * nv));
+        HIPCALL(dp.reserve(sizeof(float) * np));
+        HIPCALL(hipMemcpyAsync(dp.p, hprj, sizeof(float) * np, hipMemcpyHostToDevice, P->stream));
+        TRY(ddfan_plan_back(P, dp.as<float>(), dv.as<float>(), 0, 1, PN));
+        HIPCALL(hipMemcpyAsync(hvol, dv.p, sizeof(float) * nv, hipMemcpyDeviceToHost, P->stream));
+        HIPCALL(hipStreamSynchronize(P->stream));
+        return DDFAN_OK;
+    }();
+    ddfan_plan_destroy(P);
+    return rc;
+}
+
+// ------------------------------------------------------------ rebin
+namespace {
+int rebin_common(float* Proj, float* Views, const void* raw, bool rawOnDevice, int rawKind,
+                 float countScale, int layout, long long TotalViews, const float* zPos, int SLN,
+                 float SD, float SO, float BVAngle, int W, int H, float PerDetW, float PerDetH,
+                 int DefTimes, float DetCenterW, float pitch, bool outOnDevice,
+                 unsigned char* d_cut, int32_t* d_idv) {
+    CHECK(Proj && Views && raw && (zPos || SLN == 0), DDFAN_EINVAL, "NULL buffer");
+    CHECK(SLN >= 0 && W >= 1 && W <= 2048 && H >= 2 && DefTimes >= 1 && TotalViews >= 0,
+          DDFAN_EINVAL, "bad rebin shape");
+    CHECK(rawKind == 0 || rawKind == 1, DDFAN_EINVAL, "raw_kind must be 0 or 1");
+    CHECK(layout == 0 || layout == 1, DDFAN_EINVAL, "layout must be 0 or 1");
+    CHECK(isfinite(BVAngle) && fabsf(BVAngle) < 1.0e5f && isfinite(SD) && isfinite(SO) &&
+              SD > 0.0f && SO > 0.0f && PerDetH > 0.0f && PerDetW > 0.0f && pitch > 0.0f,
+          DDFAN_EINVAL, "bad rebin geometry");
+    CHECK((double)TotalViews * (2.0 * 3.141592653589793 / DefTimes) < 1.0e6, DDFAN_EINVAL,
+          "view angles beyond 1e6 rad");
+    if (SLN == 0) return DDFAN_OK;
+    const RebinParams rp = rebin_params(SD, SO, H, PerDetW, PerDetH, DefTimes, pitch);
+    std::vector<int> start(SLN);
+    long long vmin = (long long)1 << 62, vmax = -1;  // 1-based raw views read
+    for (int s = 0; s < SLN; ++s) {
+        start[s] = rebin_start(zPos[s], rp.h, rp.deltaZ);
+        if (start[s] >= 0) {
+            const long long lo = std::max(1, start[s]);
+            const long long hi = (long long)start[s] + DefTimes - 1;
+            CHECK(hi <= TotalViews, DDFAN_EBOUNDS,
+                  "slice %d needs raw views up to %lld but only %lld exist", s, hi, TotalViews);
+            if (hi >= lo) {
+                vmin = std::min(vmin, lo);
+                vmax = std::max(vmax, hi);
+            }
+        }
+    }
+    int dev;
+    TRY(check_device(&dev));
+    std::vector<float> SM(W);
+    rebin_sm_table(W, SO, DetCenterW, rp.PerDetA, SM.data());
+    RebinDev p{};
+    p.SD = SD;
+    p.SO = SO;
+    p.BVAngle = BVAngle;
+    p.PerDetH = PerDetH;
+    p.DetCenterHreal = rp.DetCenterHreal;
+    p.delta = rp.delta;
+    p.deltaZ = rp.deltaZ;
+    p.twopi = (float)(2 * 3.141592653589793);
+    p.W = W;
+    p.H = H;
+    p.DefTimes = DefTimes;
+    p.SLN = SLN;
+    hipStream_t st = nullptr;
+    HIPCALL(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    int rc = [&]() -> int {
+        DBuf dz, dstart, dsm, draw, dP, dV;
+        TRY(upload(dz, zPos, sizeof(float) * SLN, st));
+        TRY(upload(dstart, start.data(), sizeof(int) * SLN, st));
+        TRY(upload(dsm, SM.data(), sizeof(float) * W, st));
+        const size_t esz = rawKind == 0 ? 4 : 2;
+        const size_t perView = (size_t)W * H * esz;
+        const void* rawDev = raw;
+        if (vmax < vmin) {  // no slice reads raw data
+            vmin = 1;
+            vmax = 0;
+        }
+        if (rawOnDevice) {
+            p.viewOffset = 0;
+        } else {
+            p.viewOffset = vmin - 1;
+            const size_t nbytes = perView * (size_t)(vmax - vmin + 1);
+            HIPCALL(draw.reserve(nbytes));
+            if (nbytes)
+                HIPCALL(hipMemcpyAsync(draw.p, (const char*)raw + perView * (size_t)(vmin - 1),
+                                       nbytes, hipMemcpyHostToDevice, st));
+            rawDev = draw.p;
+        }
+        const size_t nOut = (size_t)SLN * DefTimes * W, nViews = (size_t)SLN * DefTimes;
+        float* outP = Proj;
+        float* outV = Views;
+        if (!outOnDevice) {
+            HIPCALL(dP.reserve(sizeof(float) * nOut));
+            HIPCALL(dV.reserve(sizeof(float) * nViews));
+            outP = dP.as<float>();
+            outV = dV.as<float>();
+        }
+        HIPCALL(launch_rebin(p, rawKind, layout, rawDev, countScale, dz.as<float>(),
+                             dstart.as<int>(), dsm.as<float>(), outP, outV, d_cut, d_idv, st));
+        if (!outOnDevice) {
+            HIPCALL(hipMemcpyAsync(Proj, outP, sizeof(float) * nOut, hipMemcpyDeviceToHost, st));
+            HIPCALL(hipMemcpyAsync(Views, outV, sizeof(float) * nViews, hipMemcpyDeviceToHost, st));
+        }
+        HIPCALL(hipStreamSynchronize(st));
+        return DDFAN_OK;
+    }();
+    (void)hipStreamDestroy(st);
+    return rc;
+}
+}  // namespace
+
+int ddfan_helical_to_fan(float* Proj, float* Views, const float* proj, long long TotalViews,
+                         const float* zPos, int SLN, float SD, float SO, float BVAngle,
+                         int DetWidth, int DetHeight, float PerDetW, float PerDetH, int DefTimes,
+                         float DetCenterW, float SpiralPitchFactor) {
+    return rebin_common(Proj, Views, proj, false, 0, 1.0f, 0, TotalViews, zPos, SLN, SD, SO,
+                        BVAngle, DetWidth, DetHeight, PerDetW, PerDetH, DefTimes, DetCenterW,
+                        SpiralPitchFactor, false, nullptr, nullptr);
+}
+
+int ddfan_helical_to_fan_dev(float* d_Proj, float* d_Views, const void* d_raw, int raw_kind,
+                             float count_scale, int layout, long long TotalViews,
+                             const float* zPos, int SLN, float SD, float SO, float BVAngle,
+                             int DetWidth, int DetHeight, float PerDetW, float PerDetH,
+                             int DefTimes, float DetCenterW, float SpiralPitchFactor,
+                             unsigned char* d_cut, int32_t* d_idv) {
+    CHECK(raw_kind == 0 || count_scale > 0.0f, DDFAN_EINVAL, "count_scale must be positive");
+    return rebin_common(d_Proj, d_Views, d_raw, true, raw_kind, count_scale, layout, TotalViews,
+                        zPos, SLN, SD, SO, BVAngle, DetWidth, DetHeight, PerDetW, PerDetH,
+                        DefTimes, DetCenterW, SpiralPitchFactor, true, d_cut, d_idv);
+}
+
+// ------------------------------------------------------------ plain host OS-SART
+int ddfan_ossart_host_plain(float x0, float y0, int DNU, const float* xds, const float* yds,
+                            float detCntIdx, float imgXCenter, float imgYCenter,
+                            const float* hangs, int numViews, int XN, int YN, int SLN,
+                            const float* meas, float* x, float dx, const unsigned char* mask,
+                            int numIter, int numOS, int useFISTA, int sat_mode) {
+    CHECK(hangs && meas && x, DDFAN_EINVAL, "NULL buffer");
+    CHECK(numOS >= 1 && numOS <= numViews && numIter >= 0, DDFAN_EINVAL, "bad iteration counts");
+    CHECK(sat_mode >= 0 && sat_mode <= 2, DDFAN_EINVAL, "bad sat_mode");
+    const int saved = g_default_sat.exchange(sat_mode);
+    struct Restore {
+        int v;
+        ~Restore() { g_default_sat.store(v); }
+    } restore{saved};
+    const size_t npix = (size_t)XN * YN, nvox = npix * SLN;
+    const int maxc = subset_views(numViews, numOS, 0, nullptr);
+    std::vector<std::vector<int>> idx(numOS);
+    std::vector<std::vector<float>> rowSum(numOS), colSum(numOS);
+    std::vector<float> hsub((size_t)maxc * SLN), prj((size_t)maxc * DNU * SLN), tp(nvox);
+    std::vector<float> ones(std::max(nvox, (size_t)maxc * DNU * SLN), 1.0f), lasImg;
+    // weights: full-slice calls, slice 1 kept and replicated (OSSART_AAPM.m:59-66)
+    for (int k = 0; k < numOS; ++k) {
+        const int n = subset_views(numViews, numOS, k, nullptr);
+        idx[k].resize(n);
+        subset_views(numViews, numOS, k, idx[k].data());
+        for (int j = 0; j < n; ++j)
+            for (int z = 0; z < SLN; ++z) hsub[(size_t)j * SLN + z] = hangs[(size_t)idx[k][j] * SLN + z];
+        TRY(ddfan_proj(x0, y0, DNU, xds, yds, imgXCenter, imgYCenter, hsub.data(), n, XN, YN, SLN,
+                       ones.data(), prj.data(), dx, mask, nullptr, 0));
+        rowSum[k].resize((size_t)n * DNU);
+        for (size_t r = 0; r < (size_t)n * DNU; ++r) rowSum[k][r] = prj[r * SLN];
+        TRY(ddfan_back(x0, y0, DNU, xds, yds, detCntIdx, imgXCenter, imgYCenter, hsub.data(), n, XN,
+                       YN, SLN, tp.data(), ones.data(), dx, mask, nullptr, 0));
+        colSum[k].resize(npix);
+        for (size_t p = 0; p < npix; ++p) colSum[k][p] = tp[p * SLN];
+    }
+    double t0 = 1.0;
+    for (int it = 0; it < numIter; ++it) {
+        if (useFISTA) lasImg.assign(x, x + nvox);
+        for (int k = 0; k < numOS; ++k) {
+            const int n = (int)idx[k].size();
+            for (int j = 0; j < n; ++j)
+                for (int z = 0; z < SLN; ++z) hsub[(size_t)j * SLN + z] = hangs[(size_t)idx[k][j] * SLN + z];
+            TRY(ddfan_proj(x0, y0, DNU, xds, yds, imgXCenter, imgYCenter, hsub.data(), n, XN, YN,
+                           SLN, x, prj.data(), dx, mask, nullptr, 0));
+            for (int j = 0; j < n; ++j)
+                for (int u = 0; u < DNU; ++u) {
+                    const float w = rowSum[k][(size_t)j * DNU + u];
+                    if (w == 0.0f) continue;
+                    const float* b = meas + ((size_t)idx[k][j] * DNU + u) * SLN;
+                    float* r = prj.data() + ((size_t)j * DNU + u) * SLN;
+                    for (int z = 0; z < SLN; ++z) r[z] = (b[z] - r[z]) / w;
+                }
+            TRY(ddfan_back(x0, y0, DNU, xds, yds, detCntIdx, imgXCenter, imgYCenter, hsub.data(), n,
+                           XN, YN, SLN, tp.data(), prj.data(), dx, mask, nullptr, 0));
+            for (size_t p = 0; p < npix; ++p) {
+                const float w = colSum[k][p];
+                for (int z = 0; z < SLN; ++z) {
+                    float t = tp[p * SLN + z];
+                    if (w != 0.0f) t = t / w;
+                    x[p * SLN + z] = x[p * SLN + z] + t;
+                }
+            }
+        }
+        if (useFISTA) {
+            const double t1 = (1.0 + sqrt(1.0 + 4.0 * t0 * t0)) / 2.0;
+            const float m = (float)((t0 - 1.0) / t1);
+            for (size_t o = 0; o < nvox; ++o) x[o] = x[o] + m * (x[o] - lasImg[o]);
+            t0 = t1;
+        }
+    }
+    return DDFAN_OK;
+}
+
+}  // extern "C"

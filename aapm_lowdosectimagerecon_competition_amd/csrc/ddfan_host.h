@@ -1,0 +1,55 @@
+// ddfan_host.h -- host-side geometry tables of the DD fan-beam path.
+//
+// Everything here runs on the host in FP32 (or the reference's own FP64
+// intermediate steps) with the platform libm, exactly as the reference
+// computes it on the host, so the tables handed to the kernels are the ones
+// the reference would hand to its kernels.
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include <vector>
+
+namespace ddf {
+
+// DD2MutiSlices.m:61-76: equiangular arc detector centres.
+void detector_arc(int DNU, float detCntIdx, float detCellWidth, float sdd, float sid, float* xds,
+                  float* yds);
+
+// utilities.cuh:143-148 regularizeAngle<float>.
+float regularize_angle(float c);
+
+// utilities.cuh:218-227: (cosf, sinf) of the regularised angle, interleaved.
+void cossin_table(const float* hangs, size_t n, float* cs);
+
+// utilities.cuh:268-287 DD3Boundaries<float>.
+void dd_boundaries(int nb, const float* centres, float* bnd);
+
+// MultiSlicesBackGPU.cu:211,226-227.
+void back_scalars(float x0, float y0, const float* xds, const float* yds, float* S2D,
+                  float* dbeta);
+
+// reconscript.m:28-38.
+void recon_mask(int XN, int YN, uint8_t* mask);
+
+// OSSART_AAPM.m:60 (0-based): views k, k+numOS, ... < numViews.
+int subset_views(int numViews, int numOS, int k, int* out);
+
+// HelicalToFanFunc_mex.cpp:160-164.
+struct RebinParams {
+    float PerDetA, delta, DetCenterHreal, h, deltaZ;
+};
+RebinParams rebin_params(float SD, float SO, int DetHeight, float PerDetW, float PerDetH,
+                         int DefTimes, float pitch);
+// HelicalToFanFunc_mex.cpp:71.
+int rebin_start(float z, float h, float deltaZ);
+// HelicalToFanFunc_mex.cpp:84-85: SM[j-1] = SO*cosf((j - DetCenterW)*PerDetA).
+void rebin_sm_table(int W, float SO, float DetCenterW, float PerDetA, float* SM);
+
+// Largest slice chunk ZT in {16,8,4,2,1} such that every chunk of ZT
+// consecutive slices has bit-identical angles in every view and the padding
+// (nzc*ZT - SLN) stays within an eighth of SLN.  limit caps ZT (0: none).
+int choose_slice_chunk(const float* hangs, int PN, int SLN, int limit);
+
+}  // namespace ddf

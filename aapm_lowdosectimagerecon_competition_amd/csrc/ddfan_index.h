@@ -1,0 +1,138 @@
+// ddfan_index.h -- FP32 index path shared by the GPU kernels and the host.
+//
+// Every expression here decides an integer (slab branch, texel index, rebin
+// detector row) or feeds one, so it is written as a sequence of IEEE-754
+// single operations that round identically on gfx950 and on the host:
+// + - * / sqrtf fmaf floorf only, no FMA contraction (the library is compiled
+// with -ffp-contract=off; fmaf appears only where it is written out), and
+// correctly rounded division and square root.  Where the reference relied on
+// CUDA fast-math intrinsics (rsqrtf, asinf, approximate divide) the
+// correctly rounded operation or the Cephes arcsine below is used instead.
+#pragma once
+
+#include <math.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define DDF_HD __host__ __device__ __forceinline__
+#else
+#define DDF_HD static inline
+#endif
+
+namespace ddf {
+
+// Single-precision arcsine from the Cephes method (S. L. Moshier, Cephes Math
+// Library asinf): odd minimax polynomial on [0, 0.5] and
+// asin(x) = pi/2 - 2 asin(sqrt((1-x)/2)) above.  Horner steps in fmaf.
+DDF_HD float asinf_cephes(float xx) {
+    float a = fabsf(xx);
+    if (a > 1.0f) return __builtin_nanf("");
+    if (a < 1.0e-4f) return xx;
+    float x, z;
+    bool flag = a > 0.5f;
+    if (flag) {
+        z = 0.5f * (1.0f - a);
+        x = sqrtf(z);
+    } else {
+        x = a;
+        z = x * x;
+    }
+    float p = fmaf(4.2163199048e-2f, z, 2.4181311049e-2f);
+    p = fmaf(p, z, 4.5470025998e-2f);
+    p = fmaf(p, z, 7.4953002686e-2f);
+    p = fmaf(p, z, 1.6666752422e-1f);
+    z = fmaf(p * z, x, x);
+    if (flag) {
+        z = z + z;
+        z = 1.5707963267948966f - z;
+    }
+    return xx < 0.0f ? -z : z;
+}
+
+// Texel index of an unnormalised, texel-centred linear fetch at coordinate c:
+// floor(c - 0.5), limited to [-1, N] (anything outside reads the clamped edge).
+DDF_HD int texel(float c, int N) {
+    float f = floorf(c - 0.5f);
+    f = (f >= -1.0f) ? f : -1.0f;  // NaN -> -1
+    f = (f > (float)N) ? (float)N : f;
+    return (int)f;
+}
+
+// Projector ray set-up for one (view, slice, detector)
+// (MultiSlicesProjGPU.cu:122-158,176-181).
+struct ProjRay {
+    float aL, bL, fL;  // boundary ray L: r(obj) = (obj - aL) * fL + bL
+    float aR, bR, fR;
+    bool xslab;        // |cos| <= |sin|: slabs are image columns, SAT along y
+};
+
+DDF_HD ProjRay proj_ray(float x0, float y0, float c, float s, float bxL, float byL, float bxR,
+                        float byR) {
+    const float srcx = x0 * c - y0 * s;
+    const float srcy = x0 * s + y0 * c;
+    const float dLx = bxL * c - byL * s, dLy = bxL * s + byL * c;
+    const float dRx = bxR * c - byR * s, dRy = bxR * s + byR * c;
+    ProjRay r;
+    r.xslab = fabsf(c) <= fabsf(s);
+    if (r.xslab) {
+        r.fL = (dLy - srcy) / (dLx - srcx);
+        r.fR = (dRy - srcy) / (dRx - srcx);
+        r.aL = dLx; r.bL = dLy; r.aR = dRx; r.bR = dRy;
+    } else {
+        r.fL = (dLx - srcx) / (dLy - srcy);
+        r.fR = (dRx - srcx) / (dRy - srcy);
+        r.aL = dLy; r.bL = dLx; r.aR = dRy; r.bR = dRx;
+    }
+    return r;
+}
+
+// Boundary position on slab ii and its texture coordinate
+// (MultiSlicesProjGPU.cu:161-168): obj = (ii - ocSlab)*dx,
+// r = (obj - a)*f + b, coord = r*invdx + ocK + 1.
+DDF_HD float slab_obj(int ii, float ocSlab, float dx) { return ((float)ii - ocSlab) * dx; }
+DDF_HD float slab_r(float obj, float a, float f, float b) { return (obj - a) * f + b; }
+DDF_HD float slab_coord(float r, float invdx, float ocK) { return (r * invdx + ocK) + 1.0f; }
+
+// Backprojector per (voxel, view) detector coordinates
+// (MultiSlicesBackGPU.cu:155-176).  vx, vy: voxel centre in mm.
+struct BackSample {
+    float idxL, idxR;  // texture coordinates along the detector SAT
+    float ddv, l2, l;  // for the continuous weight dx/ddv*sqrt(l2)
+};
+
+DDF_HD BackSample back_sample(float vx, float vy, float c, float s, float x0, float y0,
+                              float inv_sid, float invdb, float detCntIdx, float dx) {
+    const float srcx = x0 * c - y0 * s;
+    const float srcy = x0 * s + y0 * c;
+    const float ddx = vx - srcx, ddy = vy - srcy;
+    BackSample b;
+    b.l2 = ddx * ddx + ddy * ddy;
+    b.l = 1.0f / sqrtf(b.l2);
+    const float alpha = asinf_cephes(((srcy * ddx - srcx * ddy) * inv_sid) * b.l);
+    b.ddv = (fabsf(srcx) > fabsf(srcy)) ? ddx : ddy;
+    const float dA = ((b.ddv / b.l2) * dx) * 0.5f;
+    b.idxL = ((alpha - dA) * invdb + detCntIdx) + 1.0f;
+    b.idxR = ((alpha + dA) * invdb + detCntIdx) + 1.0f;
+    return b;
+}
+
+// Helical-to-fan detector row for channel table value SM
+// (HelicalToFanFunc_mex.cpp:85-90).  Returns dv; the row is floor(dv).
+DDF_HD float rebin_dv(float z, float zz, float SM, float SD, float DetCenterHreal, float PerDetH,
+                      float* v_out) {
+    const float tanAngV = (z - zz) / SM;
+    const float v = SD * tanAngV;
+    *v_out = v;
+    // (float)((double)q + 1.0) equals the float sum q + 1.0f: q + 1 is exact
+    // in double, so both round once from the same exact value.
+    return (DetCenterHreal + v) / PerDetH + 1.0f;
+}
+
+// AAPM::mod<float> (HelicalToFanFunc_mex.cpp:27-34).
+DDF_HD float mod_f(float lambda, float regV) {
+    float v = lambda;
+    while (v > regV) v -= regV;
+    return v;
+}
+
+}  // namespace ddf

@@ -1,0 +1,86 @@
+// ddfan_internal.h -- device-side data descriptors and kernel launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace ddf {
+
+// Summed-area-table precision variants.
+enum SatMode : int {
+    SAT_F32 = 0,       // FP32 sequential prefix (the reference's arithmetic)
+    SAT_TWOFLOAT = 1,  // FP32 compensated (TwoSum) prefix stored as hi/lo pairs
+    SAT_F64 = 2,       // FP64 prefix stored as double
+};
+
+inline size_t sat_elem_bytes(int mode) { return mode == SAT_F32 ? 4 : 8; }
+
+// Scalars every kernel needs, passed by value.
+struct GeomDev {
+    float x0, y0, dx, invdx;
+    float ocxP, ocyP;  // projector image-centre indices (MultiSlicesProjGPU.cu:225-226)
+    float ocxB, ocyB;  // backprojector image-centre indices (MultiSlicesBackGPU.cu:198-200)
+    float inv_sid;     // 1/|source| (MultiSlicesBackGPU.cu:141)
+    float invdb;       // 1/dbeta (MultiSlicesBackGPU.cu:150)
+    float detCntIdx;
+    int XN, YN, SLN, DNU;
+};
+
+// Views first, first+stride, ... (count of them) of the plan's angle table.
+struct Subset {
+    int first, stride, count;
+};
+
+// Summed-area tables are stored as [chunk][row][k][w][zt]: `zt` runs over the
+// ZT slices of a slice chunk that share one geometry, `w` is 1 (f32, f64) or
+// 2 (hi, lo) words, k = 0..N is the prefix index, rows are slabs (volume) or
+// views (sinogram).
+
+// Fused epilogues.
+enum ProjEpi : int { PROJ_PLAIN = 0, PROJ_RATIO = 1 };
+enum BackEpi : int { BACK_PLAIN = 0, BACK_UPDATE = 1, BACK_UPDATE_FISTA = 2 };
+
+hipError_t launch_sat_vol(int mode, int ZT, const GeomDev& g, int nzc, const float* vol,
+                          const uint8_t* mask, void* satX, void* satY, hipStream_t st);
+
+hipError_t launch_sat_sino(int mode, int ZT, int DNU, int SLN, int nzc, int PNs, const float* r,
+                           void* P, hipStream_t st);
+
+hipError_t launch_proj(int mode, int ZT, int epi, const GeomDev& g, int nzc, const float2* cossin,
+                       const float* bx, const float* by, const float* xds, const float* yds,
+                       const void* satX, const void* satY, Subset sub, float* out,
+                       const float* meas, const float* rowSum, hipStream_t st);
+
+hipError_t launch_back(int mode, int ZT, int epi, const GeomDev& g, int nzc, const float2* cossin,
+                       const uint8_t* mask, const void* P, Subset sub, float* vol,
+                       const float* colSum, float* xprev, const float* momTab,
+                       const int* iterCounter, hipStream_t st);
+
+hipError_t launch_counter_inc(int* counter, hipStream_t st);
+hipError_t launch_fill(float* p, float v, size_t n, hipStream_t st);
+
+// Index traces for the bit-exactness tests (same device functions as the
+// kernels above): projector [j][u][z][slab][2], backprojector [y][x][z][j][2].
+hipError_t launch_proj_trace(const GeomDev& g, const float2* cossin, const float* bx,
+                             const float* by, Subset sub, int32_t* trace, hipStream_t st);
+hipError_t launch_back_trace(const GeomDev& g, const float2* cossin, Subset sub, int32_t* trace,
+                             hipStream_t st);
+
+// Helical-to-fan rebin parameters (HelicalToFanFunc_mex.cpp:160-164).
+struct RebinDev {
+    float SD, SO, BVAngle, PerDetH, DetCenterHreal, delta, deltaZ, twopi;
+    int W, H, DefTimes, SLN;
+    long long viewOffset;  // raw view index (0-based) held at raw[0]
+};
+
+// rawKind 0: float raw line integrals; 1: uint16 counts converted to
+// -log(max(c,1)/(gain*I0)) on the fly.  outLayout 0: reference layout
+// Proj[(s*DefTimes+v)*W+j], Views[s*DefTimes+v]; 1: sinogram layout
+// prj[(v*W+j)*SLN+s], hangs[v*SLN+s].
+hipError_t launch_rebin(const RebinDev& p, int rawKind, int outLayout, const void* raw,
+                        float countScale, const float* zPos, const int* start, const float* SM,
+                        float* Proj, float* Views, uint8_t* cut, int32_t* idvTrace,
+                        hipStream_t st);
+
+}  // namespace ddf

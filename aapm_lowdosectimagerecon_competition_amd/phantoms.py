@@ -1,0 +1,117 @@
+"""Seeded synthetic phantoms and noise for the benchmark workloads.
+
+Attenuation is in 1/mm.  Volumes are returned in the C-order layout
+vol[y, x, z] (== the reference's vol[(y*XN + x)*SLN + z]); sinograms as
+prj[v, u, z].
+"""
+from __future__ import annotations
+
+import numpy as np
+
+# Modified Shepp-Logan (Toft): value, semi-axes a, b, centre x, y, angle (deg)
+_SHEPP = [
+    (1.0, 0.69, 0.92, 0.0, 0.0, 0.0),
+    (-0.8, 0.6624, 0.8740, 0.0, -0.0184, 0.0),
+    (-0.2, 0.11, 0.31, 0.22, 0.0, -18.0),
+    (-0.2, 0.16, 0.41, -0.22, 0.0, 18.0),
+    (0.1, 0.21, 0.25, 0.0, 0.35, 0.0),
+    (0.1, 0.046, 0.046, 0.0, 0.1, 0.0),
+    (0.1, 0.046, 0.046, 0.0, -0.1, 0.0),
+    (0.1, 0.046, 0.023, -0.08, -0.605, 0.0),
+    (0.1, 0.023, 0.023, 0.0, -0.606, 0.0),
+    (0.1, 0.023, 0.046, 0.06, -0.605, 0.0),
+]
+
+
+def _grid(XN, YN, dx, ss=2):
+    """Supersampled pixel-centre coordinates (mm), shape (YN, XN, ss*ss)."""
+    o = (np.arange(ss) + 0.5) / ss - 0.5
+    xs = ((np.arange(XN) - (XN - 1) / 2.0)[:, None] + o[None, :]) * dx  # (XN, ss)
+    ys = ((np.arange(YN) - (YN - 1) / 2.0)[:, None] + o[None, :]) * dx
+    X = np.broadcast_to(xs[None, :, None, :], (YN, XN, ss, ss)).reshape(YN, XN, ss * ss)
+    Y = np.broadcast_to(ys[:, None, :, None], (YN, XN, ss, ss)).reshape(YN, XN, ss * ss)
+    return X, Y
+
+
+def _ellipse_mask(X, Y, cx, cy, a, b, ang):
+    c, s = np.cos(ang), np.sin(ang)
+    xr = (X - cx) * c + (Y - cy) * s
+    yr = -(X - cx) * s + (Y - cy) * c
+    return (xr / a) ** 2 + (yr / b) ** 2 <= 1.0
+
+
+def shepp_logan_2d(XN, YN, dx, fov_radius, mu_water=0.0195, n_random=20, seed=0,
+                   mu_range=(0.01, 0.03), ss=2):
+    """Shepp-Logan scaled to water = mu_water plus ``n_random`` seeded random
+    ellipses of attenuation U(mu_range), replacing the values they cover."""
+    X, Y = _grid(XN, YN, dx, ss)
+    img = np.zeros(X.shape, np.float64)
+    R = fov_radius
+    for v, a, b, cx, cy, ang in _SHEPP:
+        img += v * mu_water * _ellipse_mask(X, Y, cx * R, cy * R, a * R, b * R, np.deg2rad(ang))
+    rng = np.random.default_rng(seed)
+    for _ in range(n_random):
+        r = rng.uniform(0.0, 0.6) * R
+        t = rng.uniform(0, 2 * np.pi)
+        a = rng.uniform(0.02, 0.12) * R
+        b = rng.uniform(0.02, 0.12) * R
+        m = _ellipse_mask(X, Y, r * np.cos(t), r * np.sin(t), a, b, rng.uniform(0, np.pi))
+        img[m] = rng.uniform(*mu_range)
+    return img.mean(axis=2).astype(np.float32)
+
+
+def ellipsoid_phantom(XN, YN, SLN, dx, dz=None, n=24, seed=1, mu_range=(0.0, 0.03),
+                      fov_mask=True):
+    """3-D seeded random-ellipsoid phantom vol[y, x, z] inside a circular FOV:
+    an elliptical body of water (0.019/mm) holding ``n`` random ellipsoids of
+    attenuation U(mu_range).  Pixel-centre sampling, bounding-box limited."""
+    dz = dx if dz is None else dz
+    rng = np.random.default_rng(seed)
+    xs = (np.arange(XN) - (XN - 1) / 2.0) * dx
+    ys = (np.arange(YN) - (YN - 1) / 2.0) * dx
+    zc = (np.arange(SLN) - (SLN - 1) / 2.0) * dz
+    R = 0.5 * min(XN, YN) * dx
+    H = 0.5 * SLN * dz
+    X, Y = np.meshgrid(xs, ys)
+    body = (((X / (0.85 * R)) ** 2 + (Y / (0.7 * R)) ** 2) <= 1.0).astype(np.float32) * 0.019
+    vol = np.repeat(body[:, :, None], SLN, axis=2)
+    for _ in range(n):
+        cx, cy = rng.uniform(-0.5, 0.5, 2) * R
+        cz = rng.uniform(-1, 1) * H
+        a, b = rng.uniform(0.05, 0.25, 2) * R
+        c = rng.uniform(0.2, 1.0) * max(H, dz)
+        ang = rng.uniform(0, np.pi)
+        mu = np.float32(rng.uniform(*mu_range))
+        r = max(a, b)
+        x0, x1 = np.searchsorted(xs, cx - r), np.searchsorted(xs, cx + r)
+        y0, y1 = np.searchsorted(ys, cy - r), np.searchsorted(ys, cy + r)
+        if x1 <= x0 or y1 <= y0:
+            continue
+        Xs, Ys = X[y0:y1, x0:x1], Y[y0:y1, x0:x1]
+        co, si = np.cos(ang), np.sin(ang)
+        xr = (Xs - cx) * co + (Ys - cy) * si
+        yr = -(Xs - cx) * si + (Ys - cy) * co
+        q = (xr / a) ** 2 + (yr / b) ** 2
+        for z in range(SLN):
+            w = 1.0 - ((zc[z] - cz) / c) ** 2
+            if w > 0:
+                sub = vol[y0:y1, x0:x1, z]
+                sub[q <= w] = mu
+    if fov_mask:
+        vol *= (np.sqrt(X ** 2 + Y ** 2) <= 0.95 * R)[:, :, None]
+    return np.ascontiguousarray(vol, np.float32)
+
+
+def poisson_sinogram(prj, dx, I0, seed=2):
+    """Noisy sinogram in the projector's units (line integral / dx):
+    counts ~ Poisson(I0 exp(-prj*dx)), p = -ln(max(counts, 1)/I0)/dx."""
+    rng = np.random.default_rng(seed)
+    lam = I0 * np.exp(-np.asarray(prj, np.float64) * dx)
+    c = rng.poisson(lam)
+    return (-np.log(np.maximum(c, 1) / I0) / dx).astype(np.float32)
+
+
+def uniform_angles(PN, SLN, start=0.0, span=2 * np.pi):
+    """hangs[v, z] = start + span*v/PN for every slice (float32)."""
+    a = (start + span * np.arange(PN) / PN).astype(np.float32)
+    return np.ascontiguousarray(np.repeat(a[:, None], SLN, axis=1))

@@ -1,0 +1,157 @@
+"""Device-resident plan: geometry tables, summed-area-table buffers and the
+OS-SART state of one reconstruction, kept in HBM across calls."""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _native as N
+from .geometry import DetectorGeometry
+
+
+def _sat_mode(m) -> int:
+    if isinstance(m, str):
+        return N.SAT_MODES[m]
+    return int(m)
+
+
+class Plan:
+    """Wraps ``ddfan_plan``.  ``hangs`` is (PN, SLN) float32 (hangs[v, z])."""
+
+    def __init__(self, geom: DetectorGeometry, hangs: np.ndarray, mask=None, sat_mode="f32",
+                 slice_chunk: int = 0):
+        self.geom = geom
+        self.hangs = np.ascontiguousarray(hangs, np.float32)
+        assert self.hangs.ndim == 2 and self.hangs.shape[1] == geom.SLN, self.hangs.shape
+        self.PN = self.hangs.shape[0]
+        self.mask = None if mask is None else np.ascontiguousarray(mask, np.uint8)
+        if self.mask is not None:
+            assert self.mask.shape == (geom.YN, geom.XN), self.mask.shape
+        self._gs = geom.as_struct()
+        h = ctypes.c_void_p()
+        N.call("ddfan_plan_create", ctypes.byref(self._gs), N.ptr(self.hangs), self.PN,
+               N.ptr(self.mask), _sat_mode(sat_mode), int(slice_chunk), ctypes.byref(h))
+        self.handle = h.value
+        zt, nzc, mode = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        N.call("ddfan_plan_query", self.handle, ctypes.byref(zt), ctypes.byref(nzc),
+               ctypes.byref(mode))
+        self.slice_chunk, self.n_chunks, self.sat_mode = zt.value, nzc.value, mode.value
+
+    # shapes
+    @property
+    def vol_shape(self):
+        g = self.geom
+        return (g.YN, g.XN, g.SLN)
+
+    def sino_shape(self, count=None):
+        g = self.geom
+        return (self.PN if count is None else count, g.DNU, g.SLN)
+
+    # operators on device buffers
+    def proj(self, d_vol, d_prj, first=0, stride=1, count=None):
+        count = self.PN if count is None else count
+        N.call("ddfan_plan_proj", self.handle, N.ptr(d_vol), N.ptr(d_prj), first, stride, count)
+
+    def back(self, d_prj, d_vol, first=0, stride=1, count=None):
+        count = self.PN if count is None else count
+        N.call("ddfan_plan_back", self.handle, N.ptr(d_prj), N.ptr(d_vol), first, stride, count)
+
+    def sync(self):
+        N.call("ddfan_plan_sync", self.handle)
+
+    def ossart(self, d_meas, d_x, num_iter, num_os, use_fista=1, graph=True):
+        N.call("ddfan_ossart", self.handle, N.ptr(d_meas), N.ptr(d_x), int(num_iter), int(num_os),
+               int(use_fista), N.OSSART_GRAPH if graph else 0)
+
+    # host-array conveniences
+    def proj_host(self, vol, first=0, stride=1, count=None):
+        count = self.PN if count is None else count
+        v = np.ascontiguousarray(vol, np.float32)
+        assert v.shape == self.vol_shape
+        out = np.empty(self.sino_shape(count), np.float32)
+        with N.DeviceBuffer.from_array(v) as dv, N.DeviceBuffer(out.nbytes) as dp:
+            self.proj(dv, dp, first, stride, count)
+            self.sync()
+            dp.download(out)
+        return out
+
+    def back_host(self, prj, first=0, stride=1, count=None):
+        count = self.PN if count is None else count
+        p = np.ascontiguousarray(prj, np.float32)
+        assert p.shape == self.sino_shape(count), (p.shape, self.sino_shape(count))
+        out = np.empty(self.vol_shape, np.float32)
+        with N.DeviceBuffer.from_array(p) as dp, N.DeviceBuffer(out.nbytes) as dv:
+            self.back(dp, dv, first, stride, count)
+            self.sync()
+            dv.download(out)
+        return out
+
+    def ossart_host(self, meas, x0, num_iter, num_os, use_fista=1, graph=True):
+        b = np.ascontiguousarray(meas, np.float32)
+        assert b.shape == self.sino_shape()
+        x = np.array(x0, np.float32, order="C")
+        assert x.shape == self.vol_shape
+        with N.DeviceBuffer.from_array(b) as db, N.DeviceBuffer.from_array(x) as dx:
+            self.ossart(db, dx, num_iter, num_os, use_fista, graph)
+            dx.download(x)
+        return x
+
+    # timing and traces
+    def timing(self, enable=True):
+        N.call("ddfan_plan_timing", self.handle, 1 if enable else 0)
+
+    def timing_read(self, kind):
+        """(total ms, launches) of kernel kind 0 volume SAT, 1 projector,
+        2 sinogram SAT, 3 backprojector since timing was (re)enabled."""
+        t, n = ctypes.c_double(), ctypes.c_int()
+        N.call("ddfan_plan_timing_read", self.handle, int(kind), ctypes.byref(t), ctypes.byref(n))
+        return t.value, n.value
+
+    def trace_proj(self, first=0, stride=1, count=None):
+        g = self.geom
+        count = self.PN if count is None else count
+        shape = (count, g.DNU, g.SLN, max(g.XN, g.YN), 2)
+        out = np.zeros(shape, np.int32)
+        with N.DeviceBuffer(out.nbytes) as d:
+            d.upload(out)
+            N.call("ddfan_plan_trace_proj", self.handle, first, stride, count, N.ptr(d))
+            d.download(out)
+        return out
+
+    def trace_back(self, first=0, stride=1, count=None):
+        g = self.geom
+        count = self.PN if count is None else count
+        out = np.zeros((g.YN, g.XN, g.SLN, count, 2), np.int32)
+        with N.DeviceBuffer(out.nbytes) as d:
+            d.upload(out)
+            N.call("ddfan_plan_trace_back", self.handle, first, stride, count, N.ptr(d))
+            d.download(out)
+        return out
+
+    def close(self):
+        if getattr(self, "handle", None):
+            N.call("ddfan_plan_destroy", self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None) and N._lib is not None:
+                N._lib.ddfan_plan_destroy(self.handle)
+                self.handle = None
+        except Exception:  # pragma: no cover
+            pass
+
+
+def plan_footprint(geom: DetectorGeometry, PN, sat_mode="f32", slice_chunk=0, num_os=0) -> int:
+    gs = geom.as_struct()
+    b = ctypes.c_size_t()
+    N.call("ddfan_plan_footprint", ctypes.byref(gs), int(PN), _sat_mode(sat_mode),
+           int(slice_chunk), int(num_os), ctypes.byref(b))
+    return b.value

@@ -1,0 +1,205 @@
+/*
+ * ddfan.h -- C ABI of the MI355X distance-driven fan-beam reconstruction path.
+ *
+ * Drop-in replacements for the reference's native entry points
+ * (Angus1996/AAPM_LowDoseCTImageRecon_competition, AAPM_script/):
+ *
+ *   ddfan_proj            <- DD2ProjMultiGPUs_CPP17  MultiSlicesProjGPU.cu:486-494
+ *                            (MEX argument list MultiSlicesProjGPU.cu:550-575)
+ *   ddfan_back            <- DD2BackMultiGPUs         MultiSlicesBackGPU.cu:349-359
+ *                            (MEX argument list MultiSlicesBackGPU.cu:409-436)
+ *   ddfan_helical_to_fan  <- HelicalToFan             HelicalToFanFunc_mex.cpp:142-158
+ *                            (MEX argument list HelicalToFanFunc_mex.cpp:190-232)
+ *
+ * plus the device-resident plan API that the OS-SART driver
+ * (OSSART_AAPM.m:49-117), the batched launch and the request loop use.
+ *
+ * Conventions (same buffers and layouts as the reference):
+ *   volume   vol[(y*XN + x)*SLN + z]          (MATLAB single(SLN, XN, YN))
+ *   sinogram prj[(v*DNU + u)*SLN + z]         (MATLAB single(SLN, DNU, PN))
+ *   angles   hangs[v*SLN + z]                 (MATLAB single(SLN, PN)), radians
+ *   mask     mask[y*XN + x], uint8; only XN*YN bytes are read (the reference's
+ *            [mask, mask'] second half is never read).  NULL means all ones.
+ *   raw helical proj[(view*DetWidth + ch)*DetHeight + row]
+ *   rebinned Proj[(s*DefTimes + v)*DetWidth + ch], Views[s*DefTimes + v]
+ *
+ * Every entry point returns an int status (DDFAN_OK = 0, negative on error)
+ * instead of calling exit(); ddfan_last_error() gives a message for the
+ * calling thread.  Host-buffer entry points are synchronous, re-entrant and
+ * thread-safe; they never reset the device.  Output buffers are caller-owned.
+ */
+#ifndef DDFAN_H
+#define DDFAN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DDFAN_OK 0
+#define DDFAN_EINVAL (-1)  /* invalid argument or shape */
+#define DDFAN_ENOMEM (-2)  /* device or host allocation failed */
+#define DDFAN_EBOUNDS (-3) /* input would be read out of bounds */
+#define DDFAN_EDEVICE (-4) /* HIP runtime error */
+#define DDFAN_ENODEV (-5)  /* no usable GPU */
+#define DDFAN_EBUSY (-6)   /* request rejected by the HBM budget */
+
+/* Summed-area-table precision. */
+#define DDFAN_SAT_F32 0      /* FP32 sequential prefix: the reference's arithmetic */
+#define DDFAN_SAT_TWOFLOAT 1 /* compensated FP32 (TwoSum) prefix, hi/lo storage */
+#define DDFAN_SAT_F64 2      /* FP64 prefix */
+
+const char* ddfan_status_string(int status);
+const char* ddfan_last_error(void);
+int ddfan_version(void);
+/* Number of visible GPUs (the reference's gpuDeviceCount_mex.cu:7-9). */
+int ddfan_device_count(int* count);
+
+/* Precision used by the drop-in entry points below (default DDFAN_SAT_F32). */
+int ddfan_set_default_sat_mode(int sat_mode);
+
+/* ---------------------------------------------------------------- drop-ins */
+
+/* DD projector.  Same argument order and meaning as DD2ProjMultiGPUs_CPP17;
+ * nStart is the length of startIdx (the reference reads one entry per GPU).
+ * The slice split is validated (startIdx[0] == 0, non-decreasing, < SLN) and
+ * all shards run on the current device.  hvol is not modified; the mask
+ * multiplies a private copy (MultiSlicesProjGPU.cu:202-209,215).  startIdx may
+ * be NULL (nStart 0). */
+int ddfan_proj(float x0, float y0, int DNU, const float* xds, const float* yds,
+               float imgXCenter, float imgYCenter, const float* hangs, int PN, int XN, int YN,
+               int SLN, const float* hvol, float* hprj, float dx, const unsigned char* mask,
+               const int* startIdx, int nStart);
+
+/* DD backprojector.  Same argument order as DD2BackMultiGPUs.  Masked voxels
+ * are written as 0 (the reference leaves them uninitialised). */
+int ddfan_back(float x0, float y0, int DNU, const float* xds, const float* yds, float detCntIdx,
+               float imgXCenter, float imgYCenter, const float* hangs, int PN, int XN, int YN,
+               int SLN, float* hvol, const float* hprj, float dx, const unsigned char* mask,
+               const int* startIdx, int nStart);
+
+/* Helical-to-fan single-slice rebinning.  Same arguments as HelicalToFan plus
+ * TotalViews, the number of raw views in proj: a slice whose rotation window
+ * ends past TotalViews returns DDFAN_EBOUNDS (the reference reads out of
+ * bounds there). */
+int ddfan_helical_to_fan(float* Proj, float* Views, const float* proj, long long TotalViews,
+                         const float* zPos, int SLN, float SD, float SO, float BVAngle,
+                         int DetWidth, int DetHeight, float PerDetW, float PerDetH, int DefTimes,
+                         float DetCenterW, float SpiralPitchFactor);
+
+/* ------------------------------------------------------------ geometry tables */
+
+/* Equiangular arc detector centres, DD2MutiSlices.m:61-76 (MATLAB single
+ * arithmetic, platform libm). */
+int ddfan_detector_arc(int DNU, float detCntIdx, float detCellWidth, float sdd, float sid,
+                       float* xds, float* yds);
+/* Circular reconstruction mask, reconscript.m:28-38 (mask[y*XN + x]). */
+int ddfan_recon_mask(int XN, int YN, unsigned char* mask);
+/* OS-SART subset k (0-based) of OSSART_AAPM.m:60: views k, k+numOS, ...
+ * Writes *count entries to out (out may be NULL to query the count). */
+int ddfan_subset_views(int numViews, int numOS, int k, int* out, int* count);
+/* Rebin per-slice first raw view, HelicalToFanFunc_mex.cpp:71 (1-based view). */
+int ddfan_rebin_starts(const float* zPos, int SLN, float SD, float SO, int DetHeight,
+                       float PerDetW, float PerDetH, int DefTimes, float SpiralPitchFactor,
+                       int* start);
+
+/* --------------------------------------------------------- device memory */
+int ddfan_dev_alloc(size_t bytes, void** ptr);
+int ddfan_dev_free(void* ptr);
+int ddfan_h2d(void* dst, const void* src, size_t bytes);
+int ddfan_d2h(void* dst, const void* src, size_t bytes);
+int ddfan_d2d(void* dst, const void* src, size_t bytes);
+int ddfan_dev_mem_info(size_t* free_bytes, size_t* total_bytes);
+
+/* ------------------------------------------------------------ plan API */
+
+/* Geometry of one reconstruction, the fields DD2MutiSlices.m:49-76 derives
+ * from conf.  The plan copies xds/yds. */
+typedef struct ddfan_geom {
+    float x0, y0;  /* source position at angle 0, (0, sid) in the reference */
+    int DNU;
+    const float* xds;
+    const float* yds;
+    float detCntIdx;
+    float imgXCenter, imgYCenter;
+    int XN, YN, SLN;
+    float dx;
+} ddfan_geom;
+
+typedef struct ddfan_plan ddfan_plan;
+
+/* Creates a plan on the current device for PN views with angles
+ * hangs[v*SLN + z] (copied).  slice_chunk: 0 picks the widest chunk of
+ * consecutive slices whose angles agree in every view (1, 2, 4, 8 or 16);
+ * a positive value caps it. */
+int ddfan_plan_create(const ddfan_geom* geom, const float* hangs, int PN,
+                      const unsigned char* mask, int sat_mode, int slice_chunk,
+                      ddfan_plan** out);
+int ddfan_plan_destroy(ddfan_plan* plan);
+int ddfan_plan_query(const ddfan_plan* plan, int* slice_chunk, int* n_chunks, int* sat_mode);
+/* Device bytes a plan of this shape holds for full-view operators plus an
+ * OS-SART run with numOS subsets (0: operators only). */
+int ddfan_plan_footprint(const ddfan_geom* geom, int PN, int sat_mode, int slice_chunk, int numOS,
+                         size_t* bytes);
+
+/* Device-resident operators, enqueued on the plan's stream.  The view subset
+ * is first, first+stride, ... (count views) of the plan's angle table;
+ * (0, 1, PN) is the reference's full call.  d_prj holds the subset's views
+ * only, in subset order. */
+int ddfan_plan_proj(ddfan_plan* plan, const float* d_vol, float* d_prj, int first, int stride,
+                    int count);
+int ddfan_plan_back(ddfan_plan* plan, const float* d_prj, float* d_vol, int first, int stride,
+                    int count);
+int ddfan_plan_sync(ddfan_plan* plan);
+
+/* OS-SART(+FISTA) of OSSART_AAPM.m:49-117 on device-resident data:
+ * d_meas[(v*DNU + u)*SLN + z] over the plan's PN views, d_x in/out.
+ * flags: DDFAN_OSSART_GRAPH replays one captured iteration as a hipGraph. */
+#define DDFAN_OSSART_GRAPH 1
+int ddfan_ossart(ddfan_plan* plan, const float* d_meas, float* d_x, int numIter, int numOS,
+                 int useFISTA, int flags);
+
+/* The reference's plain form on host buffers: every subset calls the
+ * drop-in projector and backprojector (allocate, copy in, compute, copy out,
+ * free) and the ratio/normalise/update run on the host (OSSART_AAPM.m:49-117
+ * with the calls of DD2MutiSlices.m:106-115). */
+int ddfan_ossart_host_plain(float x0, float y0, int DNU, const float* xds, const float* yds,
+                            float detCntIdx, float imgXCenter, float imgYCenter,
+                            const float* hangs, int numViews, int XN, int YN, int SLN,
+                            const float* meas, float* x, float dx, const unsigned char* mask,
+                            int numIter, int numOS, int useFISTA, int sat_mode);
+
+/* Kernel timing on the plan's stream: when enabled every kernel launch is
+ * bracketed by HIP events; ddfan_plan_timing_read synchronises and returns
+ * the total milliseconds and launch count of one kernel kind since the last
+ * reset.  kind: 0 volume SAT, 1 projector, 2 sinogram SAT, 3 backprojector. */
+int ddfan_plan_timing(ddfan_plan* plan, int enable);
+int ddfan_plan_timing_read(ddfan_plan* plan, int kind, double* total_ms, int* launches);
+
+/* Integer index traces computed by the same device functions as the
+ * operators (for bit-exactness checks).  Projector: d_trace int32
+ * [count][DNU][SLN][max(XN,YN)][2] texel indices (L, R) per slab;
+ * backprojector: [YN][XN][SLN][count][2] detector texel indices. */
+int ddfan_plan_trace_proj(ddfan_plan* plan, int first, int stride, int count, int32_t* d_trace);
+int ddfan_plan_trace_back(ddfan_plan* plan, int first, int stride, int count, int32_t* d_trace);
+
+/* Device-resident rebin.  raw_kind 0: d_raw float line integrals; 1: d_raw
+ * uint16 counts, converted to log(count_scale / max(count, 1)) on the fly.
+ * layout 0: reference layout (Proj[(s*DefTimes+v)*W+ch], Views[s*DefTimes+v]);
+ * layout 1: sinogram layout (prj[(v*W+ch)*SLN+s], hangs[v*SLN+s]).
+ * d_cut (optional) [SLN][DefTimes] receives 1 for rows zeroed as cut views;
+ * d_idv (optional, layout-independent) int32 [SLN][DefTimes][W] source rows. */
+int ddfan_helical_to_fan_dev(float* d_Proj, float* d_Views, const void* d_raw, int raw_kind,
+                             float count_scale, int layout, long long TotalViews,
+                             const float* zPos, int SLN, float SD, float SO, float BVAngle,
+                             int DetWidth, int DetHeight, float PerDetW, float PerDetH,
+                             int DefTimes, float DetCenterW, float SpiralPitchFactor,
+                             unsigned char* d_cut, int32_t* d_idv);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DDFAN_H */
