@@ -66,23 +66,42 @@ struct ProjRay {
     bool xslab;        // |cos| <= |sin|: slabs are image columns, SAT along y
 };
 
-DDF_HD ProjRay proj_ray(float x0, float y0, float c, float s, float bxL, float byL, float bxR,
-                        float byR) {
+// One detector boundary ray: detector u's left boundary is boundary u and its
+// right boundary is boundary u+1, so neighbouring detectors share a boundary
+// ray bit for bit (MultiSlicesProjGPU.cu:131-137,155-156,178-179).
+struct ProjBoundary {
+    float a, b, f;  // r(obj) = (obj - a) * f + b
+    bool xslab;
+};
+
+DDF_HD bool proj_xslab(float c, float s) { return fabsf(c) <= fabsf(s); }
+
+DDF_HD ProjBoundary proj_boundary(float x0, float y0, float c, float s, float bx, float by) {
     const float srcx = x0 * c - y0 * s;
     const float srcy = x0 * s + y0 * c;
-    const float dLx = bxL * c - byL * s, dLy = bxL * s + byL * c;
-    const float dRx = bxR * c - byR * s, dRy = bxR * s + byR * c;
-    ProjRay r;
-    r.xslab = fabsf(c) <= fabsf(s);
+    const float dx = bx * c - by * s, dy = bx * s + by * c;
+    ProjBoundary r;
+    r.xslab = proj_xslab(c, s);
     if (r.xslab) {
-        r.fL = (dLy - srcy) / (dLx - srcx);
-        r.fR = (dRy - srcy) / (dRx - srcx);
-        r.aL = dLx; r.bL = dLy; r.aR = dRx; r.bR = dRy;
+        r.f = (dy - srcy) / (dx - srcx);
+        r.a = dx;
+        r.b = dy;
     } else {
-        r.fL = (dLx - srcx) / (dLy - srcy);
-        r.fR = (dRx - srcx) / (dRy - srcy);
-        r.aL = dLy; r.bL = dLx; r.aR = dRy; r.bR = dRx;
+        r.f = (dx - srcx) / (dy - srcy);
+        r.a = dy;
+        r.b = dx;
     }
+    return r;
+}
+
+DDF_HD ProjRay proj_ray(float x0, float y0, float c, float s, float bxL, float byL, float bxR,
+                        float byR) {
+    const ProjBoundary L = proj_boundary(x0, y0, c, s, bxL, byL);
+    const ProjBoundary R = proj_boundary(x0, y0, c, s, bxR, byR);
+    ProjRay r;
+    r.xslab = L.xslab;
+    r.aL = L.a; r.bL = L.b; r.fL = L.f;
+    r.aR = R.a; r.bR = R.b; r.fR = R.f;
     return r;
 }
 

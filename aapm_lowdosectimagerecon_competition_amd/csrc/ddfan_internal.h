@@ -14,7 +14,8 @@ enum SatMode : int {
     SAT_F64 = 2,       // FP64 prefix stored as double
 };
 
-inline size_t sat_elem_bytes(int mode) { return mode == SAT_F32 ? 4 : 8; }
+// Bytes of one table entry: (S, v) float pair, (hi, lo, v, 0) floats, (S, v) doubles.
+inline size_t sat_elem_bytes(int mode) { return mode == SAT_F32 ? 8 : 16; }
 
 // Scalars every kernel needs, passed by value.
 struct GeomDev {
@@ -32,10 +33,10 @@ struct Subset {
     int first, stride, count;
 };
 
-// Summed-area tables are stored as [chunk][row][k][w][zt]: `zt` runs over the
-// ZT slices of a slice chunk that share one geometry, `w` is 1 (f32, f64) or
-// 2 (hi, lo) words, k = 0..N is the prefix index, rows are slabs (volume) or
-// views (sinogram).
+// Summed-area tables are stored as [chunk][row][zt][k] of entries pairing the
+// prefix S[k] with the sample v[k]: `zt` runs over the ZT slices of a chunk
+// that share one geometry, k = 0..N, rows are slabs (volume) or views
+// (sinogram).
 
 // Fused epilogues.
 enum ProjEpi : int { PROJ_PLAIN = 0, PROJ_RATIO = 1 };

@@ -1,43 +1,43 @@
 // ddfan_kernels.hip -- gfx950 kernels of the DD fan-beam path.
 //
-//   k_sat_vol    volume summed-area tables, both slab orientations, mask fused
-//                (MultiSlicesProjGPU.cu:26-97,202-209)
+//   k_sat_rows   summed-area tables along one axis (volume rows along x or y
+//                with the mask fused, or sinogram rows along the detector)
+//                (MultiSlicesProjGPU.cu:26-97,202-209; MultiSlicesBackGPU.cu:12-66)
 //   k_proj       branchless DD projector (MultiSlicesProjGPU.cu:100-200) with an
 //                optional OS-SART ratio epilogue (OSSART_AAPM.m:81-88)
-//   k_sat_sino   detector-axis summed-area table of a sinogram
-//                (MultiSlicesBackGPU.cu:12-66)
 //   k_back       voxel-driven DD backprojector (MultiSlicesBackGPU.cu:115-184)
 //                with optional OS-SART normalise/update and FISTA epilogues
 //                (OSSART_AAPM.m:91-97,111-115)
 //   k_rebin      helical-to-fan gather (HelicalToFanFunc_mex.cpp:41-139)
 //
-// No texture units: every fetch is an explicit clamp + FP32/FP64 lerp from
-// global memory with the reference's texel-centred convention.  Geometry that
-// decides an integer goes through ddfan_index.h and is bit-identical to the
-// host restatement.  Slices of a chunk (ZT of them) share one view angle, so
-// the per-ray index arithmetic is done once per chunk and the ZT slices ride
-// in registers; a chunk of ZT = 1 is the general per-slice-angle path.
+// No texture units.  A summed-area-table entry k holds the prefix S[k] (sum of
+// samples 0..k-1) next to the sample value v[k] (v[N] = 0), so the
+// reference's clamp-to-edge linear fetch T(c) = S[i] + t*(S[i+1] - S[i]) is a
+// single fma on one paired load: T = S[i0] + t*v[i0] with i0 = clamp(i, 0, N)
+// and t = 0 where the clamp bites.  Tables are stored [chunk][row][zt][k]:
+// the ZT slices of a chunk share one view angle, so the per-ray index
+// arithmetic runs once per chunk and lanes read consecutive k.  ZT = 1 is the
+// general per-slice-angle path.  Geometry that decides an integer goes through
+// ddfan_index.h and is bit-identical to the host restatement.
 #include "ddfan_index.h"
 #include "ddfan_internal.h"
 
 namespace ddf {
 
+// Table entry of each precision variant: (S, v), (S_hi, S_lo, v, 0), (S, v).
 template <int MODE>
 struct Sat;
 template <>
 struct Sat<SAT_F32> {
-    using T = float;
-    static constexpr int W = 1;
+    using E = float2;
 };
 template <>
 struct Sat<SAT_TWOFLOAT> {
-    using T = float;
-    static constexpr int W = 2;
+    using E = float4;
 };
 template <>
 struct Sat<SAT_F64> {
-    using T = double;
-    static constexpr int W = 1;
+    using E = double2;
 };
 
 // gfx950 hands consecutive workgroups to its 8 XCDs round-robin.  Blocks that
@@ -49,249 +49,243 @@ __device__ __forceinline__ int xcd_linear(int b, int nblocks) {
 }
 static inline int xcd_grid(int nwork) { return ((nwork + 7) / 8) * 8; }
 
-template <int N>
-__device__ __forceinline__ void ldv(float (&d)[N], const float* __restrict__ p) {
-    if constexpr (N % 4 == 0) {
-#pragma unroll
-        for (int q = 0; q < N / 4; ++q) {
-            const float4 t = reinterpret_cast<const float4*>(p)[q];
-            d[4 * q] = t.x;
-            d[4 * q + 1] = t.y;
-            d[4 * q + 2] = t.z;
-            d[4 * q + 3] = t.w;
-        }
-    } else if constexpr (N % 2 == 0) {
-#pragma unroll
-        for (int q = 0; q < N / 2; ++q) {
-            const float2 t = reinterpret_cast<const float2*>(p)[q];
-            d[2 * q] = t.x;
-            d[2 * q + 1] = t.y;
-        }
-    } else {
-#pragma unroll
-        for (int q = 0; q < N; ++q) d[q] = p[q];
-    }
+// Value of the same register in lane - 1 (lane 0 reads 0): DPP wave_shr:1.
+__device__ __forceinline__ float lane_prev(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xF, 0xF, true));
+}
+__device__ __forceinline__ double lane_prev(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)(b & 0xffffffffu), 0x138, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), 0x138, 0xF, 0xF, true);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
-template <int N>
-__device__ __forceinline__ void ldv(double (&d)[N], const double* __restrict__ p) {
-    if constexpr (N % 2 == 0) {
-#pragma unroll
-        for (int q = 0; q < N / 2; ++q) {
-            const double2 t = reinterpret_cast<const double2*>(p)[q];
-            d[2 * q] = t.x;
-            d[2 * q + 1] = t.y;
-        }
-    } else {
-#pragma unroll
-        for (int q = 0; q < N; ++q) d[q] = p[q];
-    }
-}
-
-// acc[zt] += (T(cR) - T(cL)) * scale for the ZT slices of one SAT row, where
-// T(c) is the clamp-to-edge linear fetch at texel pair (i, i+1) with weight t.
-template <int MODE, int ZT>
-__device__ __forceinline__ void sat_diff_accum(float (&acc)[ZT],
-                                               const typename Sat<MODE>::T* __restrict__ row,
-                                               int N, int iL, float tL, int iR, float tR,
-                                               float scale) {
-    constexpr int STR = ZT * Sat<MODE>::W;
-    const int l0 = min(max(iL, 0), N), l1 = min(max(iL + 1, 0), N);
-    const int r0 = min(max(iR, 0), N), r1 = min(max(iR + 1, 0), N);
-    tL = (l0 == l1) ? 0.0f : tL;
-    tR = (r0 == r1) ? 0.0f : tR;
-    if constexpr (MODE == SAT_F32) {
-        float a0[ZT], a1[ZT], b0[ZT], b1[ZT];
-        ldv(a0, row + (size_t)l0 * STR);
-        ldv(a1, row + (size_t)l1 * STR);
-        ldv(b0, row + (size_t)r0 * STR);
-        ldv(b1, row + (size_t)r1 * STR);
-#pragma unroll
-        for (int q = 0; q < ZT; ++q) {
-            const float TL = fmaf(tL, a1[q] - a0[q], a0[q]);
-            const float TR = fmaf(tR, b1[q] - b0[q], b0[q]);
-            acc[q] = fmaf(TR - TL, scale, acc[q]);
-        }
-    } else if constexpr (MODE == SAT_TWOFLOAT) {
-        // T = hi[i0] + (lo[i0] + t*((hi[i1]-hi[i0]) + (lo[i1]-lo[i0]))) kept as a
-        // (hi, lo) pair so T(cR)-T(cL) cancels without losing the low word.
-        float h0[ZT], h1[ZT], g0[ZT], g1[ZT];
-        ldv(h0, row + (size_t)l0 * STR);
-        ldv(h1, row + (size_t)l1 * STR);
-        ldv(g0, row + (size_t)r0 * STR);
-        ldv(g1, row + (size_t)r1 * STR);
-        float m0[ZT], m1[ZT], n0[ZT], n1[ZT];
-        ldv(m0, row + (size_t)l0 * STR + ZT);
-        ldv(m1, row + (size_t)l1 * STR + ZT);
-        ldv(n0, row + (size_t)r0 * STR + ZT);
-        ldv(n1, row + (size_t)r1 * STR + ZT);
-#pragma unroll
-        for (int q = 0; q < ZT; ++q) {
-            const float TLlo = fmaf(tL, (h1[q] - h0[q]) + (m1[q] - m0[q]), m0[q]);
-            const float TRlo = fmaf(tR, (g1[q] - g0[q]) + (n1[q] - n0[q]), n0[q]);
-            const float D = (g0[q] - h0[q]) + (TRlo - TLlo);
-            acc[q] = fmaf(D, scale, acc[q]);
-        }
-    } else {
-        double a0[ZT], a1[ZT], b0[ZT], b1[ZT];
-        ldv(a0, row + (size_t)l0 * STR);
-        ldv(a1, row + (size_t)l1 * STR);
-        ldv(b0, row + (size_t)r0 * STR);
-        ldv(b1, row + (size_t)r1 * STR);
-        const double dtL = tL, dtR = tR;
-#pragma unroll
-        for (int q = 0; q < ZT; ++q) {
-            const double TL = fma(dtL, a1[q] - a0[q], a0[q]);
-            const double TR = fma(dtR, b1[q] - b0[q], b0[q]);
-            acc[q] = fmaf((float)(TR - TL), scale, acc[q]);
-        }
-    }
+// Fetch position of a texture coordinate c on a table with entries 0..N:
+// i0 = clamp(floor(c - 0.5), 0, N) and the weight t, forced to 0 where the
+// clamp-to-edge fetch reads one texel twice.
+struct Fetch {
+    unsigned i0;
+    float t;
+};
+__device__ __forceinline__ Fetch fetch_pos(float c, int N) {
+    const int i = texel(c, N);
+    const int i0 = min(max(i, 0), N);
+    Fetch f;
+    f.i0 = (unsigned)i0;
+    f.t = (i < 0 || i >= N) ? 0.0f : (c - 0.5f) - (float)i;
+    return f;
 }
 
 // ------------------------------------------------------------------ SAT
-template <int MODE, int ZT>
-struct SatWriter {
-    using T = typename Sat<MODE>::T;
-    static constexpr int W = Sat<MODE>::W;
-    float hi = 0.0f, lo = 0.0f;
-    double d = 0.0;
-    __device__ __forceinline__ void add(float v) {
-        if constexpr (MODE == SAT_F32) {
-            hi = hi + v;
-        } else if constexpr (MODE == SAT_TWOFLOAT) {
-            const float s = hi + v;  // TwoSum(hi, v)
-            const float bb = s - hi;
-            const float e = (hi - (s - bb)) + (v - bb);
-            hi = s;
-            lo = lo + e;
-        } else {
-            d = d + (double)v;
-        }
+template <int MODE>
+struct Scan;
+template <>
+struct Scan<SAT_F32> {
+    using L = float;
+    float a = 0.0f;
+    __device__ __forceinline__ void add(float v) { a = a + v; }
+    __device__ __forceinline__ L get() const { return a; }
+    __device__ static __forceinline__ L zero() { return 0.0f; }
+    __device__ static __forceinline__ L plus(L x, L y) { return x + y; }
+    __device__ static __forceinline__ float2 entry(L s, float v) { return make_float2(s, v); }
+};
+template <>
+struct Scan<SAT_TWOFLOAT> {
+    using L = float2;  // unnormalised (hi, lo); value = hi + lo
+    float h = 0.0f, l = 0.0f;
+    __device__ static __forceinline__ float2 two_sum(float a, float b) {
+        const float s = a + b;
+        const float bb = s - a;
+        return make_float2(s, (a - (s - bb)) + (b - bb));
     }
-    // out points at element zt of prefix entry k.
-    __device__ __forceinline__ void store(T* __restrict__ out) const {
-        if constexpr (MODE == SAT_F32) {
-            out[0] = hi;
-        } else if constexpr (MODE == SAT_TWOFLOAT) {
-            const float H = hi + lo;  // FastTwoSum normalisation, |hi| >= |lo|
-            const float L = lo - (H - hi);
-            out[0] = H;
-            out[ZT] = L;
-        } else {
-            out[0] = d;
-        }
+    __device__ __forceinline__ void add(float v) {
+        const float2 t = two_sum(h, v);
+        h = t.x;
+        l = l + t.y;
+    }
+    __device__ __forceinline__ L get() const { return make_float2(h, l); }
+    __device__ static __forceinline__ L zero() { return make_float2(0.0f, 0.0f); }
+    __device__ static __forceinline__ L plus(L x, L y) {
+        const float2 t = two_sum(x.x, y.x);
+        return make_float2(t.x, (x.y + y.y) + t.y);
+    }
+    __device__ static __forceinline__ float4 entry(L s, float v) {
+        const float H = s.x + s.y;  // FastTwoSum normalisation
+        return make_float4(H, s.y - (H - s.x), v, 0.0f);
+    }
+};
+template <>
+struct Scan<SAT_F64> {
+    using L = double;
+    double a = 0.0;
+    __device__ __forceinline__ void add(float v) { a = a + (double)v; }
+    __device__ __forceinline__ L get() const { return a; }
+    __device__ static __forceinline__ L zero() { return 0.0; }
+    __device__ static __forceinline__ L plus(L x, L y) { return x + y; }
+    __device__ static __forceinline__ double2 entry(L s, float v) {
+        return make_double2(s, (double)v);
     }
 };
 
-template <int MODE, int ZT>
-__global__ __launch_bounds__(256) void k_sat_vol(const GeomDev g, const int nzc,
-                                                 const float* __restrict__ vol,
-                                                 const uint8_t* __restrict__ mask,
-                                                 typename Sat<MODE>::T* __restrict__ satX,
-                                                 typename Sat<MODE>::T* __restrict__ satY) {
-    constexpr int STR = ZT * Sat<MODE>::W;
-    const int SLNp = nzc * ZT;
-    const bool isY = blockIdx.y == 1;  // SAT along y: rows are image columns x
-    const int nrows = isY ? g.XN : g.YN;
-    const int Nk = isY ? g.YN : g.XN;
-    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long long)nrows * SLNp) return;
-    const int z = (int)(idx % SLNp);
-    const int row = (int)(idx / SLNp);
-    const int zc = z / ZT, zt = z - zc * ZT;
-    typename Sat<MODE>::T* out =
-        (isY ? satY : satX) + ((size_t)zc * nrows + row) * (size_t)(Nk + 1) * STR + zt;
-    const bool real = z < g.SLN;
-    SatWriter<MODE, ZT> w;
-    w.store(out);
-    for (int k = 0; k < Nk; ++k) {
+// Summed-area tables of one axis.  Sample (row, k, z) sits at
+// in[(row*rs + k*ks)*SLN + z], optionally times mask[row*rs + k*ks] (its byte
+// value, MultiSlicesProjGPU.cu:202-209).  One block scans one row for G
+// consecutive slices: 256/G threads per slice own contiguous k segments
+// (loads coalesce over the slices, contiguous in memory), segment totals are
+// combined through LDS, and finished rows leave through LDS as contiguous k
+// runs of paired entries.
+struct SatJob {
+    const float* in;
+    const uint8_t* mask;
+    void* out;
+    int nrows, Nk, rs, ks;
+};
+struct SatJobs {
+    SatJob j[2];
+};
+
+template <int MODE, int G>
+__global__ __launch_bounds__(256) void k_sat_rows(const SatJobs jobs, const int SLN, const int ZT,
+                                                  const int nzc, const int nzg) {
+    using S = Scan<MODE>;
+    using L = typename S::L;
+    using E = typename Sat<MODE>::E;
+    constexpr int NSEG = 256 / G;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const SatJob jb = jobs.j[blockIdx.y];
+    const int K = jb.Nk + 1;
+    const int row = blockIdx.x / nzg;
+    const int zg = blockIdx.x - row * nzg;
+    if (row >= jb.nrows) return;  // whole block
+    const int tz = threadIdx.x % G, seg = threadIdx.x / G;
+    const int z = zg * G + tz;
+    const bool real = z < SLN;
+    L* pre = reinterpret_cast<L*>(smem);          // [G][K] exclusive prefixes
+    L* tot = pre + (size_t)G * K;                 // [G][NSEG] segment totals
+    float* val = reinterpret_cast<float*>(tot + (size_t)G * NSEG);  // [G][K] samples
+    const int Lseg = (jb.Nk + NSEG - 1) / NSEG;
+    const int k0 = min(seg * Lseg, jb.Nk), k1 = min(k0 + Lseg, jb.Nk);
+    S acc;
+    for (int k = k0; k < k1; ++k) {
         float v = 0.0f;
         if (real) {
-            const int x = isY ? row : k, y = isY ? k : row;
-            const size_t p = (size_t)y * g.XN + x;
-            v = vol[p * g.SLN + z] * (float)mask[p];
+            const size_t p = (size_t)row * jb.rs + (size_t)k * jb.ks;
+            v = jb.in[p * SLN + z];
+            if (jb.mask) v = v * (float)jb.mask[p];
         }
-        w.add(v);
-        w.store(out + (size_t)(k + 1) * STR);
+        pre[(size_t)tz * K + k] = acc.get();
+        val[(size_t)tz * K + k] = v;
+        acc.add(v);
+    }
+    tot[tz * NSEG + seg] = acc.get();
+    if (seg == NSEG - 1) val[(size_t)tz * K + jb.Nk] = 0.0f;
+    __syncthreads();
+    L off = S::zero();
+    for (int q = 0; q < seg; ++q) off = S::plus(off, tot[tz * NSEG + q]);
+    for (int k = k0; k < k1; ++k) pre[(size_t)tz * K + k] = S::plus(pre[(size_t)tz * K + k], off);
+    if (seg == NSEG - 1) pre[(size_t)tz * K + jb.Nk] = S::plus(off, acc.get());
+    __syncthreads();
+    const int SLNp = nzc * ZT;
+    E* base = reinterpret_cast<E*>(jb.out);
+    for (int idx = threadIdx.x; idx < G * K; idx += 256) {
+        const int q = idx / K, k = idx - q * K;
+        const int zz = zg * G + q;
+        if (zz < SLNp) {
+            const int zc = zz / ZT, zt = zz - zc * ZT;
+            base[(((size_t)zc * jb.nrows + row) * ZT + zt) * K + k] = S::entry(pre[idx], val[idx]);
+        }
     }
 }
 
-template <int MODE, int ZT>
-__global__ __launch_bounds__(256) void k_sat_sino(const int DNU, const int SLN, const int nzc,
-                                                  const int PNs, const float* __restrict__ r,
-                                                  typename Sat<MODE>::T* __restrict__ P) {
-    constexpr int STR = ZT * Sat<MODE>::W;
-    const int SLNp = nzc * ZT;
-    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long long)PNs * SLNp) return;
-    const int z = (int)(idx % SLNp);
-    const int j = (int)(idx / SLNp);
-    const int zc = z / ZT, zt = z - zc * ZT;
-    typename Sat<MODE>::T* out = P + ((size_t)zc * PNs + j) * (size_t)(DNU + 1) * STR + zt;
-    const bool real = z < SLN;
-    SatWriter<MODE, ZT> w;
-    w.store(out);
-    for (int u = 0; u < DNU; ++u) {
-        const float v = real ? r[((size_t)j * DNU + u) * SLN + z] : 0.0f;
-        w.add(v);
-        w.store(out + (size_t)(u + 1) * STR);
+// T(c) from one entry: scalar for f32/f64, (hi, lo) for the two-float table.
+template <int MODE>
+struct Samp;
+template <>
+struct Samp<SAT_F32> {
+    float v;
+    __device__ __forceinline__ Samp(float2 e, float t) : v(fmaf(t, e.y, e.x)) {}
+    __device__ __forceinline__ float minus_prev_lane() const { return v - lane_prev(v); }
+    __device__ __forceinline__ float minus(const Samp& o) const { return v - o.v; }
+};
+template <>
+struct Samp<SAT_TWOFLOAT> {
+    float h, l;
+    __device__ __forceinline__ Samp(float4 e, float t) : h(e.x), l(fmaf(t, e.z, e.y)) {}
+    __device__ __forceinline__ float minus_prev_lane() const {
+        return (h - lane_prev(h)) + (l - lane_prev(l));
     }
-}
+    __device__ __forceinline__ float minus(const Samp& o) const { return (h - o.h) + (l - o.l); }
+};
+template <>
+struct Samp<SAT_F64> {
+    double v;
+    __device__ __forceinline__ Samp(double2 e, float t) : v(fma((double)t, e.y, e.x)) {}
+    __device__ __forceinline__ float minus_prev_lane() const { return (float)(v - lane_prev(v)); }
+    __device__ __forceinline__ float minus(const Samp& o) const { return (float)(v - o.v); }
+};
 
 // ------------------------------------------------------------------ projector
+// Branchless DD projector.  One wave = one (view, slice chunk, group of 63
+// detectors); lane i owns detector boundary ray b = 63*group + i, so every
+// boundary is intersected with each slab once and its table sample T_b is
+// handed to the next lane through DPP: detector u = b - 1 accumulates
+// (T_b - T_{b-1}) / (r_b - r_{b-1}), the reference's
+// (T(realR) - T(realL)) / intersectLength (MultiSlicesProjGPU.cu:160-171,
+// 183-194), since neighbouring detectors share the boundary ray bit for bit.
 template <int MODE, int ZT, int EPI>
 __global__ __launch_bounds__(64) void k_proj(
-    const GeomDev g, const int nzc, const int nub, const int nwork,
+    const GeomDev g, const int nzc, const int nug, const int nwork,
     const float2* __restrict__ cossin, const float* __restrict__ bx, const float* __restrict__ by,
     const float* __restrict__ xds, const float* __restrict__ yds,
-    const typename Sat<MODE>::T* __restrict__ satX, const typename Sat<MODE>::T* __restrict__ satY,
+    const typename Sat<MODE>::E* __restrict__ satX, const typename Sat<MODE>::E* __restrict__ satY,
     const Subset sub, float* __restrict__ out, const float* __restrict__ meas,
     const float* __restrict__ rowSum) {
-    using T = typename Sat<MODE>::T;
-    constexpr int STR = ZT * Sat<MODE>::W;
+    using E = typename Sat<MODE>::E;
     const int lin = xcd_linear(blockIdx.x, gridDim.x);
-    if (lin >= nwork) return;
-    const int ub = lin % nub;
-    const int rest = lin / nub;
+    if (lin >= nwork) return;  // whole wave
+    const int ug = lin % nug;
+    const int rest = lin / nug;
     const int j = rest % sub.count;
     const int zc = rest / sub.count;
 
-    const int u = ub * 64 + threadIdx.x;
-    const bool act = u < g.DNU;
-    const int uu = act ? u : g.DNU - 1;
+    const int lane = threadIdx.x;
+    const int bnd = min(ug * 63 + lane, g.DNU);
+    const int u = ug * 63 + lane - 1;
+    const bool act = lane >= 1 && u < g.DNU;
+    const int uu = min(max(u, 0), g.DNU - 1);
     const int vg = sub.first + j * sub.stride;
     const int z0 = zc * ZT;
     const float2 cs = cossin[(size_t)vg * g.SLN + min(z0, g.SLN - 1)];
-    const ProjRay ray = proj_ray(g.x0, g.y0, cs.x, cs.y, bx[uu], by[uu], bx[uu + 1], by[uu + 1]);
+    const ProjBoundary pb = proj_boundary(g.x0, g.y0, cs.x, cs.y, bx[bnd], by[bnd]);
 
     // continuous: dx / |normalize(det - src)| component (MultiSlicesProjGPU.cu:139,158,181)
     const float srcx = g.x0 * cs.x - g.y0 * cs.y, srcy = g.x0 * cs.y + g.y0 * cs.x;
     const float ddx = (xds[uu] * cs.x - yds[uu] * cs.y) - srcx;
     const float ddy = (xds[uu] * cs.y + yds[uu] * cs.x) - srcy;
     const float nrm = sqrtf(fmaf(ddx, ddx, ddy * ddy));
-    const float constVal = g.dx * nrm / fabsf(ray.xslab ? ddx : ddy);
+    const float constVal = g.dx * nrm / fabsf(pb.xslab ? ddx : ddy);
 
-    const int Ns = ray.xslab ? g.XN : g.YN;
-    const int Nk = ray.xslab ? g.YN : g.XN;
-    const float ocS = ray.xslab ? g.ocxP : g.ocyP;
-    const float ocK = ray.xslab ? g.ocyP : g.ocxP;
-    const size_t rowStride = (size_t)(Nk + 1) * STR;
-    const T* sat = (ray.xslab ? satY : satX) + (size_t)zc * Ns * rowStride;
+    const int Ns = pb.xslab ? g.XN : g.YN;
+    const int Nk = pb.xslab ? g.YN : g.XN;
+    const int K = Nk + 1;
+    const float ocS = pb.xslab ? g.ocxP : g.ocyP;
+    const float ocK = pb.xslab ? g.ocyP : g.ocxP;
+    const size_t rowStride = (size_t)ZT * K;
+    const E* sat = (pb.xslab ? satY : satX) + (size_t)zc * Ns * rowStride;
 
     float acc[ZT];
 #pragma unroll
     for (int q = 0; q < ZT; ++q) acc[q] = 0.0f;
     for (int ii = 0; ii < Ns; ++ii) {
-        const float obj = slab_obj(ii, ocS, g.dx);
-        const float rL = slab_r(obj, ray.aL, ray.fL, ray.bL);
-        const float rR = slab_r(obj, ray.aR, ray.fR, ray.bR);
-        const float cL = slab_coord(rL, g.invdx, ocK);
-        const float cR = slab_coord(rR, g.invdx, ocK);
-        const int iL = texel(cL, Nk), iR = texel(cR, Nk);
-        const float tL = (cL - 0.5f) - (float)iL;
-        const float tR = (cR - 0.5f) - (float)iR;
-        const float invlen = __builtin_amdgcn_rcpf(rR - rL);
-        sat_diff_accum<MODE, ZT>(acc, sat + (size_t)ii * rowStride, Nk, iL, tL, iR, tR, invlen);
+        const float r = slab_r(slab_obj(ii, ocS, g.dx), pb.a, pb.f, pb.b);
+        const Fetch f = fetch_pos(slab_coord(r, g.invdx, ocK), Nk);
+        const float invlen = __builtin_amdgcn_rcpf(r - lane_prev(r));
+        const E* row = sat + (size_t)ii * rowStride;
+#pragma unroll
+        for (int q = 0; q < ZT; ++q) {
+            const Samp<MODE> T(row[(size_t)q * K + f.i0], f.t);
+            acc[q] = fmaf(T.minus_prev_lane(), invlen, acc[q]);
+        }
     }
     if (!act) return;
     const size_t o0 = ((size_t)j * g.DNU + u) * g.SLN + z0;
@@ -312,15 +306,19 @@ __global__ __launch_bounds__(64) void k_proj(
 }
 
 // ------------------------------------------------------------------ backprojector
+// Voxel-driven DD backprojector: a 256-thread block is a 16x16 voxel tile of
+// one slice chunk; each lane loops over the subset's views, computes the
+// voxel footprint on the detector once for the chunk and accumulates
+// (T(idxR) - T(idxL)) * cosVal for its ZT slices.  Masked voxels skip the
+// work and write 0 (or leave the iterate unchanged in the OS-SART epilogues).
 template <int MODE, int ZT, int EPI>
 __global__ __launch_bounds__(256) void k_back(
     const GeomDev g, const int nzc, const int ntx, const int nty, const int nwork,
     const float2* __restrict__ cossin, const uint8_t* __restrict__ mask,
-    const typename Sat<MODE>::T* __restrict__ P, const Subset sub, float* __restrict__ vol,
+    const typename Sat<MODE>::E* __restrict__ P, const Subset sub, float* __restrict__ vol,
     const float* __restrict__ colSum, float* __restrict__ xprev, const float* __restrict__ momTab,
     const int* __restrict__ iterCounter) {
-    using T = typename Sat<MODE>::T;
-    constexpr int STR = ZT * Sat<MODE>::W;
+    using E = typename Sat<MODE>::E;
     const int lin = xcd_linear(blockIdx.x, gridDim.x);
     if (lin >= nwork) return;
     const int tile = lin % (ntx * nty);
@@ -332,8 +330,9 @@ __global__ __launch_bounds__(256) void k_back(
     const int z0 = zc * ZT;
     const int zg = min(z0, g.SLN - 1);
     const int DNU = g.DNU;
-    const size_t rowStride = (size_t)(DNU + 1) * STR;
-    const T* Pc = P + (size_t)zc * sub.count * rowStride;
+    const int K = DNU + 1;
+    const size_t viewStride = (size_t)ZT * K;
+    const E* Pc = P + (size_t)zc * sub.count * viewStride;
 
     float acc[ZT];
 #pragma unroll
@@ -346,12 +345,16 @@ __global__ __launch_bounds__(256) void k_back(
             const float2 cs = cossin[(size_t)vg * g.SLN + zg];
             const BackSample bs =
                 back_sample(vx, vy, cs.x, cs.y, g.x0, g.y0, g.inv_sid, g.invdb, g.detCntIdx, g.dx);
-            const int iL = texel(bs.idxL, DNU), iR = texel(bs.idxR, DNU);
-            const float tL = (bs.idxL - 0.5f) - (float)iL;
-            const float tR = (bs.idxR - 0.5f) - (float)iR;
+            const Fetch fL = fetch_pos(bs.idxL, DNU), fR = fetch_pos(bs.idxR, DNU);
             // dx / ddv * sqrt(l2) with sqrt(l2) = 1 / l  (MultiSlicesBackGPU.cu:173)
             const float cosVal = g.dx * __builtin_amdgcn_rcpf(bs.ddv * bs.l);
-            sat_diff_accum<MODE, ZT>(acc, Pc + (size_t)j * rowStride, DNU, iL, tL, iR, tR, cosVal);
+            const E* row = Pc + (size_t)j * viewStride;
+#pragma unroll
+            for (int q = 0; q < ZT; ++q) {
+                const Samp<MODE> TL(row[(size_t)q * K + fL.i0], fL.t);
+                const Samp<MODE> TR(row[(size_t)q * K + fR.i0], fR.t);
+                acc[q] = fmaf(TR.minus(TL), cosVal, acc[q]);
+            }
         }
     }
     if (!inb) return;
@@ -541,27 +544,58 @@ __global__ __launch_bounds__(256) void k_rebin(const RebinDev p, const void* __r
         default: return hipErrorInvalidValue;                             \
     }
 
+// Slices per SAT block: the widest G whose LDS tiles fit in 64 KiB.
+static int sat_group(int mode, int K) {
+    const size_t lsz = mode == SAT_F32 ? 4 : 8;
+    for (int G : {16, 8, 4, 2, 1})
+        if ((size_t)G * K * (lsz + 4) + (size_t)256 * lsz <= 65536) return G;
+    return 0;
+}
+
+static hipError_t launch_sat_jobs(int mode, int ZT, int nzc, int SLN, const SatJobs& jobs,
+                                  int njobs, hipStream_t st) {
+    int maxRows = 0, maxK = 0;
+    for (int i = 0; i < njobs; ++i) {
+        maxRows = jobs.j[i].nrows > maxRows ? jobs.j[i].nrows : maxRows;
+        maxK = jobs.j[i].Nk + 1 > maxK ? jobs.j[i].Nk + 1 : maxK;
+    }
+    const int G = sat_group(mode, maxK);
+    if (G == 0) return hipErrorInvalidValue;
+    const int nzg = (nzc * ZT + G - 1) / G;
+    const size_t lsz = mode == SAT_F32 ? 4 : 8;
+    const size_t lds = (size_t)G * maxK * (lsz + 4) + (size_t)256 * lsz;
+    const long long nb = (long long)maxRows * nzg;
+    if (nb == 0) return hipSuccess;
+    if (nb > 0x7fffffffLL) return hipErrorInvalidValue;
+    dim3 grid((unsigned)nb, (unsigned)njobs, 1);
+#define DDF_SATV(GG) \
+    DDF_MODE_SWITCH(mode, k_sat_rows<MC, GG><<<grid, 256, lds, st>>>(jobs, SLN, ZT, nzc, nzg))
+    switch (G) {
+        case 16: DDF_SATV(16); break;
+        case 8: DDF_SATV(8); break;
+        case 4: DDF_SATV(4); break;
+        case 2: DDF_SATV(2); break;
+        default: DDF_SATV(1); break;
+    }
+#undef DDF_SATV
+    return hipGetLastError();
+}
+
 hipError_t launch_sat_vol(int mode, int ZT, const GeomDev& g, int nzc, const float* vol,
                           const uint8_t* mask, void* satX, void* satY, hipStream_t st) {
-    const long long n = (long long)(g.XN > g.YN ? g.XN : g.YN) * nzc * ZT;
-    dim3 grid((unsigned)((n + 255) / 256), 2, 1);
-    DDF_MODE_SWITCH(mode, DDF_ZT_SWITCH(ZT, {
-        using T = typename Sat<MC>::T;
-        k_sat_vol<MC, ZTC><<<grid, 256, 0, st>>>(g, nzc, vol, mask, (T*)satX, (T*)satY);
-    }));
-    return hipGetLastError();
+    SatJobs jobs{};
+    jobs.j[0] = SatJob{vol, mask, satX, g.YN, g.XN, g.XN, 1};  // rows y, prefix along x
+    jobs.j[1] = SatJob{vol, mask, satY, g.XN, g.YN, 1, g.XN};  // rows x, prefix along y
+    return launch_sat_jobs(mode, ZT, nzc, g.SLN, jobs, 2, st);
 }
 
 hipError_t launch_sat_sino(int mode, int ZT, int DNU, int SLN, int nzc, int PNs, const float* r,
                            void* P, hipStream_t st) {
-    const long long n = (long long)PNs * nzc * ZT;
-    if (n == 0) return hipSuccess;
-    dim3 grid((unsigned)((n + 255) / 256));
-    DDF_MODE_SWITCH(mode, DDF_ZT_SWITCH(ZT, {
-        using T = typename Sat<MC>::T;
-        k_sat_sino<MC, ZTC><<<grid, 256, 0, st>>>(DNU, SLN, nzc, PNs, r, (T*)P);
-    }));
-    return hipGetLastError();
+    if (PNs == 0) return hipSuccess;
+    SatJobs jobs{};
+    jobs.j[0] = SatJob{r, nullptr, P, PNs, DNU, DNU, 1};  // rows = views, prefix along u
+    jobs.j[1] = jobs.j[0];
+    return launch_sat_jobs(mode, ZT, nzc, SLN, jobs, 1, st);
 }
 
 hipError_t launch_proj(int mode, int ZT, int epi, const GeomDev& g, int nzc, const float2* cossin,
@@ -569,21 +603,21 @@ hipError_t launch_proj(int mode, int ZT, int epi, const GeomDev& g, int nzc, con
                        const void* satX, const void* satY, Subset sub, float* out,
                        const float* meas, const float* rowSum, hipStream_t st) {
     if (sub.count <= 0) return hipSuccess;
-    const int nub = (g.DNU + 63) / 64;
-    const long long nw = (long long)nub * sub.count * nzc;
+    const int nug = (g.DNU + 62) / 63;  // 63 detectors per wave (64 boundaries)
+    const long long nw = (long long)nug * sub.count * nzc;
     if (nw > 0x7fffff00LL) return hipErrorInvalidValue;
     const int nwork = (int)nw;
     dim3 grid((unsigned)xcd_grid(nwork));
     DDF_MODE_SWITCH(mode, DDF_ZT_SWITCH(ZT, {
-        using T = typename Sat<MC>::T;
+        using E = typename Sat<MC>::E;
         if (epi == PROJ_PLAIN)
-            k_proj<MC, ZTC, PROJ_PLAIN><<<grid, 64, 0, st>>>(g, nzc, nub, nwork, cossin, bx, by,
-                                                             xds, yds, (const T*)satX,
-                                                             (const T*)satY, sub, out, meas, rowSum);
+            k_proj<MC, ZTC, PROJ_PLAIN><<<grid, 64, 0, st>>>(g, nzc, nug, nwork, cossin, bx, by,
+                                                             xds, yds, (const E*)satX,
+                                                             (const E*)satY, sub, out, meas, rowSum);
         else
-            k_proj<MC, ZTC, PROJ_RATIO><<<grid, 64, 0, st>>>(g, nzc, nub, nwork, cossin, bx, by,
-                                                             xds, yds, (const T*)satX,
-                                                             (const T*)satY, sub, out, meas, rowSum);
+            k_proj<MC, ZTC, PROJ_RATIO><<<grid, 64, 0, st>>>(g, nzc, nug, nwork, cossin, bx, by,
+                                                             xds, yds, (const E*)satX,
+                                                             (const E*)satY, sub, out, meas, rowSum);
     }));
     return hipGetLastError();
 }
@@ -596,8 +630,8 @@ hipError_t launch_back(int mode, int ZT, int epi, const GeomDev& g, int nzc, con
     const int nwork = ntx * nty * nzc;
     dim3 grid((unsigned)xcd_grid(nwork));
     DDF_MODE_SWITCH(mode, DDF_ZT_SWITCH(ZT, {
-        using T = typename Sat<MC>::T;
-        const T* Pt = (const T*)P;
+        using E = typename Sat<MC>::E;
+        const E* Pt = (const E*)P;
         if (epi == BACK_PLAIN)
             k_back<MC, ZTC, BACK_PLAIN><<<grid, 256, 0, st>>>(g, nzc, ntx, nty, nwork, cossin, mask,
                                                               Pt, sub, vol, colSum, xprev, momTab,
