@@ -36,6 +36,7 @@ SAT_F64 = 2
 SAT_MODES = {"f32": SAT_F32, "twofloat": SAT_TWOFLOAT, "f64": SAT_F64}
 
 OSSART_GRAPH = 1
+PARAM_SLAB_TILE = 1
 
 
 class DDFanError(RuntimeError):
@@ -76,6 +77,8 @@ _SIGS = {
     "ddfan_plan_create": [ctypes.POINTER(GeomStruct), _p, _i, _p, _i, _i, ctypes.POINTER(_p)],
     "ddfan_plan_destroy": [_p],
     "ddfan_plan_query": [_p, _ip, _ip, _ip],
+    "ddfan_plan_set_param": [_p, _i, _i],
+    "ddfan_plan_get_param": [_p, _i, _ip],
     "ddfan_plan_footprint": [ctypes.POINTER(GeomStruct), _i, _i, _i, _i, ctypes.POINTER(_sz)],
     "ddfan_plan_proj": [_p, _p, _p, _i, _i, _i],
     "ddfan_plan_back": [_p, _p, _p, _i, _i, _i],

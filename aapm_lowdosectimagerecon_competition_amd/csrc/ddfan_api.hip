@@ -107,6 +107,16 @@ int check_start_idx(const int* startIdx, int nStart, int SLN) {
     return DDFAN_OK;
 }
 
+// Slabs per projector pass: the band of table rows one pass reads should stay
+// well inside an XCD's 4 MiB L2 (about 2 MiB per slice chunk).
+int default_slab_tile(int mode, int ZT, int XN, int YN) {
+    const int Nmax = XN > YN ? XN : YN;
+    const size_t row = sat_elem_bytes(mode) * (size_t)ZT * (size_t)(Nmax + 1);
+    int T = (int)((size_t)(2u << 20) / row);
+    T = T < 16 ? 16 : T;
+    return T >= Nmax ? 0 : T;
+}
+
 size_t sat_vol_bytes(int mode, int nzc, int ZT, int XN, int YN) {
     return sat_elem_bytes(mode) * (size_t)nzc * ZT * (size_t)(XN + 1) * YN;
 }
@@ -122,6 +132,7 @@ struct ddfan_plan {
     hipStream_t stream = nullptr;
     GeomDev g{};
     int PN = 0, mode = 0, ZT = 1, nzc = 1;
+    int slabTile = 0;  // projector slabs per pass (0: all slabs in one pass)
     std::vector<float> h_hangs, h_cs;
     DBuf cs, bx, by, xds, yds, mask;
     DBuf satX, satY, satP;
@@ -352,6 +363,7 @@ int ddfan_plan_create(const ddfan_geom* G, const float* hangs, int PN, const uns
         P->mode = sat_mode;
         P->ZT = choose_slice_chunk(hangs, PN, SLN, slice_chunk);
         P->nzc = (SLN + P->ZT - 1) / P->ZT;
+        P->slabTile = default_slab_tile(sat_mode, P->ZT, G->XN, G->YN);
         P->h_hangs.assign(hangs, hangs + (size_t)PN * SLN);
         P->h_cs.resize((size_t)PN * SLN * 2);
         cossin_table(hangs, (size_t)PN * SLN, P->h_cs.data());
@@ -394,6 +406,26 @@ int ddfan_plan_query(const ddfan_plan* P, int* zt, int* nzc, int* mode) {
     return DDFAN_OK;
 }
 
+int ddfan_plan_set_param(ddfan_plan* P, int param, int value) {
+    CHECK(P, DDFAN_EINVAL, "plan is NULL");
+    switch (param) {
+        case DDFAN_PARAM_SLAB_TILE:
+            CHECK(value >= 0, DDFAN_EINVAL, "slab tile must be >= 0");
+            P->slabTile = value;
+            return DDFAN_OK;
+        default:
+            return fail(DDFAN_EINVAL, "unknown plan parameter %d", param);
+    }
+}
+
+int ddfan_plan_get_param(const ddfan_plan* P, int param, int* value) {
+    CHECK(P && value, DDFAN_EINVAL, "NULL argument");
+    switch (param) {
+        case DDFAN_PARAM_SLAB_TILE: *value = P->slabTile; return DDFAN_OK;
+        default: return fail(DDFAN_EINVAL, "unknown plan parameter %d", param);
+    }
+}
+
 int ddfan_plan_sync(ddfan_plan* P) {
     CHECK(P, DDFAN_EINVAL, "plan is NULL");
     HIPCALL(hipSetDevice(P->device));
@@ -416,7 +448,7 @@ int ddfan_plan_proj(ddfan_plan* P, const float* d_vol, float* d_prj, int first, 
         return launch_proj(P->mode, P->ZT, PROJ_PLAIN, P->g, P->nzc, P->cs.as<float2>(),
                            P->bx.as<float>(), P->by.as<float>(), P->xds.as<float>(),
                            P->yds.as<float>(), P->satX.p, P->satY.p, sub, d_prj, nullptr, nullptr,
-                           P->stream);
+                           P->slabTile, P->stream);
     }));
     return DDFAN_OK;
 }
@@ -541,7 +573,7 @@ int ddfan_ossart(ddfan_plan* P, const float* d_meas, float* d_x, int numIter, in
                                 P->by.as<float>(), P->xds.as<float>(), P->yds.as<float>(),
                                 P->satX0.p, P->satY0.p, sub,
                                 P->rowSum.as<float>() + (size_t)k * maxc * DNU, nullptr, nullptr,
-                                st));
+                                0, st));
             HIPCALL(launch_sat_sino(mode, 1, DNU, 1, 1, cnt[k], P->onesP.as<float>(), P->satP0.p, st));
             HIPCALL(launch_back(mode, 1, BACK_PLAIN, g0, 1, P->cs0.as<float2>(),
                                 P->mask.as<uint8_t>(), P->satP0.p, sub,
@@ -584,7 +616,8 @@ int ddfan_ossart(ddfan_plan* P, const float* d_meas, float* d_x, int numIter, in
                                    P->bx.as<float>(), P->by.as<float>(), P->xds.as<float>(),
                                    P->yds.as<float>(), P->satX.p, P->satY.p, sub,
                                    P->rbuf.as<float>(), d_meas,
-                                   P->rowSum.as<float>() + (size_t)k * maxc * DNU, st);
+                                   P->rowSum.as<float>() + (size_t)k * maxc * DNU, P->slabTile,
+                                   st);
             });
             if (e != hipSuccess) return e;
             e = T(2, [&] {

@@ -232,6 +232,11 @@ struct Samp<SAT_F64> {
 // (T_b - T_{b-1}) / (r_b - r_{b-1}), the reference's
 // (T(realR) - T(realL)) / intersectLength (MultiSlicesProjGPU.cu:160-171,
 // 183-194), since neighbouring detectors share the boundary ray bit for bit.
+//
+// Slab passes: a launch covers slabs [ii0, ii1) only, so every block resident on
+// an XCD reads the same band of table rows and the band stays in that XCD's L2
+// for all views; pass sums accumulate in `out` (first pass writes, last pass
+// applies the epilogue), in a fixed order.
 template <int MODE, int ZT, int EPI>
 __global__ __launch_bounds__(64) void k_proj(
     const GeomDev g, const int nzc, const int nug, const int nwork,
@@ -239,7 +244,8 @@ __global__ __launch_bounds__(64) void k_proj(
     const float* __restrict__ xds, const float* __restrict__ yds,
     const typename Sat<MODE>::E* __restrict__ satX, const typename Sat<MODE>::E* __restrict__ satY,
     const Subset sub, float* __restrict__ out, const float* __restrict__ meas,
-    const float* __restrict__ rowSum) {
+    const float* __restrict__ rowSum, const int ii0, const int ii1, const int firstPass,
+    const int lastPass) {
     using E = typename Sat<MODE>::E;
     const int lin = xcd_linear(blockIdx.x, gridDim.x);
     if (lin >= nwork) return;  // whole wave
@@ -276,7 +282,8 @@ __global__ __launch_bounds__(64) void k_proj(
     float acc[ZT];
 #pragma unroll
     for (int q = 0; q < ZT; ++q) acc[q] = 0.0f;
-    for (int ii = 0; ii < Ns; ++ii) {
+    const int iiEnd = min(ii1, Ns);
+    for (int ii = ii0; ii < iiEnd; ++ii) {
         const float r = slab_r(slab_obj(ii, ocS, g.dx), pb.a, pb.f, pb.b);
         const Fetch f = fetch_pos(slab_coord(r, g.invdx, ocK), Nk);
         const float invlen = __builtin_amdgcn_rcpf(r - lane_prev(r));
@@ -293,8 +300,9 @@ __global__ __launch_bounds__(64) void k_proj(
     for (int q = 0; q < ZT; ++q) {
         const int z = z0 + q;
         if (z < g.SLN) {
-            const float p = acc[q] * constVal;
-            if constexpr (EPI == PROJ_PLAIN) {
+            float p = acc[q] * constVal;
+            if (!firstPass) p = p + out[o0 + q];
+            if (EPI == PROJ_PLAIN || !lastPass) {
                 out[o0 + q] = p;
             } else {
                 const float w = rowSum[(size_t)j * g.DNU + u];
@@ -601,25 +609,32 @@ hipError_t launch_sat_sino(int mode, int ZT, int DNU, int SLN, int nzc, int PNs,
 hipError_t launch_proj(int mode, int ZT, int epi, const GeomDev& g, int nzc, const float2* cossin,
                        const float* bx, const float* by, const float* xds, const float* yds,
                        const void* satX, const void* satY, Subset sub, float* out,
-                       const float* meas, const float* rowSum, hipStream_t st) {
+                       const float* meas, const float* rowSum, int slabTile, hipStream_t st) {
     if (sub.count <= 0) return hipSuccess;
     const int nug = (g.DNU + 62) / 63;  // 63 detectors per wave (64 boundaries)
     const long long nw = (long long)nug * sub.count * nzc;
     if (nw > 0x7fffff00LL) return hipErrorInvalidValue;
     const int nwork = (int)nw;
     dim3 grid((unsigned)xcd_grid(nwork));
-    DDF_MODE_SWITCH(mode, DDF_ZT_SWITCH(ZT, {
-        using E = typename Sat<MC>::E;
-        if (epi == PROJ_PLAIN)
-            k_proj<MC, ZTC, PROJ_PLAIN><<<grid, 64, 0, st>>>(g, nzc, nug, nwork, cossin, bx, by,
-                                                             xds, yds, (const E*)satX,
-                                                             (const E*)satY, sub, out, meas, rowSum);
-        else
-            k_proj<MC, ZTC, PROJ_RATIO><<<grid, 64, 0, st>>>(g, nzc, nug, nwork, cossin, bx, by,
-                                                             xds, yds, (const E*)satX,
-                                                             (const E*)satY, sub, out, meas, rowSum);
-    }));
-    return hipGetLastError();
+    const int Nmax = g.XN > g.YN ? g.XN : g.YN;
+    const int T = (slabTile <= 0 || slabTile > Nmax) ? Nmax : slabTile;
+    for (int ii0 = 0; ii0 < Nmax; ii0 += T) {
+        const int ii1 = ii0 + T, first = ii0 == 0, last = ii1 >= Nmax;
+        DDF_MODE_SWITCH(mode, DDF_ZT_SWITCH(ZT, {
+            using E = typename Sat<MC>::E;
+            if (epi == PROJ_PLAIN)
+                k_proj<MC, ZTC, PROJ_PLAIN><<<grid, 64, 0, st>>>(
+                    g, nzc, nug, nwork, cossin, bx, by, xds, yds, (const E*)satX, (const E*)satY,
+                    sub, out, meas, rowSum, ii0, ii1, first, last);
+            else
+                k_proj<MC, ZTC, PROJ_RATIO><<<grid, 64, 0, st>>>(
+                    g, nzc, nug, nwork, cossin, bx, by, xds, yds, (const E*)satX, (const E*)satY,
+                    sub, out, meas, rowSum, ii0, ii1, first, last);
+        }));
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_back(int mode, int ZT, int epi, const GeomDev& g, int nzc, const float2* cossin,

@@ -97,6 +97,23 @@ class Plan:
             dx.download(x)
         return x
 
+    # tunables
+    def set_param(self, param, value):
+        N.call("ddfan_plan_set_param", self.handle, int(param), int(value))
+
+    def get_param(self, param):
+        v = ctypes.c_int()
+        N.call("ddfan_plan_get_param", self.handle, int(param), ctypes.byref(v))
+        return v.value
+
+    @property
+    def slab_tile(self):
+        return self.get_param(N.PARAM_SLAB_TILE)
+
+    @slab_tile.setter
+    def slab_tile(self, v):
+        self.set_param(N.PARAM_SLAB_TILE, v)
+
     # timing and traces
     def timing(self, enable=True):
         N.call("ddfan_plan_timing", self.handle, 1 if enable else 0)

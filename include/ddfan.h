@@ -139,6 +139,12 @@ int ddfan_plan_create(const ddfan_geom* geom, const float* hangs, int PN,
                       ddfan_plan** out);
 int ddfan_plan_destroy(ddfan_plan* plan);
 int ddfan_plan_query(const ddfan_plan* plan, int* slice_chunk, int* n_chunks, int* sat_mode);
+/* Tunable launch parameters (the autotuner sets them; defaults are heuristic).
+ * DDFAN_PARAM_SLAB_TILE: projector slabs per pass (0 = one pass over all
+ * slabs); smaller passes keep the table band each pass reads inside L2. */
+#define DDFAN_PARAM_SLAB_TILE 1
+int ddfan_plan_set_param(ddfan_plan* plan, int param, int value);
+int ddfan_plan_get_param(const ddfan_plan* plan, int param, int* value);
 /* Device bytes a plan of this shape holds for full-view operators plus an
  * OS-SART run with numOS subsets (0: operators only). */
 int ddfan_plan_footprint(const ddfan_geom* geom, int PN, int sat_mode, int slice_chunk, int numOS,

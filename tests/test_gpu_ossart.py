@@ -78,3 +78,16 @@ def test_ossart_without_fista_reduces_residual(oracle):
             res.append(np.linalg.norm(plan.proj_host(x) - meas))
     print(f"[ossart] residuals {res}", flush=True)
     assert all(b < a for a, b in zip(res, res[1:]))
+
+
+def test_ossart_with_slab_passes_matches_oracle(oracle):
+    dd, conf, g, hangs, vol, mask, meas = _problem()
+    print("\n[ossart] multi-pass projector", flush=True)
+    x0 = np.zeros(vol.shape, np.float32)
+    with dd.Plan(g, hangs, mask, sat_mode="twofloat") as plan:
+        plan.slab_tile = 5
+        got = plan.ossart_host(meas, x0, 2, 8, use_fista=1, graph=True)
+    ref = oracle.ossart(g.as_dict(), hangs, meas, mask, x0, 2, 8, 1)
+    e = rel_l2(got, ref)
+    print(f"[ossart] multi-pass rel-L2 vs FP64 oracle = {e:.3e}", flush=True)
+    assert e <= 1e-5
