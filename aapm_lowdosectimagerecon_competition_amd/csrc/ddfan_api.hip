@@ -132,7 +132,8 @@ struct ddfan_plan {
     hipStream_t stream = nullptr;
     GeomDev g{};
     int PN = 0, mode = 0, ZT = 1, nzc = 1;
-    int slabTile = 0;  // projector slabs per pass (0: all slabs in one pass)
+    int slabTile = 0;      // projector slabs per pass (0: all slabs in one pass)
+    int viewsPerWave = 1;  // projector views per wave (1, 2 or 4)
     std::vector<float> h_hangs, h_cs;
     DBuf cs, bx, by, xds, yds, mask;
     DBuf satX, satY, satP;
@@ -413,6 +414,11 @@ int ddfan_plan_set_param(ddfan_plan* P, int param, int value) {
             CHECK(value >= 0, DDFAN_EINVAL, "slab tile must be >= 0");
             P->slabTile = value;
             return DDFAN_OK;
+        case DDFAN_PARAM_VIEWS_PER_WAVE:
+            CHECK(value == 1 || value == 2 || value == 4, DDFAN_EINVAL,
+                  "views per wave must be 1, 2 or 4");
+            P->viewsPerWave = value;
+            return DDFAN_OK;
         default:
             return fail(DDFAN_EINVAL, "unknown plan parameter %d", param);
     }
@@ -422,6 +428,7 @@ int ddfan_plan_get_param(const ddfan_plan* P, int param, int* value) {
     CHECK(P && value, DDFAN_EINVAL, "NULL argument");
     switch (param) {
         case DDFAN_PARAM_SLAB_TILE: *value = P->slabTile; return DDFAN_OK;
+        case DDFAN_PARAM_VIEWS_PER_WAVE: *value = P->viewsPerWave; return DDFAN_OK;
         default: return fail(DDFAN_EINVAL, "unknown plan parameter %d", param);
     }
 }
@@ -448,7 +455,7 @@ int ddfan_plan_proj(ddfan_plan* P, const float* d_vol, float* d_prj, int first, 
         return launch_proj(P->mode, P->ZT, PROJ_PLAIN, P->g, P->nzc, P->cs.as<float2>(),
                            P->bx.as<float>(), P->by.as<float>(), P->xds.as<float>(),
                            P->yds.as<float>(), P->satX.p, P->satY.p, sub, d_prj, nullptr, nullptr,
-                           P->slabTile, P->stream);
+                           P->slabTile, P->viewsPerWave, P->stream);
     }));
     return DDFAN_OK;
 }
@@ -573,7 +580,7 @@ int ddfan_ossart(ddfan_plan* P, const float* d_meas, float* d_x, int numIter, in
                                 P->by.as<float>(), P->xds.as<float>(), P->yds.as<float>(),
                                 P->satX0.p, P->satY0.p, sub,
                                 P->rowSum.as<float>() + (size_t)k * maxc * DNU, nullptr, nullptr,
-                                0, st));
+                                0, 1, st));
             HIPCALL(launch_sat_sino(mode, 1, DNU, 1, 1, cnt[k], P->onesP.as<float>(), P->satP0.p, st));
             HIPCALL(launch_back(mode, 1, BACK_PLAIN, g0, 1, P->cs0.as<float2>(),
                                 P->mask.as<uint8_t>(), P->satP0.p, sub,
@@ -617,7 +624,7 @@ int ddfan_ossart(ddfan_plan* P, const float* d_meas, float* d_x, int numIter, in
                                    P->yds.as<float>(), P->satX.p, P->satY.p, sub,
                                    P->rbuf.as<float>(), d_meas,
                                    P->rowSum.as<float>() + (size_t)k * maxc * DNU, P->slabTile,
-                                   st);
+                                   P->viewsPerWave, st);
             });
             if (e != hipSuccess) return e;
             e = T(2, [&] {

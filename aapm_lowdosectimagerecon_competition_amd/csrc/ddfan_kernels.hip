@@ -61,19 +61,26 @@ __device__ __forceinline__ double lane_prev(double v) {
 }
 
 // Fetch position of a texture coordinate c on a table with entries 0..N:
-// i0 = clamp(floor(c - 0.5), 0, N) and the weight t, forced to 0 where the
-// clamp-to-edge fetch reads one texel twice.
+// i0 = clamp(floor(c - 0.5), 0, N) as a 32-bit byte offset of entry type E,
+// and the weight t, forced to 0 where the clamp-to-edge fetch reads one texel
+// twice.  Rows are addressed as (uniform row base) + (32-bit lane offset) so
+// loads use the scalar-base addressing form.
 struct Fetch {
-    unsigned i0;
+    unsigned off;
     float t;
 };
+template <class E>
 __device__ __forceinline__ Fetch fetch_pos(float c, int N) {
     const int i = texel(c, N);
     const int i0 = min(max(i, 0), N);
     Fetch f;
-    f.i0 = (unsigned)i0;
+    f.off = (unsigned)i0 * (unsigned)sizeof(E);
     f.t = (i < 0 || i >= N) ? 0.0f : (c - 0.5f) - (float)i;
     return f;
+}
+template <class E>
+__device__ __forceinline__ E entry_at(const E* __restrict__ row, unsigned off) {
+    return *reinterpret_cast<const E*>(reinterpret_cast<const char*>(row) + off);
 }
 
 // ------------------------------------------------------------------ SAT
@@ -225,21 +232,23 @@ struct Samp<SAT_F64> {
 };
 
 // ------------------------------------------------------------------ projector
-// Branchless DD projector.  One wave = one (view, slice chunk, group of 63
-// detectors); lane i owns detector boundary ray b = 63*group + i, so every
+// Branchless DD projector.  One wave = V views x one slice chunk x one group of
+// 63 detectors; lane i owns detector boundary ray b = 63*group + i, so every
 // boundary is intersected with each slab once and its table sample T_b is
 // handed to the next lane through DPP: detector u = b - 1 accumulates
 // (T_b - T_{b-1}) / (r_b - r_{b-1}), the reference's
 // (T(realR) - T(realL)) / intersectLength (MultiSlicesProjGPU.cu:160-171,
 // 183-194), since neighbouring detectors share the boundary ray bit for bit.
+// The V views of a wave are consecutive views of the subset: at each slab they
+// read overlapping stretches of the same table rows, which then come from L1.
 //
 // Slab passes: a launch covers slabs [ii0, ii1) only, so every block resident on
 // an XCD reads the same band of table rows and the band stays in that XCD's L2
 // for all views; pass sums accumulate in `out` (first pass writes, last pass
 // applies the epilogue), in a fixed order.
-template <int MODE, int ZT, int EPI>
+template <int MODE, int ZT, int EPI, int V>
 __global__ __launch_bounds__(64) void k_proj(
-    const GeomDev g, const int nzc, const int nug, const int nwork,
+    const GeomDev g, const int nzc, const int nug, const int nvg, const int nwork,
     const float2* __restrict__ cossin, const float* __restrict__ bx, const float* __restrict__ by,
     const float* __restrict__ xds, const float* __restrict__ yds,
     const typename Sat<MODE>::E* __restrict__ satX, const typename Sat<MODE>::E* __restrict__ satY,
@@ -251,63 +260,87 @@ __global__ __launch_bounds__(64) void k_proj(
     if (lin >= nwork) return;  // whole wave
     const int ug = lin % nug;
     const int rest = lin / nug;
-    const int j = rest % sub.count;
-    const int zc = rest / sub.count;
+    const int jg = rest % nvg;
+    const int zc = rest / nvg;
 
     const int lane = threadIdx.x;
     const int bnd = min(ug * 63 + lane, g.DNU);
     const int u = ug * 63 + lane - 1;
     const bool act = lane >= 1 && u < g.DNU;
     const int uu = min(max(u, 0), g.DNU - 1);
-    const int vg = sub.first + j * sub.stride;
     const int z0 = zc * ZT;
-    const float2 cs = cossin[(size_t)vg * g.SLN + min(z0, g.SLN - 1)];
-    const ProjBoundary pb = proj_boundary(g.x0, g.y0, cs.x, cs.y, bx[bnd], by[bnd]);
+    const float bxb = bx[bnd], byb = by[bnd], xdu = xds[uu], ydu = yds[uu];
 
-    // continuous: dx / |normalize(det - src)| component (MultiSlicesProjGPU.cu:139,158,181)
-    const float srcx = g.x0 * cs.x - g.y0 * cs.y, srcy = g.x0 * cs.y + g.y0 * cs.x;
-    const float ddx = (xds[uu] * cs.x - yds[uu] * cs.y) - srcx;
-    const float ddy = (xds[uu] * cs.y + yds[uu] * cs.x) - srcy;
-    const float nrm = sqrtf(fmaf(ddx, ddx, ddy * ddy));
-    const float constVal = g.dx * nrm / fabsf(pb.xslab ? ddx : ddy);
-
-    const int Ns = pb.xslab ? g.XN : g.YN;
-    const int Nk = pb.xslab ? g.YN : g.XN;
-    const int K = Nk + 1;
-    const float ocS = pb.xslab ? g.ocxP : g.ocyP;
-    const float ocK = pb.xslab ? g.ocyP : g.ocxP;
-    const size_t rowStride = (size_t)ZT * K;
-    const E* sat = (pb.xslab ? satY : satX) + (size_t)zc * Ns * rowStride;
-
-    float acc[ZT];
+    float pa[V], pf[V], pbb[V], cval[V], ocSv[V], ocKv[V];
+    int NsV[V], NkV[V];
+    const E* satv[V];
 #pragma unroll
-    for (int q = 0; q < ZT; ++q) acc[q] = 0.0f;
-    const int iiEnd = min(ii1, Ns);
+    for (int v = 0; v < V; ++v) {
+        const int j = min(jg * V + v, sub.count - 1);
+        const int vg = sub.first + j * sub.stride;
+        const float2 cs = cossin[(size_t)vg * g.SLN + min(z0, g.SLN - 1)];
+        const ProjBoundary pb = proj_boundary(g.x0, g.y0, cs.x, cs.y, bxb, byb);
+        // continuous: dx / |normalize(det - src)| component (MultiSlicesProjGPU.cu:139,158,181)
+        const float srcx = g.x0 * cs.x - g.y0 * cs.y, srcy = g.x0 * cs.y + g.y0 * cs.x;
+        const float ddx = (xdu * cs.x - ydu * cs.y) - srcx;
+        const float ddy = (xdu * cs.y + ydu * cs.x) - srcy;
+        const float nrm = sqrtf(fmaf(ddx, ddx, ddy * ddy));
+        cval[v] = g.dx * nrm / fabsf(pb.xslab ? ddx : ddy);
+        pa[v] = pb.a;
+        pf[v] = pb.f;
+        pbb[v] = pb.b;
+        NsV[v] = pb.xslab ? g.XN : g.YN;
+        NkV[v] = pb.xslab ? g.YN : g.XN;
+        ocSv[v] = pb.xslab ? g.ocxP : g.ocyP;
+        ocKv[v] = pb.xslab ? g.ocyP : g.ocxP;
+        satv[v] = (pb.xslab ? satY : satX) + (size_t)zc * NsV[v] * (size_t)ZT * (NkV[v] + 1);
+    }
+
+    float acc[V][ZT];
+#pragma unroll
+    for (int v = 0; v < V; ++v)
+#pragma unroll
+        for (int q = 0; q < ZT; ++q) acc[v][q] = 0.0f;
+    int iiEnd = 0;
+#pragma unroll
+    for (int v = 0; v < V; ++v) iiEnd = max(iiEnd, min(ii1, NsV[v]));
     for (int ii = ii0; ii < iiEnd; ++ii) {
-        const float r = slab_r(slab_obj(ii, ocS, g.dx), pb.a, pb.f, pb.b);
-        const Fetch f = fetch_pos(slab_coord(r, g.invdx, ocK), Nk);
-        const float invlen = __builtin_amdgcn_rcpf(r - lane_prev(r));
-        const E* row = sat + (size_t)ii * rowStride;
 #pragma unroll
-        for (int q = 0; q < ZT; ++q) {
-            const Samp<MODE> T(row[(size_t)q * K + f.i0], f.t);
-            acc[q] = fmaf(T.minus_prev_lane(), invlen, acc[q]);
+        for (int v = 0; v < V; ++v) {
+            if (ii < NsV[v]) {  // uniform across the wave
+                const int K = NkV[v] + 1;
+                const float r = slab_r(slab_obj(ii, ocSv[v], g.dx), pa[v], pf[v], pbb[v]);
+                const Fetch f = fetch_pos<E>(slab_coord(r, g.invdx, ocKv[v]), NkV[v]);
+                const float invlen = __builtin_amdgcn_rcpf(r - lane_prev(r));
+                const E* row = satv[v] + (size_t)ii * ZT * K;
+#pragma unroll
+                for (int q = 0; q < ZT; ++q) {
+                    const Samp<MODE> T(entry_at(row + (size_t)q * K, f.off), f.t);
+                    acc[v][q] = fmaf(T.minus_prev_lane(), invlen, acc[v][q]);
+                }
+            }
         }
     }
     if (!act) return;
-    const size_t o0 = ((size_t)j * g.DNU + u) * g.SLN + z0;
 #pragma unroll
-    for (int q = 0; q < ZT; ++q) {
-        const int z = z0 + q;
-        if (z < g.SLN) {
-            float p = acc[q] * constVal;
-            if (!firstPass) p = p + out[o0 + q];
-            if (EPI == PROJ_PLAIN || !lastPass) {
-                out[o0 + q] = p;
-            } else {
-                const float w = rowSum[(size_t)j * g.DNU + u];
-                const float b = meas[((size_t)vg * g.DNU + u) * g.SLN + z];
-                out[o0 + q] = (w != 0.0f) ? (b - p) / w : p;
+    for (int v = 0; v < V; ++v) {
+        const int j = jg * V + v;
+        if (j >= sub.count) break;
+        const int vg = sub.first + j * sub.stride;
+        const size_t o0 = ((size_t)j * g.DNU + u) * g.SLN + z0;
+#pragma unroll
+        for (int q = 0; q < ZT; ++q) {
+            const int z = z0 + q;
+            if (z < g.SLN) {
+                float p = acc[v][q] * cval[v];
+                if (!firstPass) p = p + out[o0 + q];
+                if (EPI == PROJ_PLAIN || !lastPass) {
+                    out[o0 + q] = p;
+                } else {
+                    const float w = rowSum[(size_t)j * g.DNU + u];
+                    const float b = meas[((size_t)vg * g.DNU + u) * g.SLN + z];
+                    out[o0 + q] = (w != 0.0f) ? (b - p) / w : p;
+                }
             }
         }
     }
@@ -353,14 +386,14 @@ __global__ __launch_bounds__(256) void k_back(
             const float2 cs = cossin[(size_t)vg * g.SLN + zg];
             const BackSample bs =
                 back_sample(vx, vy, cs.x, cs.y, g.x0, g.y0, g.inv_sid, g.invdb, g.detCntIdx, g.dx);
-            const Fetch fL = fetch_pos(bs.idxL, DNU), fR = fetch_pos(bs.idxR, DNU);
+            const Fetch fL = fetch_pos<E>(bs.idxL, DNU), fR = fetch_pos<E>(bs.idxR, DNU);
             // dx / ddv * sqrt(l2) with sqrt(l2) = 1 / l  (MultiSlicesBackGPU.cu:173)
             const float cosVal = g.dx * __builtin_amdgcn_rcpf(bs.ddv * bs.l);
             const E* row = Pc + (size_t)j * viewStride;
 #pragma unroll
             for (int q = 0; q < ZT; ++q) {
-                const Samp<MODE> TL(row[(size_t)q * K + fL.i0], fL.t);
-                const Samp<MODE> TR(row[(size_t)q * K + fR.i0], fR.t);
+                const Samp<MODE> TL(entry_at(row + (size_t)q * K, fL.off), fL.t);
+                const Samp<MODE> TR(entry_at(row + (size_t)q * K, fR.off), fR.t);
                 acc[q] = fmaf(TR.minus(TL), cosVal, acc[q]);
             }
         }
@@ -609,10 +642,13 @@ hipError_t launch_sat_sino(int mode, int ZT, int DNU, int SLN, int nzc, int PNs,
 hipError_t launch_proj(int mode, int ZT, int epi, const GeomDev& g, int nzc, const float2* cossin,
                        const float* bx, const float* by, const float* xds, const float* yds,
                        const void* satX, const void* satY, Subset sub, float* out,
-                       const float* meas, const float* rowSum, int slabTile, hipStream_t st) {
+                       const float* meas, const float* rowSum, int slabTile, int viewsPerWave,
+                       hipStream_t st) {
     if (sub.count <= 0) return hipSuccess;
+    const int V = viewsPerWave == 2 || viewsPerWave == 4 ? viewsPerWave : 1;
     const int nug = (g.DNU + 62) / 63;  // 63 detectors per wave (64 boundaries)
-    const long long nw = (long long)nug * sub.count * nzc;
+    const int nvg = (sub.count + V - 1) / V;
+    const long long nw = (long long)nug * nvg * nzc;
     if (nw > 0x7fffff00LL) return hipErrorInvalidValue;
     const int nwork = (int)nw;
     dim3 grid((unsigned)xcd_grid(nwork));
@@ -620,17 +656,26 @@ hipError_t launch_proj(int mode, int ZT, int epi, const GeomDev& g, int nzc, con
     const int T = (slabTile <= 0 || slabTile > Nmax) ? Nmax : slabTile;
     for (int ii0 = 0; ii0 < Nmax; ii0 += T) {
         const int ii1 = ii0 + T, first = ii0 == 0, last = ii1 >= Nmax;
-        DDF_MODE_SWITCH(mode, DDF_ZT_SWITCH(ZT, {
-            using E = typename Sat<MC>::E;
-            if (epi == PROJ_PLAIN)
-                k_proj<MC, ZTC, PROJ_PLAIN><<<grid, 64, 0, st>>>(
-                    g, nzc, nug, nwork, cossin, bx, by, xds, yds, (const E*)satX, (const E*)satY,
-                    sub, out, meas, rowSum, ii0, ii1, first, last);
-            else
-                k_proj<MC, ZTC, PROJ_RATIO><<<grid, 64, 0, st>>>(
-                    g, nzc, nug, nwork, cossin, bx, by, xds, yds, (const E*)satX, (const E*)satY,
-                    sub, out, meas, rowSum, ii0, ii1, first, last);
-        }));
+#define DDF_PROJ(VV)                                                                               \
+    DDF_MODE_SWITCH(mode, DDF_ZT_SWITCH(ZT, {                                                      \
+        using E = typename Sat<MC>::E;                                                             \
+        if (epi == PROJ_PLAIN)                                                                     \
+            k_proj<MC, ZTC, PROJ_PLAIN, VV><<<grid, 64, 0, st>>>(                                  \
+                g, nzc, nug, nvg, nwork, cossin, bx, by, xds, yds, (const E*)satX, (const E*)satY, \
+                sub, out, meas, rowSum, ii0, ii1, first, last);                                    \
+        else                                                                                       \
+            k_proj<MC, ZTC, PROJ_RATIO, VV><<<grid, 64, 0, st>>>(                                  \
+                g, nzc, nug, nvg, nwork, cossin, bx, by, xds, yds, (const E*)satX, (const E*)satY, \
+                sub, out, meas, rowSum, ii0, ii1, first, last);                                    \
+    }))
+        if (V == 4) {
+            DDF_PROJ(4);
+        } else if (V == 2) {
+            DDF_PROJ(2);
+        } else {
+            DDF_PROJ(1);
+        }
+#undef DDF_PROJ
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

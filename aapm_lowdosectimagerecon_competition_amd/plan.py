@@ -114,6 +114,14 @@ class Plan:
     def slab_tile(self, v):
         self.set_param(N.PARAM_SLAB_TILE, v)
 
+    @property
+    def views_per_wave(self):
+        return self.get_param(N.PARAM_VIEWS_PER_WAVE)
+
+    @views_per_wave.setter
+    def views_per_wave(self, v):
+        self.set_param(N.PARAM_VIEWS_PER_WAVE, v)
+
     # timing and traces
     def timing(self, enable=True):
         N.call("ddfan_plan_timing", self.handle, 1 if enable else 0)

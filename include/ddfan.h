@@ -143,6 +143,9 @@ int ddfan_plan_query(const ddfan_plan* plan, int* slice_chunk, int* n_chunks, in
  * DDFAN_PARAM_SLAB_TILE: projector slabs per pass (0 = one pass over all
  * slabs); smaller passes keep the table band each pass reads inside L2. */
 #define DDFAN_PARAM_SLAB_TILE 1
+/* DDFAN_PARAM_VIEWS_PER_WAVE: consecutive views one projector wave handles
+ * (1, 2 or 4); they share table rows through L1. */
+#define DDFAN_PARAM_VIEWS_PER_WAVE 2
 int ddfan_plan_set_param(ddfan_plan* plan, int param, int value);
 int ddfan_plan_get_param(const ddfan_plan* plan, int param, int* value);
 /* Device bytes a plan of this shape holds for full-view operators plus an
