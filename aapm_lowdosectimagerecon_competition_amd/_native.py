@@ -38,6 +38,8 @@ SAT_MODES = {"f32": SAT_F32, "twofloat": SAT_TWOFLOAT, "f64": SAT_F64}
 OSSART_GRAPH = 1
 PARAM_SLAB_TILE = 1
 PARAM_VIEWS_PER_WAVE = 2
+PARAM_PROJ_CHUNK = 3
+PARAM_BACK_CHUNK = 4
 
 
 class DDFanError(RuntimeError):

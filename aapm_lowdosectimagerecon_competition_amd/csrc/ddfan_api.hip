@@ -107,16 +107,6 @@ int check_start_idx(const int* startIdx, int nStart, int SLN) {
     return DDFAN_OK;
 }
 
-// Slabs per projector pass: the band of table rows one pass reads should stay
-// well inside an XCD's 4 MiB L2 (about 2 MiB per slice chunk).
-int default_slab_tile(int mode, int ZT, int XN, int YN) {
-    const int Nmax = XN > YN ? XN : YN;
-    const size_t row = sat_elem_bytes(mode) * (size_t)ZT * (size_t)(Nmax + 1);
-    int T = (int)((size_t)(2u << 20) / row);
-    T = T < 16 ? 16 : T;
-    return T >= Nmax ? 0 : T;
-}
-
 size_t sat_vol_bytes(int mode, int nzc, int ZT, int XN, int YN) {
     return sat_elem_bytes(mode) * (size_t)nzc * ZT * (size_t)(XN + 1) * YN;
 }
@@ -131,9 +121,12 @@ struct ddfan_plan {
     int device = 0;
     hipStream_t stream = nullptr;
     GeomDev g{};
-    int PN = 0, mode = 0, ZT = 1, nzc = 1;
+    int PN = 0, mode = 0;
+    int ZTmax = 1;       // widest chunk of consecutive slices with identical angles
+    int ZT = 1, nzc = 1;   // projector slice chunk and chunk count
+    int ZTb = 1, nzcb = 1; // backprojector slice chunk and chunk count
     int slabTile = 0;      // projector slabs per pass (0: all slabs in one pass)
-    int viewsPerWave = 1;  // projector views per wave (1, 2 or 4)
+    int viewsPerWave = 4;  // projector views per wave (1, 2 or 4)
     std::vector<float> h_hangs, h_cs;
     DBuf cs, bx, by, xds, yds, mask;
     DBuf satX, satY, satP;
@@ -362,9 +355,16 @@ int ddfan_plan_create(const ddfan_geom* G, const float* hangs, int PN, const uns
         fill_geom(P->g, G, SLN);
         P->PN = PN;
         P->mode = sat_mode;
-        P->ZT = choose_slice_chunk(hangs, PN, SLN, slice_chunk);
+        P->ZTmax = choose_slice_chunk(hangs, PN, SLN, slice_chunk);
+        // Launch-shape defaults measured on MI355X at 512x512x64, 2304 views:
+        // the projector runs best on chunks of <= 8 slices with 4 views per
+        // wave in one slab pass, the backprojector on the widest chunk.
+        P->ZT = std::min(P->ZTmax, 8);
         P->nzc = (SLN + P->ZT - 1) / P->ZT;
-        P->slabTile = default_slab_tile(sat_mode, P->ZT, G->XN, G->YN);
+        P->ZTb = P->ZTmax;
+        P->nzcb = (SLN + P->ZTb - 1) / P->ZTb;
+        P->viewsPerWave = 4;
+        P->slabTile = 0;
         P->h_hangs.assign(hangs, hangs + (size_t)PN * SLN);
         P->h_cs.resize((size_t)PN * SLN * 2);
         cossin_table(hangs, (size_t)PN * SLN, P->h_cs.data());
@@ -401,8 +401,8 @@ int ddfan_plan_destroy(ddfan_plan* P) {
 
 int ddfan_plan_query(const ddfan_plan* P, int* zt, int* nzc, int* mode) {
     CHECK(P, DDFAN_EINVAL, "plan is NULL");
-    if (zt) *zt = P->ZT;
-    if (nzc) *nzc = P->nzc;
+    if (zt) *zt = P->ZTmax;
+    if (nzc) *nzc = (P->g.SLN + P->ZTmax - 1) / P->ZTmax;
     if (mode) *mode = P->mode;
     return DDFAN_OK;
 }
@@ -419,6 +419,23 @@ int ddfan_plan_set_param(ddfan_plan* P, int param, int value) {
                   "views per wave must be 1, 2 or 4");
             P->viewsPerWave = value;
             return DDFAN_OK;
+        case DDFAN_PARAM_PROJ_CHUNK:
+        case DDFAN_PARAM_BACK_CHUNK: {
+            CHECK(value >= 1 && value <= P->ZTmax && (value & (value - 1)) == 0, DDFAN_EINVAL,
+                  "slice chunk must be a power of two <= %d (slices with identical angles)",
+                  P->ZTmax);
+            HIPCALL(hipSetDevice(P->device));
+            HIPCALL(hipStreamSynchronize(P->stream));
+            const int n = (P->g.SLN + value - 1) / value;
+            if (param == DDFAN_PARAM_PROJ_CHUNK) {
+                P->ZT = value;
+                P->nzc = n;
+            } else {
+                P->ZTb = value;
+                P->nzcb = n;
+            }
+            return DDFAN_OK;
+        }
         default:
             return fail(DDFAN_EINVAL, "unknown plan parameter %d", param);
     }
@@ -429,6 +446,8 @@ int ddfan_plan_get_param(const ddfan_plan* P, int param, int* value) {
     switch (param) {
         case DDFAN_PARAM_SLAB_TILE: *value = P->slabTile; return DDFAN_OK;
         case DDFAN_PARAM_VIEWS_PER_WAVE: *value = P->viewsPerWave; return DDFAN_OK;
+        case DDFAN_PARAM_PROJ_CHUNK: *value = P->ZT; return DDFAN_OK;
+        case DDFAN_PARAM_BACK_CHUNK: *value = P->ZTb; return DDFAN_OK;
         default: return fail(DDFAN_EINVAL, "unknown plan parameter %d", param);
     }
 }
@@ -440,12 +459,25 @@ int ddfan_plan_sync(ddfan_plan* P) {
     return DDFAN_OK;
 }
 
+// Volume summed-area tables sized for the projector's current slice chunking.
+static int ensure_vol_sat(ddfan_plan* P) {
+    const size_t bx = sat_vol_bytes(P->mode, P->nzc, P->ZT, P->g.XN, P->g.YN);
+    const size_t by = sat_vol_bytes(P->mode, P->nzc, P->ZT, P->g.YN, P->g.XN);
+    if (bx > P->satX.bytes || by > P->satY.bytes) {
+        HIPCALL(hipStreamSynchronize(P->stream));
+        HIPCALL(P->satX.reserve(bx));
+        HIPCALL(P->satY.reserve(by));
+    }
+    return DDFAN_OK;
+}
+
 int ddfan_plan_proj(ddfan_plan* P, const float* d_vol, float* d_prj, int first, int stride,
                     int count) {
     CHECK(P && d_vol && d_prj, DDFAN_EINVAL, "NULL argument");
     TRY(check_subset(P->PN, first, stride, count));
     if (count == 0) return DDFAN_OK;
     HIPCALL(hipSetDevice(P->device));
+    TRY(ensure_vol_sat(P));
     const Subset sub{first, stride, count};
     HIPCALL(P->timed(0, [&] {
         return launch_sat_vol(P->mode, P->ZT, P->g, P->nzc, d_vol, P->mask.as<uint8_t>(), P->satX.p,
@@ -466,17 +498,17 @@ int ddfan_plan_back(ddfan_plan* P, const float* d_prj, float* d_vol, int first, 
     TRY(check_subset(P->PN, first, stride, count));
     HIPCALL(hipSetDevice(P->device));
     const Subset sub{first, stride, count};
-    const size_t need = sat_sino_bytes(P->mode, P->nzc, P->ZT, P->g.DNU, std::max(count, 1));
+    const size_t need = sat_sino_bytes(P->mode, P->nzcb, P->ZTb, P->g.DNU, std::max(count, 1));
     if (need > P->satP.bytes) {
         HIPCALL(hipStreamSynchronize(P->stream));
         HIPCALL(P->satP.reserve(need));
     }
     HIPCALL(P->timed(2, [&] {
-        return launch_sat_sino(P->mode, P->ZT, P->g.DNU, P->g.SLN, P->nzc, count, d_prj, P->satP.p,
+        return launch_sat_sino(P->mode, P->ZTb, P->g.DNU, P->g.SLN, P->nzcb, count, d_prj, P->satP.p,
                                P->stream);
     }));
     HIPCALL(P->timed(3, [&] {
-        return launch_back(P->mode, P->ZT, BACK_PLAIN, P->g, P->nzc, P->cs.as<float2>(),
+        return launch_back(P->mode, P->ZTb, BACK_PLAIN, P->g, P->nzcb, P->cs.as<float2>(),
                            P->mask.as<uint8_t>(), P->satP.p, sub, d_vol, nullptr, nullptr, nullptr,
                            nullptr, P->stream);
     }));
@@ -548,10 +580,11 @@ int ddfan_ossart(ddfan_plan* P, const float* d_meas, float* d_x, int numIter, in
     HIPCALL(P->rowSum.reserve(sizeof(float) * (size_t)numOS * maxc * DNU));
     HIPCALL(P->colSum.reserve(sizeof(float) * (size_t)numOS * npix));
     HIPCALL(P->rbuf.reserve(sizeof(float) * (size_t)maxc * DNU * SLN));
-    HIPCALL(P->satP.reserve(std::max(P->satP.bytes, sat_sino_bytes(mode, P->nzc, P->ZT, DNU, maxc))));
+    HIPCALL(P->satP.reserve(std::max(P->satP.bytes, sat_sino_bytes(mode, P->nzcb, P->ZTb, DNU, maxc))));
     HIPCALL(P->momTab.reserve(sizeof(float) * std::max(numIter, 1)));
     HIPCALL(P->counter.reserve(sizeof(int)));
     if (useFISTA) HIPCALL(P->xprev.reserve(sizeof(float) * nvox));
+    TRY(ensure_vol_sat(P));
 
     // OS-SART weights from slice 0, replicated to every slice (OSSART_AAPM.m:59-66):
     // rowSum_k = A_k(mask*1), colSum_k = A_k^T(1), both on a one-slice geometry
@@ -628,13 +661,13 @@ int ddfan_ossart(ddfan_plan* P, const float* d_meas, float* d_x, int numIter, in
             });
             if (e != hipSuccess) return e;
             e = T(2, [&] {
-                return launch_sat_sino(mode, P->ZT, DNU, SLN, P->nzc, cnt[k], P->rbuf.as<float>(),
+                return launch_sat_sino(mode, P->ZTb, DNU, SLN, P->nzcb, cnt[k], P->rbuf.as<float>(),
                                        P->satP.p, st);
             });
             if (e != hipSuccess) return e;
             const int epi = (useFISTA && k == numOS - 1) ? BACK_UPDATE_FISTA : BACK_UPDATE;
             e = T(3, [&] {
-                return launch_back(mode, P->ZT, epi, g, P->nzc, P->cs.as<float2>(),
+                return launch_back(mode, P->ZTb, epi, g, P->nzcb, P->cs.as<float2>(),
                                    P->mask.as<uint8_t>(), P->satP.p, sub, d_x,
                                    P->colSum.as<float>() + (size_t)k * npix, P->xprev.as<float>(),
                                    P->momTab.as<float>(), P->counter.as<int>(), st);

@@ -42,18 +42,69 @@ struct Subset {
 enum ProjEpi : int { PROJ_PLAIN = 0, PROJ_RATIO = 1 };
 enum BackEpi : int { BACK_PLAIN = 0, BACK_UPDATE = 1, BACK_UPDATE_FISTA = 2 };
 
+// Batched launches.  Every operator launch takes a list of independent jobs
+// (one per reconstruction); jobs sharing the kernel's template shape go into
+// one launch of up to kBatch jobs, whose blocks are laid out job after job.
+constexpr int kBatch = 8;
+
+struct SatVolJob {  // both volume tables of one reconstruction
+    GeomDev g;
+    int nzc, ZT;
+    const float* vol;
+    const uint8_t* mask;
+    void* satX;
+    void* satY;
+};
+struct SatSinoJob {  // detector-axis table of one sinogram subset
+    int DNU, SLN, nzc, ZT, PNs;
+    const float* r;
+    void* P;
+};
+struct ProjJob {
+    GeomDev g;
+    Subset sub;
+    int nzc, ZT, slabTile, epi;
+    const float2* cossin;
+    const float* bx;
+    const float* by;
+    const float* xds;
+    const float* yds;
+    const void* satX;
+    const void* satY;
+    float* out;
+    const float* meas;
+    const float* rowSum;
+};
+struct BackJob {
+    GeomDev g;
+    Subset sub;
+    int nzc, ZT, epi;
+    const float2* cossin;
+    const uint8_t* mask;
+    const void* P;
+    float* vol;
+    const float* colSum;
+    float* xprev;
+    const float* momTab;
+    const int* counter;
+};
+
+hipError_t launch_sat_vol_batch(int mode, const SatVolJob* jobs, int n, hipStream_t st);
+hipError_t launch_sat_sino_batch(int mode, const SatSinoJob* jobs, int n, hipStream_t st);
+hipError_t launch_proj_batch(int mode, int viewsPerWave, const ProjJob* jobs, int n, hipStream_t st);
+hipError_t launch_back_batch(int mode, const BackJob* jobs, int n, hipStream_t st);
+hipError_t launch_counter_inc_batch(int* const* counters, int n, hipStream_t st);
+
+// Single-job forms.
 hipError_t launch_sat_vol(int mode, int ZT, const GeomDev& g, int nzc, const float* vol,
                           const uint8_t* mask, void* satX, void* satY, hipStream_t st);
-
 hipError_t launch_sat_sino(int mode, int ZT, int DNU, int SLN, int nzc, int PNs, const float* r,
                            void* P, hipStream_t st);
-
 hipError_t launch_proj(int mode, int ZT, int epi, const GeomDev& g, int nzc, const float2* cossin,
                        const float* bx, const float* by, const float* xds, const float* yds,
                        const void* satX, const void* satY, Subset sub, float* out,
                        const float* meas, const float* rowSum, int slabTile, int viewsPerWave,
                        hipStream_t st);
-
 hipError_t launch_back(int mode, int ZT, int epi, const GeomDev& g, int nzc, const float2* cossin,
                        const uint8_t* mask, const void* P, Subset sub, float* vol,
                        const float* colSum, float* xprev, const float* momTab,

@@ -140,35 +140,41 @@ struct Scan<SAT_F64> {
 // consecutive slices: 256/G threads per slice own contiguous k segments
 // (loads coalesce over the slices, contiguous in memory), segment totals are
 // combined through LDS, and finished rows leave through LDS as contiguous k
-// runs of paired entries.
-struct SatJob {
+// runs of paired entries.  A launch holds several tables (both axes of a
+// volume, or the tables of several reconstructions), block ranges one after
+// another.
+struct SatDev {
     const float* in;
     const uint8_t* mask;
     void* out;
-    int nrows, Nk, rs, ks;
+    int nrows, Nk, rs, ks, SLN, ZT, nzc, nzg, blk0;
 };
-struct SatJobs {
-    SatJob j[2];
+struct SatLaunch {
+    int njobs;
+    SatDev j[2 * kBatch];
 };
 
 template <int MODE, int G>
-__global__ __launch_bounds__(256) void k_sat_rows(const SatJobs jobs, const int SLN, const int ZT,
-                                                  const int nzc, const int nzg) {
+__global__ __launch_bounds__(256) void k_sat_rows(const SatLaunch L) {
     using S = Scan<MODE>;
-    using L = typename S::L;
+    using Lt = typename S::L;
     using E = typename Sat<MODE>::E;
     constexpr int NSEG = 256 / G;
     extern __shared__ __align__(16) unsigned char smem[];
-    const SatJob jb = jobs.j[blockIdx.y];
+    const int bid = blockIdx.x;
+    int ji = 0;
+    for (int t = 1; t < L.njobs; ++t)
+        if (bid >= L.j[t].blk0) ji = t;
+    const SatDev& jb = L.j[ji];
+    const int local = bid - jb.blk0;
     const int K = jb.Nk + 1;
-    const int row = blockIdx.x / nzg;
-    const int zg = blockIdx.x - row * nzg;
-    if (row >= jb.nrows) return;  // whole block
+    const int row = local / jb.nzg;
+    const int zg = local - row * jb.nzg;
     const int tz = threadIdx.x % G, seg = threadIdx.x / G;
     const int z = zg * G + tz;
-    const bool real = z < SLN;
-    L* pre = reinterpret_cast<L*>(smem);          // [G][K] exclusive prefixes
-    L* tot = pre + (size_t)G * K;                 // [G][NSEG] segment totals
+    const bool real = z < jb.SLN;
+    Lt* pre = reinterpret_cast<Lt*>(smem);  // [G][K] exclusive prefixes
+    Lt* tot = pre + (size_t)G * K;          // [G][NSEG] segment totals
     float* val = reinterpret_cast<float*>(tot + (size_t)G * NSEG);  // [G][K] samples
     const int Lseg = (jb.Nk + NSEG - 1) / NSEG;
     const int k0 = min(seg * Lseg, jb.Nk), k1 = min(k0 + Lseg, jb.Nk);
@@ -177,7 +183,7 @@ __global__ __launch_bounds__(256) void k_sat_rows(const SatJobs jobs, const int 
         float v = 0.0f;
         if (real) {
             const size_t p = (size_t)row * jb.rs + (size_t)k * jb.ks;
-            v = jb.in[p * SLN + z];
+            v = jb.in[p * jb.SLN + z];
             if (jb.mask) v = v * (float)jb.mask[p];
         }
         pre[(size_t)tz * K + k] = acc.get();
@@ -187,19 +193,19 @@ __global__ __launch_bounds__(256) void k_sat_rows(const SatJobs jobs, const int 
     tot[tz * NSEG + seg] = acc.get();
     if (seg == NSEG - 1) val[(size_t)tz * K + jb.Nk] = 0.0f;
     __syncthreads();
-    L off = S::zero();
+    Lt off = S::zero();
     for (int q = 0; q < seg; ++q) off = S::plus(off, tot[tz * NSEG + q]);
     for (int k = k0; k < k1; ++k) pre[(size_t)tz * K + k] = S::plus(pre[(size_t)tz * K + k], off);
     if (seg == NSEG - 1) pre[(size_t)tz * K + jb.Nk] = S::plus(off, acc.get());
     __syncthreads();
-    const int SLNp = nzc * ZT;
+    const int SLNp = jb.nzc * jb.ZT;
     E* base = reinterpret_cast<E*>(jb.out);
     for (int idx = threadIdx.x; idx < G * K; idx += 256) {
         const int q = idx / K, k = idx - q * K;
         const int zz = zg * G + q;
         if (zz < SLNp) {
-            const int zc = zz / ZT, zt = zz - zc * ZT;
-            base[(((size_t)zc * jb.nrows + row) * ZT + zt) * K + k] = S::entry(pre[idx], val[idx]);
+            const int zc = zz / jb.ZT, zt = zz - zc * jb.ZT;
+            base[(((size_t)zc * jb.nrows + row) * jb.ZT + zt) * K + k] = S::entry(pre[idx], val[idx]);
         }
     }
 }
@@ -246,22 +252,44 @@ struct Samp<SAT_F64> {
 // an XCD reads the same band of table rows and the band stays in that XCD's L2
 // for all views; pass sums accumulate in `out` (first pass writes, last pass
 // applies the epilogue), in a fixed order.
-template <int MODE, int ZT, int EPI, int V>
-__global__ __launch_bounds__(64) void k_proj(
-    const GeomDev g, const int nzc, const int nug, const int nvg, const int nwork,
-    const float2* __restrict__ cossin, const float* __restrict__ bx, const float* __restrict__ by,
-    const float* __restrict__ xds, const float* __restrict__ yds,
-    const typename Sat<MODE>::E* __restrict__ satX, const typename Sat<MODE>::E* __restrict__ satY,
-    const Subset sub, float* __restrict__ out, const float* __restrict__ meas,
-    const float* __restrict__ rowSum, const int ii0, const int ii1, const int firstPass,
-    const int lastPass) {
+struct ProjDev {
+    GeomDev g;
+    Subset sub;
+    int nzc, nug, nvg, blk0, ii0, ii1, first, last, epi;
+    const float2* cossin;
+    const float* bx;
+    const float* by;
+    const float* xds;
+    const float* yds;
+    const void* satX;
+    const void* satY;
+    float* out;
+    const float* meas;
+    const float* rowSum;
+};
+struct ProjLaunch {
+    int njobs, nwork;
+    ProjDev j[kBatch];
+};
+
+template <int MODE, int ZT, int V>
+__global__ __launch_bounds__(64) void k_proj(const ProjLaunch L) {
     using E = typename Sat<MODE>::E;
     const int lin = xcd_linear(blockIdx.x, gridDim.x);
-    if (lin >= nwork) return;  // whole wave
-    const int ug = lin % nug;
-    const int rest = lin / nug;
-    const int jg = rest % nvg;
-    const int zc = rest / nvg;
+    if (lin >= L.nwork) return;  // whole wave
+    int ji = 0;
+    for (int t = 1; t < L.njobs; ++t)
+        if (lin >= L.j[t].blk0) ji = t;
+    const ProjDev& J = L.j[ji];
+    const GeomDev& g = J.g;
+    const Subset& sub = J.sub;
+    const int local = lin - J.blk0;
+    const int ug = local % J.nug;
+    const int rest = local / J.nug;
+    const int jg = rest % J.nvg;
+    const int zc = rest / J.nvg;
+    const E* satX = reinterpret_cast<const E*>(J.satX);
+    const E* satY = reinterpret_cast<const E*>(J.satY);
 
     const int lane = threadIdx.x;
     const int bnd = min(ug * 63 + lane, g.DNU);
@@ -269,7 +297,7 @@ __global__ __launch_bounds__(64) void k_proj(
     const bool act = lane >= 1 && u < g.DNU;
     const int uu = min(max(u, 0), g.DNU - 1);
     const int z0 = zc * ZT;
-    const float bxb = bx[bnd], byb = by[bnd], xdu = xds[uu], ydu = yds[uu];
+    const float bxb = J.bx[bnd], byb = J.by[bnd], xdu = J.xds[uu], ydu = J.yds[uu];
 
     float pa[V], pf[V], pbb[V], cval[V], ocSv[V], ocKv[V];
     int NsV[V], NkV[V];
@@ -278,7 +306,7 @@ __global__ __launch_bounds__(64) void k_proj(
     for (int v = 0; v < V; ++v) {
         const int j = min(jg * V + v, sub.count - 1);
         const int vg = sub.first + j * sub.stride;
-        const float2 cs = cossin[(size_t)vg * g.SLN + min(z0, g.SLN - 1)];
+        const float2 cs = J.cossin[(size_t)vg * g.SLN + min(z0, g.SLN - 1)];
         const ProjBoundary pb = proj_boundary(g.x0, g.y0, cs.x, cs.y, bxb, byb);
         // continuous: dx / |normalize(det - src)| component (MultiSlicesProjGPU.cu:139,158,181)
         const float srcx = g.x0 * cs.x - g.y0 * cs.y, srcy = g.x0 * cs.y + g.y0 * cs.x;
@@ -303,8 +331,8 @@ __global__ __launch_bounds__(64) void k_proj(
         for (int q = 0; q < ZT; ++q) acc[v][q] = 0.0f;
     int iiEnd = 0;
 #pragma unroll
-    for (int v = 0; v < V; ++v) iiEnd = max(iiEnd, min(ii1, NsV[v]));
-    for (int ii = ii0; ii < iiEnd; ++ii) {
+    for (int v = 0; v < V; ++v) iiEnd = max(iiEnd, min(J.ii1, NsV[v]));
+    for (int ii = J.ii0; ii < iiEnd; ++ii) {
 #pragma unroll
         for (int v = 0; v < V; ++v) {
             if (ii < NsV[v]) {  // uniform across the wave
@@ -322,6 +350,7 @@ __global__ __launch_bounds__(64) void k_proj(
         }
     }
     if (!act) return;
+    const bool ratio = J.epi == PROJ_RATIO && J.last;
 #pragma unroll
     for (int v = 0; v < V; ++v) {
         const int j = jg * V + v;
@@ -333,13 +362,13 @@ __global__ __launch_bounds__(64) void k_proj(
             const int z = z0 + q;
             if (z < g.SLN) {
                 float p = acc[v][q] * cval[v];
-                if (!firstPass) p = p + out[o0 + q];
-                if (EPI == PROJ_PLAIN || !lastPass) {
-                    out[o0 + q] = p;
+                if (!J.first) p = p + J.out[o0 + q];
+                if (!ratio) {
+                    J.out[o0 + q] = p;
                 } else {
-                    const float w = rowSum[(size_t)j * g.DNU + u];
-                    const float b = meas[((size_t)vg * g.DNU + u) * g.SLN + z];
-                    out[o0 + q] = (w != 0.0f) ? (b - p) / w : p;
+                    const float w = J.rowSum[(size_t)j * g.DNU + u];
+                    const float b = J.meas[((size_t)vg * g.DNU + u) * g.SLN + z];
+                    J.out[o0 + q] = (w != 0.0f) ? (b - p) / w : p;
                 }
             }
         }
@@ -352,28 +381,47 @@ __global__ __launch_bounds__(64) void k_proj(
 // voxel footprint on the detector once for the chunk and accumulates
 // (T(idxR) - T(idxL)) * cosVal for its ZT slices.  Masked voxels skip the
 // work and write 0 (or leave the iterate unchanged in the OS-SART epilogues).
-template <int MODE, int ZT, int EPI>
-__global__ __launch_bounds__(256) void k_back(
-    const GeomDev g, const int nzc, const int ntx, const int nty, const int nwork,
-    const float2* __restrict__ cossin, const uint8_t* __restrict__ mask,
-    const typename Sat<MODE>::E* __restrict__ P, const Subset sub, float* __restrict__ vol,
-    const float* __restrict__ colSum, float* __restrict__ xprev, const float* __restrict__ momTab,
-    const int* __restrict__ iterCounter) {
+struct BackDev {
+    GeomDev g;
+    Subset sub;
+    int nzc, ntx, nty, blk0, epi;
+    const float2* cossin;
+    const uint8_t* mask;
+    const void* P;
+    float* vol;
+    const float* colSum;
+    float* xprev;
+    const float* momTab;
+    const int* counter;
+};
+struct BackLaunch {
+    int njobs, nwork;
+    BackDev j[kBatch];
+};
+
+template <int MODE, int ZT>
+__global__ __launch_bounds__(256) void k_back(const BackLaunch L) {
     using E = typename Sat<MODE>::E;
     const int lin = xcd_linear(blockIdx.x, gridDim.x);
-    if (lin >= nwork) return;
-    const int tile = lin % (ntx * nty);
-    const int zc = lin / (ntx * nty);
-    const int x = (tile % ntx) * 16 + (threadIdx.x & 15);
-    const int y = (tile / ntx) * 16 + (threadIdx.x >> 4);
+    if (lin >= L.nwork) return;
+    int ji = 0;
+    for (int t = 1; t < L.njobs; ++t)
+        if (lin >= L.j[t].blk0) ji = t;
+    const BackDev& J = L.j[ji];
+    const GeomDev& g = J.g;
+    const int local = lin - J.blk0;
+    const int tile = local % (J.ntx * J.nty);
+    const int zc = local / (J.ntx * J.nty);
+    const int x = (tile % J.ntx) * 16 + (threadIdx.x & 15);
+    const int y = (tile / J.ntx) * 16 + (threadIdx.x >> 4);
     const bool inb = x < g.XN && y < g.YN;
-    const int m = inb ? mask[(size_t)y * g.XN + x] : 0;
+    const int m = inb ? J.mask[(size_t)y * g.XN + x] : 0;
     const int z0 = zc * ZT;
     const int zg = min(z0, g.SLN - 1);
     const int DNU = g.DNU;
     const int K = DNU + 1;
     const size_t viewStride = (size_t)ZT * K;
-    const E* Pc = P + (size_t)zc * sub.count * viewStride;
+    const E* Pc = reinterpret_cast<const E*>(J.P) + (size_t)zc * J.sub.count * viewStride;
 
     float acc[ZT];
 #pragma unroll
@@ -381,9 +429,9 @@ __global__ __launch_bounds__(256) void k_back(
     if (m != 0) {
         const float vx = ((float)x - g.ocxB) * g.dx;
         const float vy = ((float)y - g.ocyB) * g.dx;
-        for (int j = 0; j < sub.count; ++j) {
-            const int vg = sub.first + j * sub.stride;
-            const float2 cs = cossin[(size_t)vg * g.SLN + zg];
+        for (int j = 0; j < J.sub.count; ++j) {
+            const int vg = J.sub.first + j * J.sub.stride;
+            const float2 cs = J.cossin[(size_t)vg * g.SLN + zg];
             const BackSample bs =
                 back_sample(vx, vy, cs.x, cs.y, g.x0, g.y0, g.inv_sid, g.invdb, g.detCntIdx, g.dx);
             const Fetch fL = fetch_pos<E>(bs.idxL, DNU), fR = fetch_pos<E>(bs.idxR, DNU);
@@ -401,24 +449,27 @@ __global__ __launch_bounds__(256) void k_back(
     if (!inb) return;
     const size_t pix = (size_t)y * g.XN + x;
     const size_t o0 = pix * g.SLN + z0;
-    float w = 1.0f, mom = 0.0f;
-    if constexpr (EPI != BACK_PLAIN) w = colSum[pix];
-    if constexpr (EPI == BACK_UPDATE_FISTA) mom = momTab[*iterCounter];
+    float* vol = J.vol;
+    if (J.epi == BACK_PLAIN) {
+#pragma unroll
+        for (int q = 0; q < ZT; ++q)
+            if (z0 + q < g.SLN) vol[o0 + q] = acc[q];
+        return;
+    }
+    const float w = J.colSum[pix];
+    const bool fista = J.epi == BACK_UPDATE_FISTA;
+    const float mom = fista ? J.momTab[*J.counter] : 0.0f;
 #pragma unroll
     for (int q = 0; q < ZT; ++q) {
         if (z0 + q < g.SLN) {
-            if constexpr (EPI == BACK_PLAIN) {
-                vol[o0 + q] = acc[q];
+            const float upd = (w != 0.0f) ? acc[q] / w : acc[q];
+            const float xn = vol[o0 + q] + upd;
+            if (!fista) {
+                vol[o0 + q] = xn;
             } else {
-                const float upd = (w != 0.0f) ? acc[q] / w : acc[q];
-                const float xn = vol[o0 + q] + upd;
-                if constexpr (EPI == BACK_UPDATE) {
-                    vol[o0 + q] = xn;
-                } else {
-                    const float xo = xn + mom * (xn - xprev[o0 + q]);
-                    vol[o0 + q] = xo;
-                    xprev[o0 + q] = xo;
-                }
+                const float xo = xn + mom * (xn - J.xprev[o0 + q]);
+                vol[o0 + q] = xo;
+                J.xprev[o0 + q] = xo;
             }
         }
     }
@@ -475,6 +526,14 @@ __global__ __launch_bounds__(256) void k_back_trace(const GeomDev g, const float
 // ------------------------------------------------------------------ small helpers
 __global__ void k_counter_inc(int* c) {
     if (threadIdx.x == 0 && blockIdx.x == 0) c[0] = c[0] + 1;
+}
+
+struct CounterLaunch {
+    int n;
+    int* c[kBatch];
+};
+__global__ void k_counter_inc_batch(const CounterLaunch L) {
+    if (blockIdx.x == 0 && (int)threadIdx.x < L.n) L.c[threadIdx.x][0] = L.c[threadIdx.x][0] + 1;
 }
 
 __global__ __launch_bounds__(256) void k_fill(float* __restrict__ p, float v, size_t n) {
@@ -593,24 +652,30 @@ static int sat_group(int mode, int K) {
     return 0;
 }
 
-static hipError_t launch_sat_jobs(int mode, int ZT, int nzc, int SLN, const SatJobs& jobs,
-                                  int njobs, hipStream_t st) {
-    int maxRows = 0, maxK = 0;
-    for (int i = 0; i < njobs; ++i) {
-        maxRows = jobs.j[i].nrows > maxRows ? jobs.j[i].nrows : maxRows;
-        maxK = jobs.j[i].Nk + 1 > maxK ? jobs.j[i].Nk + 1 : maxK;
-    }
+// One launch over up to 2*kBatch tables (all rows of every table).
+static hipError_t launch_sat_tables(int mode, SatDev* t, int n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    int maxK = 0;
+    for (int i = 0; i < n; ++i) maxK = t[i].Nk + 1 > maxK ? t[i].Nk + 1 : maxK;
     const int G = sat_group(mode, maxK);
     if (G == 0) return hipErrorInvalidValue;
-    const int nzg = (nzc * ZT + G - 1) / G;
-    const size_t lsz = mode == SAT_F32 ? 4 : 8;
-    const size_t lds = (size_t)G * maxK * (lsz + 4) + (size_t)256 * lsz;
-    const long long nb = (long long)maxRows * nzg;
+    SatLaunch L{};
+    long long nb = 0;
+    int m = 0;
+    for (int i = 0; i < n; ++i) {
+        if (t[i].nrows <= 0) continue;
+        t[i].nzg = (t[i].nzc * t[i].ZT + G - 1) / G;
+        t[i].blk0 = (int)nb;
+        nb += (long long)t[i].nrows * t[i].nzg;
+        L.j[m++] = t[i];
+    }
+    L.njobs = m;
     if (nb == 0) return hipSuccess;
     if (nb > 0x7fffffffLL) return hipErrorInvalidValue;
-    dim3 grid((unsigned)nb, (unsigned)njobs, 1);
+    const size_t lsz = mode == SAT_F32 ? 4 : 8;
+    const size_t lds = (size_t)G * maxK * (lsz + 4) + (size_t)256 * lsz;
 #define DDF_SATV(GG) \
-    DDF_MODE_SWITCH(mode, k_sat_rows<MC, GG><<<grid, 256, lds, st>>>(jobs, SLN, ZT, nzc, nzg))
+    DDF_MODE_SWITCH(mode, k_sat_rows<MC, GG><<<(unsigned)nb, 256, lds, st>>>(L))
     switch (G) {
         case 16: DDF_SATV(16); break;
         case 8: DDF_SATV(8); break;
@@ -622,52 +687,100 @@ static hipError_t launch_sat_jobs(int mode, int ZT, int nzc, int SLN, const SatJ
     return hipGetLastError();
 }
 
+hipError_t launch_sat_vol_batch(int mode, const SatVolJob* jobs, int n, hipStream_t st) {
+    for (int i0 = 0; i0 < n; i0 += kBatch) {
+        SatDev t[2 * kBatch];
+        int m = 0;
+        for (int i = i0; i < n && i < i0 + kBatch; ++i) {
+            const SatVolJob& J = jobs[i];
+            // rows y, prefix along x; rows x, prefix along y
+            t[m++] = SatDev{J.vol, J.mask, J.satX, J.g.YN, J.g.XN, J.g.XN, 1, J.g.SLN, J.ZT, J.nzc, 0, 0};
+            t[m++] = SatDev{J.vol, J.mask, J.satY, J.g.XN, J.g.YN, 1, J.g.XN, J.g.SLN, J.ZT, J.nzc, 0, 0};
+        }
+        const hipError_t e = launch_sat_tables(mode, t, m, st);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_sat_sino_batch(int mode, const SatSinoJob* jobs, int n, hipStream_t st) {
+    for (int i0 = 0; i0 < n; i0 += 2 * kBatch) {
+        SatDev t[2 * kBatch];
+        int m = 0;
+        for (int i = i0; i < n && i < i0 + 2 * kBatch; ++i) {
+            const SatSinoJob& J = jobs[i];  // rows = views, prefix along u
+            t[m++] = SatDev{J.r, nullptr, J.P, J.PNs, J.DNU, J.DNU, 1, J.SLN, J.ZT, J.nzc, 0, 0};
+        }
+        const hipError_t e = launch_sat_tables(mode, t, m, st);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 hipError_t launch_sat_vol(int mode, int ZT, const GeomDev& g, int nzc, const float* vol,
                           const uint8_t* mask, void* satX, void* satY, hipStream_t st) {
-    SatJobs jobs{};
-    jobs.j[0] = SatJob{vol, mask, satX, g.YN, g.XN, g.XN, 1};  // rows y, prefix along x
-    jobs.j[1] = SatJob{vol, mask, satY, g.XN, g.YN, 1, g.XN};  // rows x, prefix along y
-    return launch_sat_jobs(mode, ZT, nzc, g.SLN, jobs, 2, st);
+    const SatVolJob j{g, nzc, ZT, vol, mask, satX, satY};
+    return launch_sat_vol_batch(mode, &j, 1, st);
 }
 
 hipError_t launch_sat_sino(int mode, int ZT, int DNU, int SLN, int nzc, int PNs, const float* r,
                            void* P, hipStream_t st) {
     if (PNs == 0) return hipSuccess;
-    SatJobs jobs{};
-    jobs.j[0] = SatJob{r, nullptr, P, PNs, DNU, DNU, 1};  // rows = views, prefix along u
-    jobs.j[1] = jobs.j[0];
-    return launch_sat_jobs(mode, ZT, nzc, SLN, jobs, 1, st);
+    const SatSinoJob j{DNU, SLN, nzc, ZT, PNs, r, P};
+    return launch_sat_sino_batch(mode, &j, 1, st);
 }
 
-hipError_t launch_proj(int mode, int ZT, int epi, const GeomDev& g, int nzc, const float2* cossin,
-                       const float* bx, const float* by, const float* xds, const float* yds,
-                       const void* satX, const void* satY, Subset sub, float* out,
-                       const float* meas, const float* rowSum, int slabTile, int viewsPerWave,
-                       hipStream_t st) {
-    if (sub.count <= 0) return hipSuccess;
-    const int V = viewsPerWave == 2 || viewsPerWave == 4 ? viewsPerWave : 1;
-    const int nug = (g.DNU + 62) / 63;  // 63 detectors per wave (64 boundaries)
-    const int nvg = (sub.count + V - 1) / V;
-    const long long nw = (long long)nug * nvg * nzc;
-    if (nw > 0x7fffff00LL) return hipErrorInvalidValue;
-    const int nwork = (int)nw;
-    dim3 grid((unsigned)xcd_grid(nwork));
-    const int Nmax = g.XN > g.YN ? g.XN : g.YN;
-    const int T = (slabTile <= 0 || slabTile > Nmax) ? Nmax : slabTile;
-    for (int ii0 = 0; ii0 < Nmax; ii0 += T) {
-        const int ii1 = ii0 + T, first = ii0 == 0, last = ii1 >= Nmax;
-#define DDF_PROJ(VV)                                                                               \
-    DDF_MODE_SWITCH(mode, DDF_ZT_SWITCH(ZT, {                                                      \
-        using E = typename Sat<MC>::E;                                                             \
-        if (epi == PROJ_PLAIN)                                                                     \
-            k_proj<MC, ZTC, PROJ_PLAIN, VV><<<grid, 64, 0, st>>>(                                  \
-                g, nzc, nug, nvg, nwork, cossin, bx, by, xds, yds, (const E*)satX, (const E*)satY, \
-                sub, out, meas, rowSum, ii0, ii1, first, last);                                    \
-        else                                                                                       \
-            k_proj<MC, ZTC, PROJ_RATIO, VV><<<grid, 64, 0, st>>>(                                  \
-                g, nzc, nug, nvg, nwork, cossin, bx, by, xds, yds, (const E*)satX, (const E*)satY, \
-                sub, out, meas, rowSum, ii0, ii1, first, last);                                    \
-    }))
+// Projector launches: jobs of one slice chunk ZT go together, kBatch at a time;
+// pass p of every job that still has slabs left shares one launch.
+static hipError_t launch_proj_group(int mode, int ZT, int V, const ProjJob* const* jobs, int n,
+                                    hipStream_t st) {
+    int maxPass = 0;
+    int T[kBatch], Nmax[kBatch];
+    for (int i = 0; i < n; ++i) {
+        Nmax[i] = jobs[i]->g.XN > jobs[i]->g.YN ? jobs[i]->g.XN : jobs[i]->g.YN;
+        T[i] = (jobs[i]->slabTile <= 0 || jobs[i]->slabTile > Nmax[i]) ? Nmax[i] : jobs[i]->slabTile;
+        const int np = (Nmax[i] + T[i] - 1) / T[i];
+        maxPass = np > maxPass ? np : maxPass;
+    }
+    for (int p = 0; p < maxPass; ++p) {
+        ProjLaunch L{};
+        long long nw = 0;
+        int m = 0;
+        for (int i = 0; i < n; ++i) {
+            const ProjJob& J = *jobs[i];
+            const int ii0 = p * T[i];
+            if (ii0 >= Nmax[i] || J.sub.count <= 0) continue;
+            ProjDev& d = L.j[m++];
+            d.g = J.g;
+            d.sub = J.sub;
+            d.nzc = J.nzc;
+            d.nug = (J.g.DNU + 62) / 63;  // 63 detectors per wave (64 boundaries)
+            d.nvg = (J.sub.count + V - 1) / V;
+            d.blk0 = (int)nw;
+            d.ii0 = ii0;
+            d.ii1 = ii0 + T[i];
+            d.first = p == 0;
+            d.last = ii0 + T[i] >= Nmax[i];
+            d.epi = J.epi;
+            d.cossin = J.cossin;
+            d.bx = J.bx;
+            d.by = J.by;
+            d.xds = J.xds;
+            d.yds = J.yds;
+            d.satX = J.satX;
+            d.satY = J.satY;
+            d.out = J.out;
+            d.meas = J.meas;
+            d.rowSum = J.rowSum;
+            nw += (long long)d.nug * d.nvg * d.nzc;
+        }
+        if (m == 0) continue;
+        if (nw > 0x7fffff00LL) return hipErrorInvalidValue;
+        L.njobs = m;
+        L.nwork = (int)nw;
+        dim3 grid((unsigned)xcd_grid(L.nwork));
+#define DDF_PROJ(VV) \
+    DDF_MODE_SWITCH(mode, DDF_ZT_SWITCH(ZT, (k_proj<MC, ZTC, VV><<<grid, 64, 0, st>>>(L))))
         if (V == 4) {
             DDF_PROJ(4);
         } else if (V == 2) {
@@ -682,30 +795,112 @@ hipError_t launch_proj(int mode, int ZT, int epi, const GeomDev& g, int nzc, con
     return hipSuccess;
 }
 
+hipError_t launch_proj_batch(int mode, int viewsPerWave, const ProjJob* jobs, int n,
+                             hipStream_t st) {
+    const int V = viewsPerWave == 2 || viewsPerWave == 4 ? viewsPerWave : 1;
+    for (int ZT : {1, 2, 4, 8, 16}) {
+        const ProjJob* grp[kBatch];
+        int m = 0;
+        for (int i = 0; i < n; ++i) {
+            if (jobs[i].ZT != ZT) continue;
+            grp[m++] = &jobs[i];
+            if (m == kBatch) {
+                const hipError_t e = launch_proj_group(mode, ZT, V, grp, m, st);
+                if (e != hipSuccess) return e;
+                m = 0;
+            }
+        }
+        if (m) {
+            const hipError_t e = launch_proj_group(mode, ZT, V, grp, m, st);
+            if (e != hipSuccess) return e;
+        }
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_proj(int mode, int ZT, int epi, const GeomDev& g, int nzc, const float2* cossin,
+                       const float* bx, const float* by, const float* xds, const float* yds,
+                       const void* satX, const void* satY, Subset sub, float* out,
+                       const float* meas, const float* rowSum, int slabTile, int viewsPerWave,
+                       hipStream_t st) {
+    if (sub.count <= 0) return hipSuccess;
+    const ProjJob j{g, sub, nzc, ZT, slabTile, epi, cossin, bx, by, xds, yds, satX, satY, out, meas,
+                    rowSum};
+    return launch_proj_batch(mode, viewsPerWave, &j, 1, st);
+}
+
+static hipError_t launch_back_group(int mode, int ZT, const BackJob* const* jobs, int n,
+                                    hipStream_t st) {
+    BackLaunch L{};
+    long long nw = 0;
+    int m = 0;
+    for (int i = 0; i < n; ++i) {
+        const BackJob& J = *jobs[i];
+        BackDev& d = L.j[m++];
+        d.g = J.g;
+        d.sub = J.sub;
+        d.nzc = J.nzc;
+        d.ntx = (J.g.XN + 15) / 16;
+        d.nty = (J.g.YN + 15) / 16;
+        d.blk0 = (int)nw;
+        d.epi = J.epi;
+        d.cossin = J.cossin;
+        d.mask = J.mask;
+        d.P = J.P;
+        d.vol = J.vol;
+        d.colSum = J.colSum;
+        d.xprev = J.xprev;
+        d.momTab = J.momTab;
+        d.counter = J.counter;
+        nw += (long long)d.ntx * d.nty * d.nzc;
+    }
+    if (nw == 0) return hipSuccess;
+    if (nw > 0x7fffff00LL) return hipErrorInvalidValue;
+    L.njobs = m;
+    L.nwork = (int)nw;
+    dim3 grid((unsigned)xcd_grid(L.nwork));
+    DDF_MODE_SWITCH(mode, DDF_ZT_SWITCH(ZT, (k_back<MC, ZTC><<<grid, 256, 0, st>>>(L))));
+    return hipGetLastError();
+}
+
+hipError_t launch_back_batch(int mode, const BackJob* jobs, int n, hipStream_t st) {
+    for (int ZT : {1, 2, 4, 8, 16}) {
+        const BackJob* grp[kBatch];
+        int m = 0;
+        for (int i = 0; i < n; ++i) {
+            if (jobs[i].ZT != ZT) continue;
+            grp[m++] = &jobs[i];
+            if (m == kBatch) {
+                const hipError_t e = launch_back_group(mode, ZT, grp, m, st);
+                if (e != hipSuccess) return e;
+                m = 0;
+            }
+        }
+        if (m) {
+            const hipError_t e = launch_back_group(mode, ZT, grp, m, st);
+            if (e != hipSuccess) return e;
+        }
+    }
+    return hipSuccess;
+}
+
 hipError_t launch_back(int mode, int ZT, int epi, const GeomDev& g, int nzc, const float2* cossin,
                        const uint8_t* mask, const void* P, Subset sub, float* vol,
                        const float* colSum, float* xprev, const float* momTab,
                        const int* iterCounter, hipStream_t st) {
-    const int ntx = (g.XN + 15) / 16, nty = (g.YN + 15) / 16;
-    const int nwork = ntx * nty * nzc;
-    dim3 grid((unsigned)xcd_grid(nwork));
-    DDF_MODE_SWITCH(mode, DDF_ZT_SWITCH(ZT, {
-        using E = typename Sat<MC>::E;
-        const E* Pt = (const E*)P;
-        if (epi == BACK_PLAIN)
-            k_back<MC, ZTC, BACK_PLAIN><<<grid, 256, 0, st>>>(g, nzc, ntx, nty, nwork, cossin, mask,
-                                                              Pt, sub, vol, colSum, xprev, momTab,
-                                                              iterCounter);
-        else if (epi == BACK_UPDATE)
-            k_back<MC, ZTC, BACK_UPDATE><<<grid, 256, 0, st>>>(g, nzc, ntx, nty, nwork, cossin,
-                                                               mask, Pt, sub, vol, colSum, xprev,
-                                                               momTab, iterCounter);
-        else
-            k_back<MC, ZTC, BACK_UPDATE_FISTA><<<grid, 256, 0, st>>>(
-                g, nzc, ntx, nty, nwork, cossin, mask, Pt, sub, vol, colSum, xprev, momTab,
-                iterCounter);
-    }));
-    return hipGetLastError();
+    const BackJob j{g, sub, nzc, ZT, epi, cossin, mask, P, vol, colSum, xprev, momTab, iterCounter};
+    return launch_back_batch(mode, &j, 1, st);
+}
+
+hipError_t launch_counter_inc_batch(int* const* counters, int n, hipStream_t st) {
+    for (int i0 = 0; i0 < n; i0 += kBatch) {
+        CounterLaunch L{};
+        for (int i = i0; i < n && i < i0 + kBatch; ++i) L.c[L.n++] = counters[i];
+        k_counter_inc_batch<<<1, 64, 0, st>>>(L);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_proj_trace(const GeomDev& g, const float2* cossin, const float* bx,

@@ -122,6 +122,22 @@ class Plan:
     def views_per_wave(self, v):
         self.set_param(N.PARAM_VIEWS_PER_WAVE, v)
 
+    @property
+    def proj_chunk(self):
+        return self.get_param(N.PARAM_PROJ_CHUNK)
+
+    @proj_chunk.setter
+    def proj_chunk(self, v):
+        self.set_param(N.PARAM_PROJ_CHUNK, v)
+
+    @property
+    def back_chunk(self):
+        return self.get_param(N.PARAM_BACK_CHUNK)
+
+    @back_chunk.setter
+    def back_chunk(self, v):
+        self.set_param(N.PARAM_BACK_CHUNK, v)
+
     # timing and traces
     def timing(self, enable=True):
         N.call("ddfan_plan_timing", self.handle, 1 if enable else 0)

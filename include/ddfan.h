@@ -138,6 +138,7 @@ int ddfan_plan_create(const ddfan_geom* geom, const float* hangs, int PN,
                       const unsigned char* mask, int sat_mode, int slice_chunk,
                       ddfan_plan** out);
 int ddfan_plan_destroy(ddfan_plan* plan);
+/* slice_chunk: widest chunk of consecutive slices with identical angles. */
 int ddfan_plan_query(const ddfan_plan* plan, int* slice_chunk, int* n_chunks, int* sat_mode);
 /* Tunable launch parameters (the autotuner sets them; defaults are heuristic).
  * DDFAN_PARAM_SLAB_TILE: projector slabs per pass (0 = one pass over all
@@ -146,6 +147,11 @@ int ddfan_plan_query(const ddfan_plan* plan, int* slice_chunk, int* n_chunks, in
 /* DDFAN_PARAM_VIEWS_PER_WAVE: consecutive views one projector wave handles
  * (1, 2 or 4); they share table rows through L1. */
 #define DDFAN_PARAM_VIEWS_PER_WAVE 2
+/* DDFAN_PARAM_PROJ_CHUNK / DDFAN_PARAM_BACK_CHUNK: slices per chunk (a power
+ * of two up to the widest chunk whose angles agree) of the projector and of
+ * the backprojector. */
+#define DDFAN_PARAM_PROJ_CHUNK 3
+#define DDFAN_PARAM_BACK_CHUNK 4
 int ddfan_plan_set_param(ddfan_plan* plan, int param, int value);
 int ddfan_plan_get_param(const ddfan_plan* plan, int param, int* value);
 /* Device bytes a plan of this shape holds for full-view operators plus an
