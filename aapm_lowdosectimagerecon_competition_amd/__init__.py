@@ -9,16 +9,19 @@ the C ABI in include/ddfan.h (libddfan.so).
 """
 from . import _native
 from ._native import DDFanError, DeviceBuffer, device_count
+from .batch import Batch
 from .dd2 import dd2_multi_slices
 from .geometry import Conf, detector_arc, detector_geometry, recon_mask, subset_views
 from .ossart import ossart_aapm
 from .plan import Plan, plan_footprint
 from .rebin import HelicalCfg, helical_to_fan_func, helical_to_fan_routine
+from .serve import Result, Server
 
 __all__ = [
-    "Conf", "DDFanError", "DeviceBuffer", "HelicalCfg", "Plan", "dd2_multi_slices",
+    "Batch", "Conf", "DDFanError", "DeviceBuffer", "HelicalCfg", "Plan", "dd2_multi_slices",
     "detector_arc", "detector_geometry", "device_count", "helical_to_fan_func",
-    "helical_to_fan_routine", "ossart_aapm", "plan_footprint", "recon_mask", "subset_views",
+    "helical_to_fan_routine", "ossart_aapm", "plan_footprint", "recon_mask", "Result", "Server",
+    "subset_views",
 ]
 
 

@@ -93,6 +93,20 @@ _SIGS = {
     "ddfan_plan_timing_read": [_p, _i, ctypes.POINTER(ctypes.c_double), _ip],
     "ddfan_plan_trace_proj": [_p, _i, _i, _i, _p],
     "ddfan_plan_trace_back": [_p, _i, _i, _i, _p],
+    "ddfan_batch_create": [_i, _i, ctypes.POINTER(_p)],
+    "ddfan_batch_add": [_p, _p, _p, _p, _i, _i, _i, _ip],
+    "ddfan_batch_size": [_p, _ip],
+    "ddfan_batch_run": [_p, _i],
+    "ddfan_batch_destroy": [_p],
+    "ddfan_server_create": [_sz, _i, _i, ctypes.POINTER(_p)],
+    "ddfan_server_submit": [_p, ctypes.POINTER(GeomStruct), _p, _i, _p, _p, _p, _i, _i, _i,
+                            ctypes.POINTER(_ll)],
+    "ddfan_server_poll": [_p, _ll, _ip],
+    "ddfan_server_wait": [_p, _ll, _p, ctypes.POINTER(ctypes.c_double),
+                          ctypes.POINTER(ctypes.c_double)],
+    "ddfan_server_stats": [_p, ctypes.POINTER(_ll), ctypes.POINTER(_ll), _ip, ctypes.POINTER(_ll),
+                           ctypes.POINTER(_sz), ctypes.POINTER(ctypes.c_double)],
+    "ddfan_server_destroy": [_p],
     "ddfan_helical_to_fan_dev": [_p, _p, _p, _i, _f, _i, _ll, _p, _i, _f, _f, _f, _i, _i, _f, _f,
                                  _i, _f, _f, _p, _p],
 }

@@ -1,0 +1,57 @@
+"""Batched OS-SART: many reconstructions of uneven size share every kernel
+launch (``ddfan_batch_*``), each batch iteration replayed as a hipGraph.
+
+Per reconstruction the arithmetic is that of :meth:`Plan.ossart`, so a
+reconstruction's result does not depend on what else is in the batch."""
+from __future__ import annotations
+
+import ctypes
+
+from . import _native as N
+from .plan import Plan, _sat_mode
+
+
+class Batch:
+    """Wraps ``ddfan_batch``.  Every plan must use ``sat_mode``; a plan holds
+    one reconstruction's state and is added once."""
+
+    def __init__(self, sat_mode="twofloat", views_per_wave: int = 0):
+        h = ctypes.c_void_p()
+        N.call("ddfan_batch_create", _sat_mode(sat_mode), int(views_per_wave), ctypes.byref(h))
+        self.handle = h.value
+        self._keep = []  # plans and buffers referenced by the batch
+
+    def add(self, plan: Plan, d_meas, d_x, num_iter, num_os, use_fista=1) -> int:
+        idx = ctypes.c_int()
+        N.call("ddfan_batch_add", self.handle, plan.handle, N.ptr(d_meas), N.ptr(d_x),
+               int(num_iter), int(num_os), int(use_fista), ctypes.byref(idx))
+        self._keep.append((plan, d_meas, d_x))
+        return idx.value
+
+    def __len__(self):
+        n = ctypes.c_int()
+        N.call("ddfan_batch_size", self.handle, ctypes.byref(n))
+        return n.value
+
+    def run(self, graph=True):
+        N.call("ddfan_batch_run", self.handle, N.OSSART_GRAPH if graph else 0)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            N.call("ddfan_batch_destroy", self.handle)
+            self.handle = None
+        self._keep = []
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None) and N._lib is not None:
+                N._lib.ddfan_batch_destroy(self.handle)
+                self.handle = None
+        except Exception:  # pragma: no cover
+            pass

@@ -10,16 +10,16 @@
 #include <string>
 #include <vector>
 
-#include "ddfan.h"
-#include "ddfan_host.h"
-#include "ddfan_internal.h"
+#include "ddfan_plan.h"
 
 using namespace ddf;
 
 namespace {
-
 thread_local std::string t_err;
 std::atomic<int> g_default_sat{DDFAN_SAT_F32};
+}  // namespace
+
+namespace ddf {
 
 int fail(int code, const char* fmt, ...) {
     char buf[512];
@@ -31,60 +31,33 @@ int fail(int code, const char* fmt, ...) {
     return code;
 }
 
-#define HIPCALL(x)                                                                           \
-    do {                                                                                     \
-        hipError_t e_ = (x);                                                                 \
-        if (e_ != hipSuccess)                                                                \
-            return fail(e_ == hipErrorOutOfMemory ? DDFAN_ENOMEM : DDFAN_EDEVICE,            \
-                        "%s: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, __LINE__);    \
-    } while (0)
-
-#define CHECK(cond, code, ...)                    \
-    do {                                          \
-        if (!(cond)) return fail(code, __VA_ARGS__); \
-    } while (0)
-
-#define TRY(x)                       \
-    do {                             \
-        int rc_ = (x);               \
-        if (rc_ != DDFAN_OK) return rc_; \
-    } while (0)
-
-// Owning device allocation.
-struct DBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    DBuf() = default;
-    DBuf(const DBuf&) = delete;
-    DBuf& operator=(const DBuf&) = delete;
-    ~DBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    hipError_t reserve(size_t b) {
-        if (p && b <= bytes) return hipSuccess;
-        release();
-        hipError_t e = hipMalloc(&p, b ? b : 16);
-        if (e == hipSuccess)
-            bytes = b;
-        else
-            p = nullptr;
-        return e;
-    }
-    template <class T>
-    T* as() const {
-        return reinterpret_cast<T*>(p);
-    }
-};
-
 int check_device(int* dev) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return fail(DDFAN_ENODEV, "no GPU visible");
     HIPCALL(hipGetDevice(dev));
     return DDFAN_OK;
 }
+
+int upload(DBuf& b, const void* src, size_t bytes, hipStream_t st) {
+    HIPCALL(b.reserve(bytes));
+    HIPCALL(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, st));
+    return DDFAN_OK;
+}
+
+int ensure_vol_sat(ddfan_plan* P) {
+    const size_t bx = sat_vol_bytes(P->mode, P->nzc, P->ZT, P->g.XN, P->g.YN);
+    const size_t by = sat_vol_bytes(P->mode, P->nzc, P->ZT, P->g.YN, P->g.XN);
+    if (bx > P->satX.bytes || by > P->satY.bytes) {
+        HIPCALL(hipStreamSynchronize(P->stream));
+        HIPCALL(P->satX.reserve(bx));
+        HIPCALL(P->satY.reserve(by));
+    }
+    return DDFAN_OK;
+}
+
+}  // namespace ddf
+
+namespace {
 
 int check_subset(int PN, int first, int stride, int count) {
     CHECK(count >= 0 && stride >= 1 && first >= 0, DDFAN_EINVAL,
@@ -107,69 +80,9 @@ int check_start_idx(const int* startIdx, int nStart, int SLN) {
     return DDFAN_OK;
 }
 
-size_t sat_vol_bytes(int mode, int nzc, int ZT, int XN, int YN) {
-    return sat_elem_bytes(mode) * (size_t)nzc * ZT * (size_t)(XN + 1) * YN;
-}
-size_t sat_sino_bytes(int mode, int nzc, int ZT, int DNU, int views) {
-    return sat_elem_bytes(mode) * (size_t)nzc * ZT * (size_t)(DNU + 1) * views;
-}
-
 }  // namespace
 
 // =================================================================== plan
-struct ddfan_plan {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    GeomDev g{};
-    int PN = 0, mode = 0;
-    int ZTmax = 1;       // widest chunk of consecutive slices with identical angles
-    int ZT = 1, nzc = 1;   // projector slice chunk and chunk count
-    int ZTb = 1, nzcb = 1; // backprojector slice chunk and chunk count
-    int slabTile = 0;      // projector slabs per pass (0: all slabs in one pass)
-    int viewsPerWave = 4;  // projector views per wave (1, 2 or 4)
-    std::vector<float> h_hangs, h_cs;
-    DBuf cs, bx, by, xds, yds, mask;
-    DBuf satX, satY, satP;
-    // OS-SART state
-    DBuf rbuf, rowSum, colSum, xprev, momTab, counter, cs0, onesV, onesP, satX0, satY0, satP0;
-    bool timing = false;
-    struct Ev {
-        int kind;
-        hipEvent_t a, b;
-    };
-    std::vector<Ev> events;
-
-    ~ddfan_plan() {
-        if (stream) (void)hipStreamSynchronize(stream);
-        clear_events();
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-    void clear_events() {
-        for (auto& e : events) {
-            (void)hipEventDestroy(e.a);
-            (void)hipEventDestroy(e.b);
-        }
-        events.clear();
-    }
-    template <class F>
-    hipError_t timed(int kind, F&& f) {
-        if (!timing) return f();
-        hipEvent_t a, b;
-        hipError_t e = hipEventCreate(&a);
-        if (e != hipSuccess) return e;
-        e = hipEventCreate(&b);
-        if (e != hipSuccess) {
-            (void)hipEventDestroy(a);
-            return e;
-        }
-        (void)hipEventRecord(a, stream);
-        e = f();
-        (void)hipEventRecord(b, stream);
-        events.push_back({kind, a, b});
-        return e;
-    }
-};
-
 namespace {
 
 void fill_geom(GeomDev& g, const ddfan_geom* G, int SLN) {
@@ -202,11 +115,6 @@ int validate_geom(const ddfan_geom* G) {
     return DDFAN_OK;
 }
 
-int upload(DBuf& b, const void* src, size_t bytes, hipStream_t st) {
-    HIPCALL(b.reserve(bytes));
-    HIPCALL(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, st));
-    return DDFAN_OK;
-}
 
 }  // namespace
 
@@ -459,18 +367,6 @@ int ddfan_plan_sync(ddfan_plan* P) {
     return DDFAN_OK;
 }
 
-// Volume summed-area tables sized for the projector's current slice chunking.
-static int ensure_vol_sat(ddfan_plan* P) {
-    const size_t bx = sat_vol_bytes(P->mode, P->nzc, P->ZT, P->g.XN, P->g.YN);
-    const size_t by = sat_vol_bytes(P->mode, P->nzc, P->ZT, P->g.YN, P->g.XN);
-    if (bx > P->satX.bytes || by > P->satY.bytes) {
-        HIPCALL(hipStreamSynchronize(P->stream));
-        HIPCALL(P->satX.reserve(bx));
-        HIPCALL(P->satY.reserve(by));
-    }
-    return DDFAN_OK;
-}
-
 int ddfan_plan_proj(ddfan_plan* P, const float* d_vol, float* d_prj, int first, int stride,
                     int count) {
     CHECK(P && d_vol && d_prj, DDFAN_EINVAL, "NULL argument");
@@ -559,151 +455,6 @@ int ddfan_plan_trace_back(ddfan_plan* P, int first, int stride, int count, int32
     HIPCALL(launch_back_trace(P->g, P->cs.as<float2>(), Subset{first, stride, count}, d_trace,
                               P->stream));
     HIPCALL(hipStreamSynchronize(P->stream));
-    return DDFAN_OK;
-}
-
-// ------------------------------------------------------------ OS-SART (device-resident)
-int ddfan_ossart(ddfan_plan* P, const float* d_meas, float* d_x, int numIter, int numOS,
-                 int useFISTA, int flags) {
-    CHECK(P && d_meas && d_x, DDFAN_EINVAL, "NULL argument");
-    CHECK(numIter >= 0, DDFAN_EINVAL, "numIter must be >= 0");
-    CHECK(numOS >= 1 && numOS <= P->PN, DDFAN_EINVAL, "numOS must be in [1, %d]", P->PN);
-    HIPCALL(hipSetDevice(P->device));
-    hipStream_t st = P->stream;
-    const GeomDev& g = P->g;
-    const int DNU = g.DNU, XN = g.XN, YN = g.YN, SLN = g.SLN, mode = P->mode;
-    const size_t nvox = (size_t)XN * YN * SLN, npix = (size_t)XN * YN;
-    std::vector<int> cnt(numOS);
-    for (int k = 0; k < numOS; ++k) cnt[k] = subset_views(P->PN, numOS, k, nullptr);
-    const int maxc = cnt[0];
-    HIPCALL(hipStreamSynchronize(st));
-    HIPCALL(P->rowSum.reserve(sizeof(float) * (size_t)numOS * maxc * DNU));
-    HIPCALL(P->colSum.reserve(sizeof(float) * (size_t)numOS * npix));
-    HIPCALL(P->rbuf.reserve(sizeof(float) * (size_t)maxc * DNU * SLN));
-    HIPCALL(P->satP.reserve(std::max(P->satP.bytes, sat_sino_bytes(mode, P->nzcb, P->ZTb, DNU, maxc))));
-    HIPCALL(P->momTab.reserve(sizeof(float) * std::max(numIter, 1)));
-    HIPCALL(P->counter.reserve(sizeof(int)));
-    if (useFISTA) HIPCALL(P->xprev.reserve(sizeof(float) * nvox));
-    TRY(ensure_vol_sat(P));
-
-    // OS-SART weights from slice 0, replicated to every slice (OSSART_AAPM.m:59-66):
-    // rowSum_k = A_k(mask*1), colSum_k = A_k^T(1), both on a one-slice geometry
-    // with the slice-0 angles.
-    std::vector<float> h_cs0((size_t)P->PN * 2);  // outlives the async upload below
-    {
-        GeomDev g0 = g;
-        g0.SLN = 1;
-        for (int v = 0; v < P->PN; ++v) {
-            h_cs0[2 * v] = P->h_cs[2 * ((size_t)v * SLN)];
-            h_cs0[2 * v + 1] = P->h_cs[2 * ((size_t)v * SLN) + 1];
-        }
-        TRY(upload(P->cs0, h_cs0.data(), sizeof(float) * h_cs0.size(), st));
-        HIPCALL(P->onesV.reserve(sizeof(float) * npix));
-        HIPCALL(P->onesP.reserve(sizeof(float) * (size_t)maxc * DNU));
-        HIPCALL(P->satX0.reserve(sat_vol_bytes(mode, 1, 1, XN, YN)));
-        HIPCALL(P->satY0.reserve(sat_vol_bytes(mode, 1, 1, YN, XN)));
-        HIPCALL(P->satP0.reserve(sat_sino_bytes(mode, 1, 1, DNU, maxc)));
-        HIPCALL(launch_fill(P->onesV.as<float>(), 1.0f, npix, st));
-        HIPCALL(launch_fill(P->onesP.as<float>(), 1.0f, (size_t)maxc * DNU, st));
-        HIPCALL(launch_sat_vol(mode, 1, g0, 1, P->onesV.as<float>(), P->mask.as<uint8_t>(),
-                               P->satX0.p, P->satY0.p, st));
-        for (int k = 0; k < numOS; ++k) {
-            const Subset sub{k, numOS, cnt[k]};
-            HIPCALL(launch_proj(mode, 1, PROJ_PLAIN, g0, 1, P->cs0.as<float2>(), P->bx.as<float>(),
-                                P->by.as<float>(), P->xds.as<float>(), P->yds.as<float>(),
-                                P->satX0.p, P->satY0.p, sub,
-                                P->rowSum.as<float>() + (size_t)k * maxc * DNU, nullptr, nullptr,
-                                0, 1, st));
-            HIPCALL(launch_sat_sino(mode, 1, DNU, 1, 1, cnt[k], P->onesP.as<float>(), P->satP0.p, st));
-            HIPCALL(launch_back(mode, 1, BACK_PLAIN, g0, 1, P->cs0.as<float2>(),
-                                P->mask.as<uint8_t>(), P->satP0.p, sub,
-                                P->colSum.as<float>() + (size_t)k * npix, nullptr, nullptr, nullptr,
-                                nullptr, st));
-        }
-    }
-    // FISTA momenta (OSSART_AAPM.m:111-115): MATLAB double scalars, applied to
-    // single arrays as single((t0-1)/t1).
-    std::vector<float> mom(std::max(numIter, 1), 0.0f);
-    {
-        double t0 = 1.0;
-        for (int it = 0; it < numIter; ++it) {
-            const double t1 = (1.0 + sqrt(1.0 + 4.0 * t0 * t0)) / 2.0;
-            mom[it] = (float)((t0 - 1.0) / t1);
-            t0 = t1;
-        }
-    }
-    HIPCALL(hipMemcpyAsync(P->momTab.p, mom.data(), sizeof(float) * mom.size(),
-                           hipMemcpyHostToDevice, st));
-    const int zero = 0;
-    HIPCALL(hipMemcpyAsync(P->counter.p, &zero, sizeof(int), hipMemcpyHostToDevice, st));
-    if (useFISTA)
-        HIPCALL(hipMemcpyAsync(P->xprev.p, d_x, sizeof(float) * nvox, hipMemcpyDeviceToDevice, st));
-    HIPCALL(hipStreamSynchronize(st));  // host staging buffers above go out of scope
-
-    const bool timing = P->timing && !(flags & DDFAN_OSSART_GRAPH);
-    auto enqueue_iteration = [&]() -> hipError_t {
-        for (int k = 0; k < numOS; ++k) {
-            const Subset sub{k, numOS, cnt[k]};
-            hipError_t e;
-            auto T = [&](int kind, auto&& f) { return timing ? P->timed(kind, f) : f(); };
-            e = T(0, [&] {
-                return launch_sat_vol(mode, P->ZT, g, P->nzc, d_x, P->mask.as<uint8_t>(), P->satX.p,
-                                      P->satY.p, st);
-            });
-            if (e != hipSuccess) return e;
-            e = T(1, [&] {
-                return launch_proj(mode, P->ZT, PROJ_RATIO, g, P->nzc, P->cs.as<float2>(),
-                                   P->bx.as<float>(), P->by.as<float>(), P->xds.as<float>(),
-                                   P->yds.as<float>(), P->satX.p, P->satY.p, sub,
-                                   P->rbuf.as<float>(), d_meas,
-                                   P->rowSum.as<float>() + (size_t)k * maxc * DNU, P->slabTile,
-                                   P->viewsPerWave, st);
-            });
-            if (e != hipSuccess) return e;
-            e = T(2, [&] {
-                return launch_sat_sino(mode, P->ZTb, DNU, SLN, P->nzcb, cnt[k], P->rbuf.as<float>(),
-                                       P->satP.p, st);
-            });
-            if (e != hipSuccess) return e;
-            const int epi = (useFISTA && k == numOS - 1) ? BACK_UPDATE_FISTA : BACK_UPDATE;
-            e = T(3, [&] {
-                return launch_back(mode, P->ZTb, epi, g, P->nzcb, P->cs.as<float2>(),
-                                   P->mask.as<uint8_t>(), P->satP.p, sub, d_x,
-                                   P->colSum.as<float>() + (size_t)k * npix, P->xprev.as<float>(),
-                                   P->momTab.as<float>(), P->counter.as<int>(), st);
-            });
-            if (e != hipSuccess) return e;
-        }
-        return launch_counter_inc(P->counter.as<int>(), st);
-    };
-
-    if ((flags & DDFAN_OSSART_GRAPH) && numIter > 0) {
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
-        HIPCALL(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        hipError_t ce = enqueue_iteration();
-        hipError_t ee = hipStreamEndCapture(st, &graph);
-        if (ce != hipSuccess) {
-            if (graph) (void)hipGraphDestroy(graph);
-            HIPCALL(ce);
-        }
-        HIPCALL(ee);
-        hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        if (ie != hipSuccess) {
-            (void)hipGraphDestroy(graph);
-            HIPCALL(ie);
-        }
-        hipError_t le = hipSuccess;
-        for (int it = 0; it < numIter && le == hipSuccess; ++it) le = hipGraphLaunch(exec, st);
-        hipError_t se = hipStreamSynchronize(st);
-        (void)hipGraphExecDestroy(exec);
-        (void)hipGraphDestroy(graph);
-        HIPCALL(le);
-        HIPCALL(se);
-    } else {
-        for (int it = 0; it < numIter; ++it) HIPCALL(enqueue_iteration());
-        HIPCALL(hipStreamSynchronize(st));
-    }
     return DDFAN_OK;
 }
 

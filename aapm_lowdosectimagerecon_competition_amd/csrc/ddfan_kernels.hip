@@ -272,8 +272,24 @@ struct ProjLaunch {
     ProjDev j[kBatch];
 };
 
+// Waves per SIMD the projector and backprojector are compiled for (the
+// compiler's register budget).  Measured on MI355X at 512x512x64, 2304 views
+// (two-float tables): the projector keeps a view's ZT table loads in flight
+// together in the registers of 5 waves (29.9 ms; 6 waves serialise them,
+// 39 ms); the backprojector's gathers contend in the texture address unit,
+// where 8 waves with few loads in flight each beat 7 waves with many (34.1 vs
+// 38.9 ms).
+#ifndef DDF_PROJ_WAVES
+#define DDF_PROJ_WAVES 5
+#endif
+#ifndef DDF_BACK_WAVES
+#define DDF_BACK_WAVES 8
+#endif
+
+
 template <int MODE, int ZT, int V>
-__global__ __launch_bounds__(64) void k_proj(const ProjLaunch L) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DDF_PROJ_WAVES, DDF_PROJ_WAVES)))
+void k_proj(const ProjLaunch L) {
     using E = typename Sat<MODE>::E;
     const int lin = xcd_linear(blockIdx.x, gridDim.x);
     if (lin >= L.nwork) return;  // whole wave
@@ -281,15 +297,16 @@ __global__ __launch_bounds__(64) void k_proj(const ProjLaunch L) {
     for (int t = 1; t < L.njobs; ++t)
         if (lin >= L.j[t].blk0) ji = t;
     const ProjDev& J = L.j[ji];
-    const GeomDev& g = J.g;
-    const Subset& sub = J.sub;
+    const GeomDev g = J.g;
+    const Subset sub = J.sub;
     const int local = lin - J.blk0;
     const int ug = local % J.nug;
     const int rest = local / J.nug;
     const int jg = rest % J.nvg;
     const int zc = rest / J.nvg;
-    const E* satX = reinterpret_cast<const E*>(J.satX);
-    const E* satY = reinterpret_cast<const E*>(J.satY);
+    // chunk zc of both tables: satX rows are y (prefix along x), satY rows are x
+    const E* baseX = reinterpret_cast<const E*>(J.satX) + (size_t)zc * g.YN * (size_t)ZT * (g.XN + 1);
+    const E* baseY = reinterpret_cast<const E*>(J.satY) + (size_t)zc * g.XN * (size_t)ZT * (g.YN + 1);
 
     const int lane = threadIdx.x;
     const int bnd = min(ug * 63 + lane, g.DNU);
@@ -299,9 +316,11 @@ __global__ __launch_bounds__(64) void k_proj(const ProjLaunch L) {
     const int z0 = zc * ZT;
     const float bxb = J.bx[bnd], byb = J.by[bnd], xdu = J.xds[uu], ydu = J.yds[uu];
 
-    float pa[V], pf[V], pbb[V], cval[V], ocSv[V], ocKv[V];
-    int NsV[V], NkV[V];
-    const E* satv[V];
+    // Per view: the boundary ray of this lane and the continuous factor; the
+    // slab orientation of view v is bit v of xs (a wave-uniform choice, so the
+    // per-view table selection below stays scalar).
+    float pa[V], pf[V], pbb[V], cval[V];
+    unsigned xs = 0;
 #pragma unroll
     for (int v = 0; v < V; ++v) {
         const int j = min(jg * V + v, sub.count - 1);
@@ -317,11 +336,7 @@ __global__ __launch_bounds__(64) void k_proj(const ProjLaunch L) {
         pa[v] = pb.a;
         pf[v] = pb.f;
         pbb[v] = pb.b;
-        NsV[v] = pb.xslab ? g.XN : g.YN;
-        NkV[v] = pb.xslab ? g.YN : g.XN;
-        ocSv[v] = pb.xslab ? g.ocxP : g.ocyP;
-        ocKv[v] = pb.xslab ? g.ocyP : g.ocxP;
-        satv[v] = (pb.xslab ? satY : satX) + (size_t)zc * NsV[v] * (size_t)ZT * (NkV[v] + 1);
+        if (proj_xslab(cs.x, cs.y)) xs |= 1u << v;
     }
 
     float acc[V][ZT];
@@ -331,19 +346,25 @@ __global__ __launch_bounds__(64) void k_proj(const ProjLaunch L) {
         for (int q = 0; q < ZT; ++q) acc[v][q] = 0.0f;
     int iiEnd = 0;
 #pragma unroll
-    for (int v = 0; v < V; ++v) iiEnd = max(iiEnd, min(J.ii1, NsV[v]));
+    for (int v = 0; v < V; ++v) iiEnd = max(iiEnd, min(J.ii1, ((xs >> v) & 1u) ? g.XN : g.YN));
     for (int ii = J.ii0; ii < iiEnd; ++ii) {
 #pragma unroll
         for (int v = 0; v < V; ++v) {
-            if (ii < NsV[v]) {  // uniform across the wave
-                const int K = NkV[v] + 1;
-                const float r = slab_r(slab_obj(ii, ocSv[v], g.dx), pa[v], pf[v], pbb[v]);
-                const Fetch f = fetch_pos<E>(slab_coord(r, g.invdx, ocKv[v]), NkV[v]);
+            const bool xv = (xs >> v) & 1u;
+            if (ii < (xv ? g.XN : g.YN)) {  // uniform across the wave
+                const int Nk = xv ? g.YN : g.XN;
+                const float ocS = xv ? g.ocxP : g.ocyP, ocK = xv ? g.ocyP : g.ocxP;
+                const int K = Nk + 1;
+                const float r = slab_r(slab_obj(ii, ocS, g.dx), pa[v], pf[v], pbb[v]);
+                const Fetch f = fetch_pos<E>(slab_coord(r, g.invdx, ocK), Nk);
                 const float invlen = __builtin_amdgcn_rcpf(r - lane_prev(r));
-                const E* row = satv[v] + (size_t)ii * ZT * K;
+                E e[ZT];
+                const E* row = (xv ? baseY : baseX) + (size_t)ii * ZT * K;
+#pragma unroll
+                for (int q = 0; q < ZT; ++q) e[q] = entry_at(row + (size_t)q * K, f.off);
 #pragma unroll
                 for (int q = 0; q < ZT; ++q) {
-                    const Samp<MODE> T(entry_at(row + (size_t)q * K, f.off), f.t);
+                    const Samp<MODE> T(e[q], f.t);
                     acc[v][q] = fmaf(T.minus_prev_lane(), invlen, acc[v][q]);
                 }
             }
@@ -351,24 +372,25 @@ __global__ __launch_bounds__(64) void k_proj(const ProjLaunch L) {
     }
     if (!act) return;
     const bool ratio = J.epi == PROJ_RATIO && J.last;
+    float* __restrict__ out = J.out;
 #pragma unroll
     for (int v = 0; v < V; ++v) {
         const int j = jg * V + v;
         if (j >= sub.count) break;
         const int vg = sub.first + j * sub.stride;
         const size_t o0 = ((size_t)j * g.DNU + u) * g.SLN + z0;
+        const float w = ratio ? J.rowSum[(size_t)j * g.DNU + u] : 0.0f;
 #pragma unroll
         for (int q = 0; q < ZT; ++q) {
             const int z = z0 + q;
             if (z < g.SLN) {
                 float p = acc[v][q] * cval[v];
-                if (!J.first) p = p + J.out[o0 + q];
+                if (!J.first) p = p + out[o0 + q];
                 if (!ratio) {
-                    J.out[o0 + q] = p;
+                    out[o0 + q] = p;
                 } else {
-                    const float w = J.rowSum[(size_t)j * g.DNU + u];
                     const float b = J.meas[((size_t)vg * g.DNU + u) * g.SLN + z];
-                    J.out[o0 + q] = (w != 0.0f) ? (b - p) / w : p;
+                    out[o0 + q] = (w != 0.0f) ? (b - p) / w : p;
                 }
             }
         }
@@ -400,7 +422,8 @@ struct BackLaunch {
 };
 
 template <int MODE, int ZT>
-__global__ __launch_bounds__(256) void k_back(const BackLaunch L) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DDF_BACK_WAVES)))
+void k_back(const BackLaunch L) {
     using E = typename Sat<MODE>::E;
     const int lin = xcd_linear(blockIdx.x, gridDim.x);
     if (lin >= L.nwork) return;

@@ -1,0 +1,326 @@
+// ddfan_ossart.hip -- device-resident OS-SART(+FISTA) of OSSART_AAPM.m:49-117:
+// the per-reconstruction set-up, the batched iteration (several reconstructions
+// of different sizes share every launch) and its hipGraph capture, behind
+// ddfan_ossart (one plan) and ddfan_batch_* (many plans).
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include <algorithm>
+#include <new>
+#include <vector>
+
+#include "ddfan_plan.h"
+
+using namespace ddf;
+
+namespace ddf {
+
+int ossart_prepare(OsJob& J, hipStream_t st) {
+    ddfan_plan* P = J.P;
+    CHECK(P && J.meas && J.x, DDFAN_EINVAL, "NULL argument");
+    CHECK(J.numIter >= 0, DDFAN_EINVAL, "numIter must be >= 0");
+    CHECK(J.numOS >= 1 && J.numOS <= P->PN, DDFAN_EINVAL, "numOS must be in [1, %d]", P->PN);
+    const GeomDev& g = P->g;
+    const int DNU = g.DNU, XN = g.XN, YN = g.YN, SLN = g.SLN, mode = P->mode;
+    const int numOS = J.numOS;
+    const size_t nvox = (size_t)XN * YN * SLN, npix = (size_t)XN * YN;
+    J.cnt.resize(numOS);
+    for (int k = 0; k < numOS; ++k) J.cnt[k] = subset_views(P->PN, numOS, k, nullptr);
+    J.maxc = J.cnt[0];
+    const int maxc = J.maxc;
+    // The plan's own stream may still use the buffers reallocated below.
+    HIPCALL(hipStreamSynchronize(P->stream));
+    HIPCALL(P->rowSum.reserve(sizeof(float) * (size_t)numOS * maxc * DNU));
+    HIPCALL(P->colSum.reserve(sizeof(float) * (size_t)numOS * npix));
+    HIPCALL(P->rbuf.reserve(sizeof(float) * (size_t)maxc * DNU * SLN));
+    HIPCALL(P->satP.reserve(std::max(P->satP.bytes, sat_sino_bytes(mode, P->nzcb, P->ZTb, DNU, maxc))));
+    HIPCALL(P->momTab.reserve(sizeof(float) * std::max(J.numIter, 1)));
+    HIPCALL(P->counter.reserve(sizeof(int)));
+    if (J.fista) HIPCALL(P->xprev.reserve(sizeof(float) * nvox));
+    TRY(ensure_vol_sat(P));
+
+    // OS-SART weights from slice 0, replicated to every slice (OSSART_AAPM.m:59-66):
+    // rowSum_k = A_k(mask*1), colSum_k = A_k^T(1), both on a one-slice geometry
+    // with the slice-0 angles.
+    GeomDev g0 = g;
+    g0.SLN = 1;
+    J.h_cs0.resize((size_t)P->PN * 2);
+    for (int v = 0; v < P->PN; ++v) {
+        J.h_cs0[2 * v] = P->h_cs[2 * ((size_t)v * SLN)];
+        J.h_cs0[2 * v + 1] = P->h_cs[2 * ((size_t)v * SLN) + 1];
+    }
+    TRY(upload(P->cs0, J.h_cs0.data(), sizeof(float) * J.h_cs0.size(), st));
+    HIPCALL(P->onesV.reserve(sizeof(float) * npix));
+    HIPCALL(P->onesP.reserve(sizeof(float) * (size_t)maxc * DNU));
+    HIPCALL(P->satX0.reserve(sat_vol_bytes(mode, 1, 1, XN, YN)));
+    HIPCALL(P->satY0.reserve(sat_vol_bytes(mode, 1, 1, YN, XN)));
+    HIPCALL(P->satP0.reserve(sat_sino_bytes(mode, 1, 1, DNU, maxc)));
+    HIPCALL(launch_fill(P->onesV.as<float>(), 1.0f, npix, st));
+    HIPCALL(launch_fill(P->onesP.as<float>(), 1.0f, (size_t)maxc * DNU, st));
+    HIPCALL(launch_sat_vol(mode, 1, g0, 1, P->onesV.as<float>(), P->mask.as<uint8_t>(), P->satX0.p,
+                           P->satY0.p, st));
+    for (int k = 0; k < numOS; ++k) {
+        const Subset sub{k, numOS, J.cnt[k]};
+        HIPCALL(launch_proj(mode, 1, PROJ_PLAIN, g0, 1, P->cs0.as<float2>(), P->bx.as<float>(),
+                            P->by.as<float>(), P->xds.as<float>(), P->yds.as<float>(), P->satX0.p,
+                            P->satY0.p, sub, P->rowSum.as<float>() + (size_t)k * maxc * DNU,
+                            nullptr, nullptr, 0, 1, st));
+        HIPCALL(launch_sat_sino(mode, 1, DNU, 1, 1, J.cnt[k], P->onesP.as<float>(), P->satP0.p, st));
+        HIPCALL(launch_back(mode, 1, BACK_PLAIN, g0, 1, P->cs0.as<float2>(), P->mask.as<uint8_t>(),
+                            P->satP0.p, sub, P->colSum.as<float>() + (size_t)k * npix, nullptr,
+                            nullptr, nullptr, nullptr, st));
+    }
+    // FISTA momenta (OSSART_AAPM.m:111-115): MATLAB double scalars, applied to
+    // single arrays as single((t0-1)/t1).
+    J.h_mom.assign(std::max(J.numIter, 1), 0.0f);
+    double t0 = 1.0;
+    for (int it = 0; it < J.numIter; ++it) {
+        const double t1 = (1.0 + sqrt(1.0 + 4.0 * t0 * t0)) / 2.0;
+        J.h_mom[it] = (float)((t0 - 1.0) / t1);
+        t0 = t1;
+    }
+    HIPCALL(hipMemcpyAsync(P->momTab.p, J.h_mom.data(), sizeof(float) * J.h_mom.size(),
+                           hipMemcpyHostToDevice, st));
+    // iteration counter = 0 (the all-zero bit pattern of 0.0f)
+    HIPCALL(launch_fill(P->counter.as<float>(), 0.0f, 1, st));
+    if (J.fista)
+        HIPCALL(hipMemcpyAsync(P->xprev.p, J.x, sizeof(float) * nvox, hipMemcpyDeviceToDevice, st));
+    return DDFAN_OK;
+}
+
+hipError_t enqueue_batch_iteration(OsJob* const* jobs, int n, int mode, int viewsPerWave,
+                                   hipStream_t st, ddfan_plan* TP) {
+    int S = 0;
+    for (int i = 0; i < n; ++i) S = std::max(S, jobs[i]->numOS);
+    std::vector<SatVolJob> sv;
+    std::vector<ProjJob> pj;
+    std::vector<SatSinoJob> ss;
+    std::vector<BackJob> bj;
+    sv.reserve(n);
+    pj.reserve(n);
+    ss.reserve(n);
+    bj.reserve(n);
+    auto T = [&](int kind, auto&& f) -> hipError_t { return TP ? TP->timed(kind, f, st) : f(); };
+    for (int r = 0; r < S; ++r) {
+        sv.clear();
+        pj.clear();
+        ss.clear();
+        bj.clear();
+        for (int i = 0; i < n; ++i) {
+            const OsJob& J = *jobs[i];
+            const long long nO = J.numOS;
+            // subset k runs in round floor(k*S/nO); k*S/nO is increasing since S >= nO
+            const long long k = ((long long)r * nO + S - 1) / S;
+            if (k >= nO || (k * S) / nO != r) continue;
+            ddfan_plan* P = J.P;
+            const GeomDev& g = P->g;
+            const size_t npix = (size_t)g.XN * g.YN;
+            const Subset sub{(int)k, (int)nO, J.cnt[k]};
+            const int epi = (J.fista && k == nO - 1) ? BACK_UPDATE_FISTA : BACK_UPDATE;
+            sv.push_back(SatVolJob{g, P->nzc, P->ZT, J.x, P->mask.as<uint8_t>(), P->satX.p, P->satY.p});
+            pj.push_back(ProjJob{g, sub, P->nzc, P->ZT, P->slabTile, PROJ_RATIO, P->cs.as<float2>(),
+                                 P->bx.as<float>(), P->by.as<float>(), P->xds.as<float>(),
+                                 P->yds.as<float>(), P->satX.p, P->satY.p, P->rbuf.as<float>(),
+                                 J.meas, P->rowSum.as<float>() + (size_t)k * J.maxc * g.DNU});
+            ss.push_back(SatSinoJob{g.DNU, g.SLN, P->nzcb, P->ZTb, J.cnt[k], P->rbuf.as<float>(),
+                                    P->satP.p});
+            bj.push_back(BackJob{g, sub, P->nzcb, P->ZTb, epi, P->cs.as<float2>(),
+                                 P->mask.as<uint8_t>(), P->satP.p, J.x,
+                                 P->colSum.as<float>() + (size_t)k * npix, P->xprev.as<float>(),
+                                 P->momTab.as<float>(), P->counter.as<int>()});
+        }
+        const int m = (int)sv.size();
+        if (m == 0) continue;
+        hipError_t e = T(0, [&] { return launch_sat_vol_batch(mode, sv.data(), m, st); });
+        if (e != hipSuccess) return e;
+        e = T(1, [&] { return launch_proj_batch(mode, viewsPerWave, pj.data(), m, st); });
+        if (e != hipSuccess) return e;
+        e = T(2, [&] { return launch_sat_sino_batch(mode, ss.data(), m, st); });
+        if (e != hipSuccess) return e;
+        e = T(3, [&] { return launch_back_batch(mode, bj.data(), m, st); });
+        if (e != hipSuccess) return e;
+    }
+    std::vector<int*> ctr(n);
+    for (int i = 0; i < n; ++i) ctr[i] = jobs[i]->P->counter.as<int>();
+    return launch_counter_inc_batch(ctr.data(), n, st);
+}
+
+int IterGraph::capture(OsJob* const* jobs, int n, int mode, int viewsPerWave, hipStream_t st) {
+    reset();
+    HIPCALL(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const hipError_t ce = enqueue_batch_iteration(jobs, n, mode, viewsPerWave, st, nullptr);
+    const hipError_t ee = hipStreamEndCapture(st, &graph);
+    if (ce != hipSuccess) {
+        reset();
+        HIPCALL(ce);
+    }
+    HIPCALL(ee);
+    const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+        reset();
+        HIPCALL(ie);
+    }
+    return DDFAN_OK;
+}
+
+int IterGraph::launch(hipStream_t st) {
+    CHECK(exec, DDFAN_EINVAL, "graph not captured");
+    HIPCALL(hipGraphLaunch(exec, st));
+    return DDFAN_OK;
+}
+
+}  // namespace ddf
+
+namespace {
+
+// Waits for a stream when leaving scope: host staging and graphs that work in
+// flight still reads are destroyed only after it.
+struct SyncOnExit {
+    hipStream_t s;
+    ~SyncOnExit() {
+        if (s) (void)hipStreamSynchronize(s);
+    }
+};
+
+// Runs `iters` iterations of every job on `st`: eagerly, or as replays of one
+// captured iteration; returns after the stream has drained.
+int run_iterations(OsJob* const* jobs, int n, int mode, int V, int iters, bool graph,
+                   hipStream_t st, ddfan_plan* timingPlan) {
+    if (iters <= 0 || n == 0) return DDFAN_OK;
+    if (graph) {
+        IterGraph G;
+        SyncOnExit sync{st};  // before G is destroyed
+        TRY(G.capture(jobs, n, mode, V, st));
+        for (int it = 0; it < iters; ++it) TRY(G.launch(st));
+        HIPCALL(hipStreamSynchronize(st));
+        return DDFAN_OK;
+    }
+    for (int it = 0; it < iters; ++it)
+        HIPCALL(enqueue_batch_iteration(jobs, n, mode, V, st, timingPlan));
+    HIPCALL(hipStreamSynchronize(st));
+    return DDFAN_OK;
+}
+
+}  // namespace
+
+struct ddfan_batch {
+    int device = 0;
+    int mode = 0;
+    int viewsPerWave = 4;
+    hipStream_t stream = nullptr;
+    std::vector<OsJob> jobs;
+    ~ddfan_batch() {
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
+    }
+};
+
+extern "C" {
+
+int ddfan_ossart(ddfan_plan* P, const float* d_meas, float* d_x, int numIter, int numOS,
+                 int useFISTA, int flags) {
+    CHECK(P && d_meas && d_x, DDFAN_EINVAL, "NULL argument");
+    HIPCALL(hipSetDevice(P->device));
+    OsJob J;
+    J.P = P;
+    J.meas = d_meas;
+    J.x = d_x;
+    J.numIter = numIter;
+    J.numOS = numOS;
+    J.fista = useFISTA ? 1 : 0;
+    SyncOnExit sync{P->stream};  // J's staging outlives the uploads
+    TRY(ossart_prepare(J, P->stream));
+    OsJob* jp = &J;
+    const bool graph = (flags & DDFAN_OSSART_GRAPH) != 0;
+    return run_iterations(&jp, 1, P->mode, P->viewsPerWave, numIter, graph, P->stream,
+                          (P->timing && !graph) ? P : nullptr);
+}
+
+int ddfan_batch_create(int sat_mode, int views_per_wave, ddfan_batch** out) {
+    CHECK(out, DDFAN_EINVAL, "out is NULL");
+    *out = nullptr;
+    CHECK(sat_mode >= 0 && sat_mode <= 2, DDFAN_EINVAL, "sat_mode must be 0, 1 or 2");
+    CHECK(views_per_wave == 0 || views_per_wave == 1 || views_per_wave == 2 || views_per_wave == 4,
+          DDFAN_EINVAL, "views_per_wave must be 0, 1, 2 or 4");
+    int dev;
+    TRY(check_device(&dev));
+    ddfan_batch* B = new (std::nothrow) ddfan_batch();
+    CHECK(B, DDFAN_ENOMEM, "host allocation failed");
+    B->device = dev;
+    B->mode = sat_mode;
+    B->viewsPerWave = views_per_wave ? views_per_wave : 4;
+    const hipError_t e = hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete B;
+        HIPCALL(e);
+    }
+    *out = B;
+    return DDFAN_OK;
+}
+
+int ddfan_batch_add(ddfan_batch* B, ddfan_plan* P, const float* d_meas, float* d_x, int numIter,
+                    int numOS, int useFISTA, int* index) {
+    CHECK(B && P && d_meas && d_x, DDFAN_EINVAL, "NULL argument");
+    CHECK(P->device == B->device, DDFAN_EINVAL, "plan and batch are on different devices");
+    CHECK(P->mode == B->mode, DDFAN_EINVAL, "plan SAT mode %d differs from the batch's %d",
+          P->mode, B->mode);
+    CHECK(numIter >= 0, DDFAN_EINVAL, "numIter must be >= 0");
+    CHECK(numOS >= 1 && numOS <= P->PN, DDFAN_EINVAL, "numOS must be in [1, %d]", P->PN);
+    for (const OsJob& J : B->jobs)
+        CHECK(J.P != P, DDFAN_EINVAL, "a plan holds the state of one reconstruction: add it once");
+    OsJob J;
+    J.P = P;
+    J.meas = d_meas;
+    J.x = d_x;
+    J.numIter = numIter;
+    J.numOS = numOS;
+    J.fista = useFISTA ? 1 : 0;
+    B->jobs.push_back(std::move(J));
+    if (index) *index = (int)B->jobs.size() - 1;
+    return DDFAN_OK;
+}
+
+int ddfan_batch_size(const ddfan_batch* B, int* n) {
+    CHECK(B && n, DDFAN_EINVAL, "NULL argument");
+    *n = (int)B->jobs.size();
+    return DDFAN_OK;
+}
+
+int ddfan_batch_run(ddfan_batch* B, int flags) {
+    CHECK(B, DDFAN_EINVAL, "batch is NULL");
+    HIPCALL(hipSetDevice(B->device));
+    const int n = (int)B->jobs.size();
+    if (n == 0) return DDFAN_OK;
+    SyncOnExit sync{B->stream};
+    for (OsJob& J : B->jobs) TRY(ossart_prepare(J, B->stream));
+    // Segments of constant membership: a job leaves after its last iteration.
+    std::vector<int> left(n);
+    for (int i = 0; i < n; ++i) left[i] = B->jobs[i].numIter;
+    const bool graph = (flags & DDFAN_OSSART_GRAPH) != 0;
+    for (;;) {
+        std::vector<OsJob*> act;
+        int seg = 0;
+        for (int i = 0; i < n; ++i)
+            if (left[i] > 0) {
+                act.push_back(&B->jobs[i]);
+                seg = seg == 0 ? left[i] : std::min(seg, left[i]);
+            }
+        if (act.empty()) break;
+        TRY(run_iterations(act.data(), (int)act.size(), B->mode, B->viewsPerWave, seg, graph,
+                           B->stream, nullptr));
+        for (int i = 0; i < n; ++i)
+            if (left[i] > 0) left[i] -= seg;
+    }
+    return DDFAN_OK;
+}
+
+int ddfan_batch_destroy(ddfan_batch* B) {
+    if (!B) return DDFAN_OK;
+    (void)hipSetDevice(B->device);
+    delete B;
+    return DDFAN_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,437 @@
+// ddfan_serve.hip -- reconstruction request loop with continuous batching.
+//
+// Submissions are copied to host memory and queued.  A worker thread owns the
+// device: it admits waiting requests in arrival order while a slot is free and
+// their device footprint fits in the HBM budget (ddf::Admission), creates
+// their plans and uploads their sinograms, and runs every admitted
+// reconstruction together, one batched OS-SART iteration (a captured hipGraph)
+// at a time.  Requests join and leave at iteration boundaries, so a new one
+// starts while others are mid-run, and each request's arithmetic is the same
+// launch sequence it would get alone.  The worker keeps at most two batch
+// iterations in flight, finishes requests whose last iteration completed
+// (result copied back on a second stream while the next iteration runs) and
+// gives their bytes back to the budget.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "ddfan_plan.h"
+#include "ddfan_sched.h"
+
+using namespace ddf;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double ms_between(Clock::time_point a, Clock::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+struct Request {
+    long long id = 0;
+    // inputs (host copies)
+    ddfan_geom G{};
+    std::vector<float> xds, yds, hangs, meas, x0;
+    std::vector<uint8_t> mask;
+    int PN = 0, numIter = 0, numOS = 1, fista = 0;
+    size_t bytes = 0;
+    // device state while admitted
+    ddfan_plan* plan = nullptr;
+    DBuf d_meas, d_x;
+    OsJob job;
+    int left = 0;                    // iterations not yet enqueued
+    hipEvent_t lastIter = nullptr;   // recorded after its last iteration
+    // outcome
+    int status = DDFAN_OK;
+    std::string err;
+    bool done = false;
+    std::vector<float> result;
+    Clock::time_point tSubmit, tAdmit, tDone;
+
+    ~Request() { release_device(); }
+    void release_device() {
+        if (lastIter) (void)hipEventDestroy(lastIter);
+        lastIter = nullptr;
+        d_meas.release();
+        d_x.release();
+        if (plan) ddfan_plan_destroy(plan);
+        plan = nullptr;
+    }
+};
+
+}  // namespace
+
+struct ddfan_server {
+    int device = 0, mode = 0, viewsPerWave = 4;
+    hipStream_t stream = nullptr, copyStream = nullptr;
+    std::mutex mu;
+    std::condition_variable cvWork, cvDone;
+    ddf::Admission adm;
+    std::map<long long, std::unique_ptr<Request>> reqs;  // every request not yet collected
+    long long nextId = 1;
+    bool stop = false;
+    std::thread worker;
+    // statistics
+    long long completed = 0, failed = 0;
+    double busyVoxelViews = 0.0;
+
+    ddfan_server(size_t budget, int slots) : adm(budget, slots) {}
+    void loop();
+};
+
+namespace {
+
+// Device bytes one request holds while admitted: its plan with OS-SART state,
+// the measured sinogram and the iterate.
+int request_bytes(const ddfan_geom* G, int PN, int mode, int numOS, size_t* bytes) {
+    size_t b = 0;
+    TRY(ddfan_plan_footprint(G, PN, mode, 0, numOS, &b));
+    b += sizeof(float) * (size_t)PN * G->DNU * G->SLN;
+    b += sizeof(float) * (size_t)G->XN * G->YN * G->SLN;
+    *bytes = b;
+    return DDFAN_OK;
+}
+
+// Creates the plan, uploads the sinogram and initial iterate on `st` and
+// enqueues the OS-SART set-up.  Host inputs stay alive in the request.
+int admit_request(Request& R, int mode, hipStream_t st) {
+    ddfan_geom G = R.G;
+    G.xds = R.xds.data();
+    G.yds = R.yds.data();
+    TRY(ddfan_plan_create(&G, R.hangs.data(), R.PN, R.mask.empty() ? nullptr : R.mask.data(), mode,
+                          0, &R.plan));
+    const size_t nv = (size_t)G.XN * G.YN * G.SLN;
+    HIPCALL(R.d_meas.reserve(R.meas.size() * sizeof(float)));
+    HIPCALL(R.d_x.reserve(nv * sizeof(float)));
+    HIPCALL(hipMemcpyAsync(R.d_meas.p, R.meas.data(), R.meas.size() * sizeof(float),
+                           hipMemcpyHostToDevice, st));
+    if (R.x0.empty())
+        HIPCALL(launch_fill(R.d_x.as<float>(), 0.0f, nv, st));
+    else
+        HIPCALL(hipMemcpyAsync(R.d_x.p, R.x0.data(), nv * sizeof(float), hipMemcpyHostToDevice, st));
+    R.job.P = R.plan;
+    R.job.meas = R.d_meas.as<float>();
+    R.job.x = R.d_x.as<float>();
+    R.job.numIter = R.numIter;
+    R.job.numOS = R.numOS;
+    R.job.fista = R.fista;
+    TRY(ossart_prepare(R.job, st));
+    HIPCALL(hipEventCreateWithFlags(&R.lastIter, hipEventDisableTiming));
+    R.left = R.numIter;
+    return DDFAN_OK;
+}
+
+}  // namespace
+
+void ddfan_server::loop() {
+    (void)hipSetDevice(device);
+    std::vector<Request*> running;   // admitted, iterations left to enqueue
+    std::vector<Request*> finishing; // last iteration enqueued
+    IterGraph graphs[3];             // current graph + up to two with launches in flight
+    hipEvent_t iterEv[2] = {nullptr, nullptr};
+    (void)hipEventCreateWithFlags(&iterEv[0], hipEventDisableTiming);
+    (void)hipEventCreateWithFlags(&iterEv[1], hipEventDisableTiming);
+    int graphIdx[2] = {-1, -1};  // graph slot each in-flight iteration used
+    int cur = -1;                // slot of the graph matching `running`
+    long long iterNo = 0;
+    bool inFlight[2] = {false, false};
+
+    auto finish = [&](Request* R, int status, const std::string& err) {
+        R->release_device();
+        std::lock_guard<std::mutex> lk(mu);
+        R->status = status;
+        R->err = err;
+        R->done = true;
+        R->tDone = Clock::now();
+        adm.release(R->bytes);
+        if (status == DDFAN_OK)
+            ++completed;
+        else
+            ++failed;
+        cvDone.notify_all();
+        cvWork.notify_all();
+    };
+    auto fail_all_running = [&](int status, const std::string& err) {
+        for (Request* R : running) finish(R, status, err);
+        for (Request* R : finishing) finish(R, status, err);
+        running.clear();
+        finishing.clear();
+    };
+    // Copies back and completes the requests whose last iteration has run.
+    auto collect = [&]() {
+        for (size_t i = 0; i < finishing.size();) {
+            Request* R = finishing[i];
+            if (hipEventQuery(R->lastIter) != hipSuccess) {
+                ++i;
+                continue;
+            }
+            const size_t nv = (size_t)R->G.XN * R->G.YN * R->G.SLN;
+            R->result.resize(nv);
+            int st = DDFAN_OK;
+            std::string err;
+            if (hipMemcpyAsync(R->result.data(), R->d_x.p, nv * sizeof(float),
+                               hipMemcpyDeviceToHost, copyStream) != hipSuccess ||
+                hipStreamSynchronize(copyStream) != hipSuccess) {
+                st = DDFAN_EDEVICE;
+                err = "result copy failed";
+            }
+            finish(R, st, err);
+            finishing.erase(finishing.begin() + i);
+        }
+    };
+
+    for (;;) {
+        // admission
+        std::vector<Request*> admitted;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            for (;;) {
+                for (long long id : adm.admit()) {
+                    Request* R = reqs[id].get();
+                    R->tAdmit = Clock::now();
+                    admitted.push_back(R);
+                }
+                const bool idle = running.empty() && finishing.empty() && admitted.empty();
+                if (!idle) break;
+                if (stop && adm.waiting() == 0) break;
+                cvWork.wait(lk);
+            }
+            if (stop && running.empty() && finishing.empty() && admitted.empty() &&
+                adm.waiting() == 0)
+                break;
+        }
+        bool changed = false;
+        for (Request* R : admitted) {
+            const int rc = admit_request(*R, mode, stream);
+            if (rc != DDFAN_OK) {
+                // the request's set-up may still be in flight on the stream
+                (void)hipStreamSynchronize(stream);
+                finish(R, rc, ddfan_last_error());
+                continue;
+            }
+            if (R->left == 0) {  // zero iterations: the result is the initial iterate
+                (void)hipEventRecord(R->lastIter, stream);
+                finishing.push_back(R);
+            } else {
+                running.push_back(R);
+                changed = true;
+            }
+        }
+        if (running.empty()) {
+            (void)hipStreamSynchronize(stream);
+            collect();
+            continue;
+        }
+        // one batch iteration of every running request
+        const int slot = (int)(iterNo & 1);
+        if (inFlight[slot]) {  // keep at most two iterations in flight
+            (void)hipEventSynchronize(iterEv[slot]);
+            inFlight[slot] = false;
+            graphIdx[slot] = -1;
+        }
+        if (changed || cur < 0) {
+            int g = 0;  // a slot that no in-flight iteration uses
+            while (g == graphIdx[0] || g == graphIdx[1]) ++g;
+            std::vector<OsJob*> jobs;
+            for (Request* R : running) jobs.push_back(&R->job);
+            const int rc = graphs[g].capture(jobs.data(), (int)jobs.size(), mode, viewsPerWave,
+                                             stream);
+            if (rc != DDFAN_OK) {
+                (void)hipStreamSynchronize(stream);
+                fail_all_running(rc, ddfan_last_error());
+                cur = -1;
+                continue;
+            }
+            cur = g;
+        }
+        if (graphs[cur].launch(stream) != DDFAN_OK) {
+            (void)hipStreamSynchronize(stream);
+            fail_all_running(DDFAN_EDEVICE, ddfan_last_error());
+            cur = -1;
+            continue;
+        }
+        (void)hipEventRecord(iterEv[slot], stream);
+        inFlight[slot] = true;
+        graphIdx[slot] = cur;
+        ++iterNo;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            for (Request* R : running) {
+                const GeomDev& g = R->plan->g;
+                busyVoxelViews += 2.0 * g.XN * g.YN * g.SLN * (double)R->PN;
+            }
+        }
+        for (size_t i = 0; i < running.size();) {
+            Request* R = running[i];
+            if (--R->left == 0) {
+                (void)hipEventRecord(R->lastIter, stream);
+                finishing.push_back(R);
+                running.erase(running.begin() + i);
+                cur = -1;  // membership changes: capture again next time
+            } else {
+                ++i;
+            }
+        }
+        // wait for the previous iteration, then complete finished requests
+        const int prev = slot ^ 1;
+        if (inFlight[prev]) {
+            (void)hipEventSynchronize(iterEv[prev]);
+            inFlight[prev] = false;
+            graphIdx[prev] = -1;
+        }
+        collect();
+    }
+    (void)hipStreamSynchronize(stream);
+    collect();
+    for (auto& g : graphs) g.reset();
+    (void)hipEventDestroy(iterEv[0]);
+    (void)hipEventDestroy(iterEv[1]);
+}
+
+extern "C" {
+
+int ddfan_server_create(size_t hbm_budget, int max_slots, int sat_mode, ddfan_server** out) {
+    CHECK(out, DDFAN_EINVAL, "out is NULL");
+    *out = nullptr;
+    CHECK(max_slots >= 1, DDFAN_EINVAL, "max_slots must be >= 1");
+    CHECK(sat_mode >= 0 && sat_mode <= 2, DDFAN_EINVAL, "sat_mode must be 0, 1 or 2");
+    int dev;
+    TRY(check_device(&dev));
+    if (hbm_budget == 0) {
+        size_t fb = 0, tb = 0;
+        HIPCALL(hipMemGetInfo(&fb, &tb));
+        hbm_budget = fb / 10 * 8;  // 80% of what is free now
+    }
+    ddfan_server* S = new (std::nothrow) ddfan_server(hbm_budget, max_slots);
+    CHECK(S, DDFAN_ENOMEM, "host allocation failed");
+    S->device = dev;
+    S->mode = sat_mode;
+    hipError_t e = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&S->copyStream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        if (S->stream) (void)hipStreamDestroy(S->stream);
+        delete S;
+        HIPCALL(e);
+    }
+    try {
+        S->worker = std::thread([S] { S->loop(); });
+    } catch (...) {
+        (void)hipStreamDestroy(S->stream);
+        (void)hipStreamDestroy(S->copyStream);
+        delete S;
+        return fail(DDFAN_ENOMEM, "cannot start the worker thread");
+    }
+    *out = S;
+    return DDFAN_OK;
+}
+
+int ddfan_server_submit(ddfan_server* S, const ddfan_geom* G, const float* hangs, int PN,
+                        const unsigned char* mask, const float* meas, const float* x0,
+                        int numIter, int numOS, int useFISTA, long long* id) {
+    CHECK(S && G && hangs && meas && id, DDFAN_EINVAL, "NULL argument");
+    CHECK(G->XN >= 1 && G->YN >= 1 && G->SLN >= 1 && G->DNU >= 2 && G->xds && G->yds,
+          DDFAN_EINVAL, "bad geometry");
+    CHECK(PN >= 1 && numIter >= 0 && numOS >= 1 && numOS <= PN, DDFAN_EINVAL,
+          "bad view or iteration counts (PN %d numIter %d numOS %d)", PN, numIter, numOS);
+    size_t bytes = 0;
+    TRY(request_bytes(G, PN, S->mode, numOS, &bytes));
+    auto R = std::make_unique<Request>();
+    R->G = *G;
+    R->xds.assign(G->xds, G->xds + G->DNU);
+    R->yds.assign(G->yds, G->yds + G->DNU);
+    R->hangs.assign(hangs, hangs + (size_t)PN * G->SLN);
+    R->meas.assign(meas, meas + (size_t)PN * G->DNU * G->SLN);
+    const size_t nv = (size_t)G->XN * G->YN * G->SLN;
+    if (x0) R->x0.assign(x0, x0 + nv);
+    if (mask) R->mask.assign(mask, mask + (size_t)G->XN * G->YN);
+    R->PN = PN;
+    R->numIter = numIter;
+    R->numOS = numOS;
+    R->fista = useFISTA ? 1 : 0;
+    R->bytes = bytes;
+    R->tSubmit = Clock::now();
+    std::lock_guard<std::mutex> lk(S->mu);
+    CHECK(!S->stop, DDFAN_EINVAL, "server is shutting down");
+    const long long rid = S->nextId;
+    CHECK(S->adm.submit(rid, bytes), DDFAN_EBUSY,
+          "request needs %zu device bytes, more than the whole budget of %zu", bytes,
+          S->adm.budget());
+    ++S->nextId;
+    R->id = rid;
+    S->reqs[rid] = std::move(R);
+    *id = rid;
+    S->cvWork.notify_all();
+    return DDFAN_OK;
+}
+
+int ddfan_server_poll(ddfan_server* S, long long id, int* done) {
+    CHECK(S && done, DDFAN_EINVAL, "NULL argument");
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->reqs.find(id);
+    CHECK(it != S->reqs.end(), DDFAN_EINVAL, "unknown request %lld", id);
+    *done = it->second->done ? 1 : 0;
+    return DDFAN_OK;
+}
+
+int ddfan_server_wait(ddfan_server* S, long long id, float* x, double* queue_ms,
+                      double* latency_ms) {
+    CHECK(S, DDFAN_EINVAL, "server is NULL");
+    std::unique_ptr<Request> R;
+    {
+        std::unique_lock<std::mutex> lk(S->mu);
+        auto it = S->reqs.find(id);
+        CHECK(it != S->reqs.end(), DDFAN_EINVAL, "unknown request %lld", id);
+        Request* r = it->second.get();
+        S->cvDone.wait(lk, [&] { return r->done; });
+        R = std::move(it->second);
+        S->reqs.erase(it);
+    }
+    if (queue_ms) *queue_ms = R->status == DDFAN_OK ? ms_between(R->tSubmit, R->tAdmit) : 0.0;
+    if (latency_ms) *latency_ms = ms_between(R->tSubmit, R->tDone);
+    if (R->status != DDFAN_OK)
+        return fail(R->status, "request %lld failed: %s", id, R->err.c_str());
+    if (x) memcpy(x, R->result.data(), R->result.size() * sizeof(float));
+    return DDFAN_OK;
+}
+
+int ddfan_server_stats(ddfan_server* S, long long* completed, long long* failed, int* running,
+                       long long* waiting, size_t* used_bytes, double* voxel_views) {
+    CHECK(S, DDFAN_EINVAL, "server is NULL");
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (completed) *completed = S->completed;
+    if (failed) *failed = S->failed;
+    if (running) *running = S->adm.active();
+    if (waiting) *waiting = (long long)S->adm.waiting();
+    if (used_bytes) *used_bytes = S->adm.used();
+    if (voxel_views) *voxel_views = S->busyVoxelViews;
+    return DDFAN_OK;
+}
+
+int ddfan_server_destroy(ddfan_server* S) {
+    if (!S) return DDFAN_OK;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stop = true;
+        S->cvWork.notify_all();
+    }
+    if (S->worker.joinable()) S->worker.join();
+    (void)hipSetDevice(S->device);
+    S->reqs.clear();
+    (void)hipStreamDestroy(S->stream);
+    (void)hipStreamDestroy(S->copyStream);
+    delete S;
+    return DDFAN_OK;
+}
+
+}  // extern "C"

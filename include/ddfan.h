@@ -176,6 +176,54 @@ int ddfan_plan_sync(ddfan_plan* plan);
 int ddfan_ossart(ddfan_plan* plan, const float* d_meas, float* d_x, int numIter, int numOS,
                  int useFISTA, int flags);
 
+/* ------------------------------------------------------------ batched OS-SART
+ *
+ * Many independent reconstructions of uneven size (volume, slices, views,
+ * subsets, iterations) run together: every kernel launch of an iteration
+ * carries up to 8 reconstructions, and each iteration of the batch is one
+ * captured hipGraph.  The reference has no batched form; per reconstruction
+ * the arithmetic is that of ddfan_ossart (bit-identical results).  Each plan
+ * holds one reconstruction's state, so a plan is added once; all plans share
+ * the batch's SAT mode and device.  views_per_wave 0 picks the default (4). */
+typedef struct ddfan_batch ddfan_batch;
+int ddfan_batch_create(int sat_mode, int views_per_wave, ddfan_batch** out);
+int ddfan_batch_add(ddfan_batch* batch, ddfan_plan* plan, const float* d_meas, float* d_x,
+                    int numIter, int numOS, int useFISTA, int* index);
+int ddfan_batch_size(const ddfan_batch* batch, int* n);
+/* Runs every reconstruction to completion (flags: DDFAN_OSSART_GRAPH). */
+int ddfan_batch_run(ddfan_batch* batch, int flags);
+int ddfan_batch_destroy(ddfan_batch* batch);
+
+/* ------------------------------------------------------------ request loop
+ *
+ * A worker thread serves reconstruction requests on the current device.
+ * Requests are admitted in arrival order while one of max_slots is free and
+ * their device footprint fits in what is left of hbm_budget bytes (0: 80% of
+ * the free device memory); a request larger than the whole budget is
+ * rejected at submission (DDFAN_EBUSY).  Admitted requests keep their
+ * sinogram and iterate in device memory and run together as a batch
+ * (ddfan_batch_run's iteration), joining and leaving at iteration boundaries
+ * while others run.  The reference has no serving layer. */
+typedef struct ddfan_server ddfan_server;
+int ddfan_server_create(size_t hbm_budget, int max_slots, int sat_mode, ddfan_server** out);
+/* Copies the request's host inputs (meas [PN][DNU][SLN], x0 [YN][XN][SLN] or
+ * NULL for zeros, mask or NULL) and queues it; *id names it. */
+int ddfan_server_submit(ddfan_server* server, const ddfan_geom* geom, const float* hangs, int PN,
+                        const unsigned char* mask, const float* meas, const float* x0,
+                        int numIter, int numOS, int useFISTA, long long* id);
+int ddfan_server_poll(ddfan_server* server, long long id, int* done);
+/* Blocks until the request is done, copies its reconstruction to x (may be
+ * NULL) and returns the milliseconds from submission to admission and to
+ * completion.  A request is collected once; a failed one returns its status. */
+int ddfan_server_wait(ddfan_server* server, long long id, float* x, double* queue_ms,
+                      double* latency_ms);
+/* Counts so far; voxel_views sums 2*XN*YN*SLN*PN over every request iteration run. */
+int ddfan_server_stats(ddfan_server* server, long long* completed, long long* failed,
+                       int* running, long long* waiting, size_t* used_bytes,
+                       double* voxel_views);
+/* Finishes every queued request, then stops the worker. */
+int ddfan_server_destroy(ddfan_server* server);
+
 /* The reference's plain form on host buffers: every subset calls the
  * drop-in projector and backprojector (allocate, copy in, compute, copy out,
  * free) and the ratio/normalise/update run on the host (OSSART_AAPM.m:49-117

@@ -1,0 +1,92 @@
+"""Request loop: continuous batching under an HBM budget.
+
+Requests submitted while others run are admitted as slots and memory free
+up and share launches with them; each result must equal the same
+reconstruction run alone (Plan.ossart) bit for bit (tolerance 0).  A request
+larger than the budget is rejected at submission; a request that fails at
+admission fails alone."""
+import time
+
+import numpy as np
+import pytest
+
+from test_gpu_batch import JOBS, alone, make_job
+
+pytestmark = pytest.mark.gpu
+
+
+def footprint(dd, J, sat="twofloat"):
+    g = J["g"]
+    PN = J["hangs"].shape[0]
+    return (dd.plan_footprint(g, PN, sat, 0, J["nos"]) + 4 * PN * g.DNU * g.SLN
+            + 4 * g.XN * g.YN * g.SLN)
+
+
+def test_served_results_equal_alone_while_mixing():
+    import aapm_lowdosectimagerecon_competition_amd as dd
+    specs = JOBS + JOBS[:3]
+    jobs = [make_job(s, 80 + i) for i, s in enumerate(specs)]
+    ref = [alone(dd, J) for J in jobs]
+    print(f"\n[serve] {len(jobs)} staggered requests", flush=True)
+    with dd.Server(hbm_budget=0, max_slots=4) as S:
+        ids = []
+        for J in jobs:
+            ids.append(S.submit(J["g"], J["hangs"], J["meas"], J["it"], J["nos"], J["fista"],
+                                mask=J["mask"]))
+            time.sleep(0.002)
+        res = [S.wait(i) for i in ids]
+        st = S.stats()
+    print(f"[serve] stats {st}", flush=True)
+    assert st["completed"] == len(jobs) and st["failed"] == 0
+    for i, (r, a) in enumerate(zip(res, ref)):
+        nd = int(np.count_nonzero(r.x != a))
+        print(f"[serve] req {i}: queue {r.queue_ms:.1f} ms latency {r.latency_ms:.1f} ms, "
+              f"{nd} voxels differ from alone", flush=True)
+        assert nd == 0
+        assert 0.0 <= r.queue_ms <= r.latency_ms
+
+
+def test_budget_admission_and_rejection():
+    import aapm_lowdosectimagerecon_competition_amd as dd
+    jobs = [make_job(JOBS[2], 100 + i) for i in range(3)]
+    fp = footprint(dd, jobs[0])
+    print(f"\n[serve] budget {int(1.5 * fp)} B: one request at a time", flush=True)
+    ref = alone(dd, jobs[0])
+    with dd.Server(hbm_budget=int(1.5 * fp), max_slots=8) as S:
+        big = make_job((160, 160, 16, 96, 4, 1, 1, "uniform"), 5)
+        with pytest.raises(dd.DDFanError) as ei:
+            S.submit(big["g"], big["hangs"], big["meas"], 1, 4)
+        assert ei.value.status == -6  # DDFAN_EBUSY
+        ids = [S.submit(J["g"], J["hangs"], J["meas"], J["it"], J["nos"], J["fista"],
+                        mask=J["mask"]) for J in jobs]
+        res = [S.wait(i) for i in ids]
+        assert S.stats()["used_bytes"] == 0
+    assert np.array_equal(res[0].x, ref)
+    # admitted one after another: each waits for the previous one to finish
+    assert res[2].queue_ms >= res[1].queue_ms >= 0.0
+
+
+def test_failed_request_fails_alone():
+    import aapm_lowdosectimagerecon_competition_amd as dd
+    good = make_job(JOBS[0], 7)
+    bad = make_job(JOBS[0], 8)
+    bad["hangs"] = bad["hangs"].copy()
+    bad["hangs"][3, 1] = np.nan  # rejected when its plan is created
+    print("\n[serve] one request fails at admission", flush=True)
+    ref = alone(dd, good)
+    with dd.Server(max_slots=4) as S:
+        with pytest.raises(dd.DDFanError):
+            S.submit(good["g"], good["hangs"], good["meas"], 1, 0)  # numOS out of range
+        ia = S.submit(good["g"], good["hangs"], good["meas"], good["it"], good["nos"],
+                      good["fista"], mask=good["mask"])
+        ib = S.submit(bad["g"], bad["hangs"], bad["meas"], bad["it"], bad["nos"], bad["fista"],
+                      mask=bad["mask"])
+        ic = S.submit(good["g"], good["hangs"], good["meas"], good["it"], good["nos"],
+                      good["fista"], mask=good["mask"])
+        ra = S.wait(ia)
+        with pytest.raises(dd.DDFanError):
+            S.wait(ib)
+        rc = S.wait(ic)
+        st = S.stats()
+    assert st["failed"] == 1 and st["completed"] == 2
+    assert np.array_equal(ra.x, ref) and np.array_equal(rc.x, ref)
