@@ -51,7 +51,7 @@ def traffic_from_profiles(kernel: str):
         return None
     for line in open(path):
         parts = line.split()
-        if len(parts) >= 2 and parts[0] == kernel:
+        if len(parts) == 2 and parts[0] == kernel:  # "key bytes" lines
             try:
                 return float(parts[1])
             except ValueError:
@@ -59,7 +59,7 @@ def traffic_from_profiles(kernel: str):
     return None
 
 
-def cpu_baseline(g, hangs, vol, mask, target_s=12.0):
+def cpu_baseline(g, hangs, vol, mask, target_s=20.0):
     """FP32 restatement (projector + backprojector) on a bounded sample of
     the workload's views, scaled to voxel-views/s."""
     import numpy as np
