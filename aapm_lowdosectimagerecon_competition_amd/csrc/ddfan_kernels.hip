@@ -675,13 +675,11 @@ static int sat_group(int mode, int K) {
     return 0;
 }
 
-// One launch over up to 2*kBatch tables (all rows of every table).
-static hipError_t launch_sat_tables(int mode, SatDev* t, int n, hipStream_t st) {
+// One launch over tables that share G (all rows of every table).
+static hipError_t launch_sat_group(int mode, int G, SatDev* t, int n, hipStream_t st) {
     if (n == 0) return hipSuccess;
     int maxK = 0;
     for (int i = 0; i < n; ++i) maxK = t[i].Nk + 1 > maxK ? t[i].Nk + 1 : maxK;
-    const int G = sat_group(mode, maxK);
-    if (G == 0) return hipErrorInvalidValue;
     SatLaunch L{};
     long long nb = 0;
     int m = 0;
@@ -708,6 +706,24 @@ static hipError_t launch_sat_tables(int mode, SatDev* t, int n, hipStream_t st) 
     }
 #undef DDF_SATV
     return hipGetLastError();
+}
+
+// Up to 2*kBatch tables.  G (slices per block) fixes how a row is split into
+// scan segments, and so the summation order: it is chosen from each table's
+// own row length, never from the other tables of the launch, so a table's
+// entries are the same whatever it is batched with.
+static hipError_t launch_sat_tables(int mode, SatDev* t, int n, hipStream_t st) {
+    for (int G : {16, 8, 4, 2, 1}) {
+        SatDev grp[2 * kBatch];
+        int m = 0;
+        for (int i = 0; i < n; ++i)
+            if (sat_group(mode, t[i].Nk + 1) == G) grp[m++] = t[i];
+        const hipError_t e = launch_sat_group(mode, G, grp, m, st);
+        if (e != hipSuccess) return e;
+    }
+    for (int i = 0; i < n; ++i)
+        if (sat_group(mode, t[i].Nk + 1) == 0) return hipErrorInvalidValue;
+    return hipSuccess;
 }
 
 hipError_t launch_sat_vol_batch(int mode, const SatVolJob* jobs, int n, hipStream_t st) {
@@ -818,20 +834,32 @@ static hipError_t launch_proj_group(int mode, int ZT, int V, const ProjJob* cons
     return hipSuccess;
 }
 
+// Batched launches stop adding jobs once they hold about one GPU's worth of
+// resident waves (256 CUs x 4 SIMDs x the kernel's waves per SIMD): small
+// reconstructions share launches, large ones keep the GPU (and its caches) to
+// themselves.  Grouping never changes a job's arithmetic.
+constexpr long long kProjFillWaves = 256LL * 4 * DDF_PROJ_WAVES;
+constexpr long long kBackFillBlocks = 256LL * 4 * DDF_BACK_WAVES / 4;
+
 hipError_t launch_proj_batch(int mode, int viewsPerWave, const ProjJob* jobs, int n,
                              hipStream_t st) {
     const int V = viewsPerWave == 2 || viewsPerWave == 4 ? viewsPerWave : 1;
     for (int ZT : {1, 2, 4, 8, 16}) {
         const ProjJob* grp[kBatch];
         int m = 0;
+        long long work = 0;
         for (int i = 0; i < n; ++i) {
-            if (jobs[i].ZT != ZT) continue;
-            grp[m++] = &jobs[i];
-            if (m == kBatch) {
+            const ProjJob& J = jobs[i];
+            if (J.ZT != ZT || J.sub.count <= 0) continue;
+            const long long w = (long long)((J.g.DNU + 62) / 63) * ((J.sub.count + V - 1) / V) * J.nzc;
+            if (m == kBatch || (m > 0 && work + w > kProjFillWaves)) {
                 const hipError_t e = launch_proj_group(mode, ZT, V, grp, m, st);
                 if (e != hipSuccess) return e;
                 m = 0;
+                work = 0;
             }
+            grp[m++] = &J;
+            work += w;
         }
         if (m) {
             const hipError_t e = launch_proj_group(mode, ZT, V, grp, m, st);
@@ -890,14 +918,19 @@ hipError_t launch_back_batch(int mode, const BackJob* jobs, int n, hipStream_t s
     for (int ZT : {1, 2, 4, 8, 16}) {
         const BackJob* grp[kBatch];
         int m = 0;
+        long long work = 0;
         for (int i = 0; i < n; ++i) {
-            if (jobs[i].ZT != ZT) continue;
-            grp[m++] = &jobs[i];
-            if (m == kBatch) {
+            const BackJob& J = jobs[i];
+            if (J.ZT != ZT) continue;
+            const long long w = (long long)((J.g.XN + 15) / 16) * ((J.g.YN + 15) / 16) * J.nzc;
+            if (m == kBatch || (m > 0 && work + w > kBackFillBlocks)) {
                 const hipError_t e = launch_back_group(mode, ZT, grp, m, st);
                 if (e != hipSuccess) return e;
                 m = 0;
+                work = 0;
             }
+            grp[m++] = &J;
+            work += w;
         }
         if (m) {
             const hipError_t e = launch_back_group(mode, ZT, grp, m, st);

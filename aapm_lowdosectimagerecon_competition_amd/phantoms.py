@@ -115,3 +115,33 @@ def uniform_angles(PN, SLN, start=0.0, span=2 * np.pi):
     """hangs[v, z] = start + span*v/PN for every slice (float32)."""
     a = (start + span * np.arange(PN) / PN).astype(np.float32)
     return np.ascontiguousarray(np.repeat(a[:, None], SLN, axis=1))
+
+
+
+def gaussian_count_sinogram(prj, dx, I0, seed=2, threads=None):
+    """Fast large-array stand-in for :func:`poisson_sinogram`: counts
+    lam + sqrt(lam)*N(0,1) with lam = I0 exp(-prj*dx) (the normal
+    approximation of Poisson(lam)), p = -ln(max(counts, 1)/I0)/dx, in float32.
+    Views are split into chunks, each with its own child seed of ``seed``, and
+    processed on a thread pool (numpy releases the GIL in these loops)."""
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+    p = np.asarray(prj, np.float32)
+    out = np.empty_like(p)
+    nchunk = max(1, min(64, p.shape[0]))
+    bounds = np.linspace(0, p.shape[0], nchunk + 1).astype(int)
+    seeds = np.random.SeedSequence(seed).spawn(nchunk)
+    f_dx, f_I0 = np.float32(dx), np.float32(I0)
+
+    def work(i):
+        a, b = bounds[i], bounds[i + 1]
+        rng = np.random.default_rng(seeds[i])
+        lam = f_I0 * np.exp(-p[a:b] * f_dx)
+        c = lam + np.sqrt(lam) * rng.standard_normal(lam.shape, dtype=np.float32)
+        np.maximum(c, np.float32(1.0), out=c)
+        out[a:b] = -np.log(c / f_I0) / f_dx
+
+    n = threads or min(16, os.cpu_count() or 1)
+    with ThreadPoolExecutor(max_workers=n) as ex:
+        list(ex.map(work, range(nchunk)))
+    return out
