@@ -13,7 +13,7 @@ from .batch import Batch
 from .dd2 import dd2_multi_slices
 from .geometry import Conf, detector_arc, detector_geometry, recon_mask, subset_views
 from .ossart import ossart_aapm
-from .plan import Plan, plan_footprint
+from .plan import Plan, plan_footprint, tune_clear, tune_load, tune_save
 from .rebin import HelicalCfg, helical_to_fan_func, helical_to_fan_routine
 from .serve import Result, Server
 
@@ -21,7 +21,7 @@ __all__ = [
     "Batch", "Conf", "DDFanError", "DeviceBuffer", "HelicalCfg", "Plan", "dd2_multi_slices",
     "detector_arc", "detector_geometry", "device_count", "helical_to_fan_func",
     "helical_to_fan_routine", "ossart_aapm", "plan_footprint", "recon_mask", "Result", "Server",
-    "subset_views",
+    "subset_views", "tune_clear", "tune_load", "tune_save",
 ]
 
 

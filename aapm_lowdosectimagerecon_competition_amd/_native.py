@@ -40,6 +40,9 @@ PARAM_SLAB_TILE = 1
 PARAM_VIEWS_PER_WAVE = 2
 PARAM_PROJ_CHUNK = 3
 PARAM_BACK_CHUNK = 4
+TUNE_USE_CACHE = 1
+BATCH_PROJ_FILL = 1
+BATCH_BACK_FILL = 2
 
 
 class DDFanError(RuntimeError):
@@ -93,9 +96,10 @@ _SIGS = {
     "ddfan_plan_timing_read": [_p, _i, ctypes.POINTER(ctypes.c_double), _ip],
     "ddfan_plan_trace_proj": [_p, _i, _i, _i, _p],
     "ddfan_plan_trace_back": [_p, _i, _i, _i, _p],
-    "ddfan_batch_create": [_i, _i, ctypes.POINTER(_p)],
+    "ddfan_batch_create": [_i, ctypes.POINTER(_p)],
     "ddfan_batch_add": [_p, _p, _p, _p, _i, _i, _i, _ip],
     "ddfan_batch_size": [_p, _ip],
+    "ddfan_batch_set_param": [_p, _i, _ll],
     "ddfan_batch_run": [_p, _i],
     "ddfan_batch_destroy": [_p],
     "ddfan_server_create": [_sz, _i, _i, ctypes.POINTER(_p)],
@@ -107,6 +111,11 @@ _SIGS = {
     "ddfan_server_stats": [_p, ctypes.POINTER(_ll), ctypes.POINTER(_ll), _ip, ctypes.POINTER(_ll),
                            ctypes.POINTER(_sz), ctypes.POINTER(ctypes.c_double)],
     "ddfan_server_destroy": [_p],
+    "ddfan_plan_autotune": [_p, _i, _i, _i, _ip, ctypes.POINTER(ctypes.c_double)],
+    "ddfan_plan_apply_tuned": [_p, _i, _ip],
+    "ddfan_tune_save": [ctypes.c_char_p],
+    "ddfan_tune_load": [ctypes.c_char_p, _ip],
+    "ddfan_tune_clear": [],
     "ddfan_helical_to_fan_dev": [_p, _p, _p, _i, _f, _i, _ll, _p, _i, _f, _f, _f, _i, _i, _f, _f,
                                  _i, _f, _f, _p, _p],
 }

@@ -15,9 +15,9 @@ class Batch:
     """Wraps ``ddfan_batch``.  Every plan must use ``sat_mode``; a plan holds
     one reconstruction's state and is added once."""
 
-    def __init__(self, sat_mode="twofloat", views_per_wave: int = 0):
+    def __init__(self, sat_mode="twofloat"):
         h = ctypes.c_void_p()
-        N.call("ddfan_batch_create", _sat_mode(sat_mode), int(views_per_wave), ctypes.byref(h))
+        N.call("ddfan_batch_create", _sat_mode(sat_mode), ctypes.byref(h))
         self.handle = h.value
         self._keep = []  # plans and buffers referenced by the batch
 
@@ -27,6 +27,14 @@ class Batch:
                int(num_iter), int(num_os), int(use_fista), ctypes.byref(idx))
         self._keep.append((plan, d_meas, d_x))
         return idx.value
+
+    def set_fill(self, proj_waves=None, back_blocks=None):
+        """Launch-group limits: reconstructions share a launch while it holds at
+        most proj_waves projector waves / back_blocks backprojector blocks."""
+        if proj_waves is not None:
+            N.call("ddfan_batch_set_param", self.handle, N.BATCH_PROJ_FILL, int(proj_waves))
+        if back_blocks is not None:
+            N.call("ddfan_batch_set_param", self.handle, N.BATCH_BACK_FILL, int(back_blocks))
 
     def __len__(self):
         n = ctypes.c_int()

@@ -834,36 +834,36 @@ static hipError_t launch_proj_group(int mode, int ZT, int V, const ProjJob* cons
     return hipSuccess;
 }
 
-// Batched launches stop adding jobs once they hold about one GPU's worth of
-// resident waves (256 CUs x 4 SIMDs x the kernel's waves per SIMD): small
-// reconstructions share launches, large ones keep the GPU (and its caches) to
-// themselves.  Grouping never changes a job's arithmetic.
-constexpr long long kProjFillWaves = 256LL * 4 * DDF_PROJ_WAVES;
-constexpr long long kBackFillBlocks = 256LL * 4 * DDF_BACK_WAVES / 4;
-
-hipError_t launch_proj_batch(int mode, int viewsPerWave, const ProjJob* jobs, int n,
-                             hipStream_t st) {
-    const int V = viewsPerWave == 2 || viewsPerWave == 4 ? viewsPerWave : 1;
-    for (int ZT : {1, 2, 4, 8, 16}) {
-        const ProjJob* grp[kBatch];
-        int m = 0;
-        long long work = 0;
-        for (int i = 0; i < n; ++i) {
-            const ProjJob& J = jobs[i];
-            if (J.ZT != ZT || J.sub.count <= 0) continue;
-            const long long w = (long long)((J.g.DNU + 62) / 63) * ((J.sub.count + V - 1) / V) * J.nzc;
-            if (m == kBatch || (m > 0 && work + w > kProjFillWaves)) {
+// Batched launches stop adding jobs once the launch holds fillWaves projector
+// waves (fillBlocks backprojector blocks): small reconstructions share
+// launches, large ones keep the GPU (and its caches) to themselves.  Grouping
+// never changes a job's arithmetic.
+hipError_t launch_proj_batch(int mode, const ProjJob* jobs, int n, hipStream_t st,
+                             long long fillWaves) {
+    for (int V : {1, 2, 4}) {
+        for (int ZT : {1, 2, 4, 8, 16}) {
+            const ProjJob* grp[kBatch];
+            int m = 0;
+            long long work = 0;
+            for (int i = 0; i < n; ++i) {
+                const ProjJob& J = jobs[i];
+                const int Vj = J.V == 2 || J.V == 4 ? J.V : 1;
+                if (J.ZT != ZT || Vj != V || J.sub.count <= 0) continue;
+                const long long w =
+                    (long long)((J.g.DNU + 62) / 63) * ((J.sub.count + V - 1) / V) * J.nzc;
+                if (m == kBatch || (m > 0 && work + w > fillWaves)) {
+                    const hipError_t e = launch_proj_group(mode, ZT, V, grp, m, st);
+                    if (e != hipSuccess) return e;
+                    m = 0;
+                    work = 0;
+                }
+                grp[m++] = &J;
+                work += w;
+            }
+            if (m) {
                 const hipError_t e = launch_proj_group(mode, ZT, V, grp, m, st);
                 if (e != hipSuccess) return e;
-                m = 0;
-                work = 0;
             }
-            grp[m++] = &J;
-            work += w;
-        }
-        if (m) {
-            const hipError_t e = launch_proj_group(mode, ZT, V, grp, m, st);
-            if (e != hipSuccess) return e;
         }
     }
     return hipSuccess;
@@ -875,9 +875,9 @@ hipError_t launch_proj(int mode, int ZT, int epi, const GeomDev& g, int nzc, con
                        const float* meas, const float* rowSum, int slabTile, int viewsPerWave,
                        hipStream_t st) {
     if (sub.count <= 0) return hipSuccess;
-    const ProjJob j{g, sub, nzc, ZT, slabTile, epi, cossin, bx, by, xds, yds, satX, satY, out, meas,
-                    rowSum};
-    return launch_proj_batch(mode, viewsPerWave, &j, 1, st);
+    const ProjJob j{g, sub, nzc, ZT, slabTile, epi, viewsPerWave, cossin, bx, by, xds, yds, satX, satY,
+                    out, meas, rowSum};
+    return launch_proj_batch(mode, &j, 1, st);
 }
 
 static hipError_t launch_back_group(int mode, int ZT, const BackJob* const* jobs, int n,
@@ -914,7 +914,8 @@ static hipError_t launch_back_group(int mode, int ZT, const BackJob* const* jobs
     return hipGetLastError();
 }
 
-hipError_t launch_back_batch(int mode, const BackJob* jobs, int n, hipStream_t st) {
+hipError_t launch_back_batch(int mode, const BackJob* jobs, int n, hipStream_t st,
+                             long long fillBlocks) {
     for (int ZT : {1, 2, 4, 8, 16}) {
         const BackJob* grp[kBatch];
         int m = 0;
@@ -923,7 +924,7 @@ hipError_t launch_back_batch(int mode, const BackJob* jobs, int n, hipStream_t s
             const BackJob& J = jobs[i];
             if (J.ZT != ZT) continue;
             const long long w = (long long)((J.g.XN + 15) / 16) * ((J.g.YN + 15) / 16) * J.nzc;
-            if (m == kBatch || (m > 0 && work + w > kBackFillBlocks)) {
+            if (m == kBatch || (m > 0 && work + w > fillBlocks)) {
                 const hipError_t e = launch_back_group(mode, ZT, grp, m, st);
                 if (e != hipSuccess) return e;
                 m = 0;

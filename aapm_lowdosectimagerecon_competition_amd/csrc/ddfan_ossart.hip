@@ -88,7 +88,7 @@ int ossart_prepare(OsJob& J, hipStream_t st) {
     return DDFAN_OK;
 }
 
-hipError_t enqueue_batch_iteration(OsJob* const* jobs, int n, int mode, int viewsPerWave,
+hipError_t enqueue_batch_iteration(OsJob* const* jobs, int n, int mode, const BatchShape& shape,
                                    hipStream_t st, ddfan_plan* TP) {
     int S = 0;
     for (int i = 0; i < n; ++i) S = std::max(S, jobs[i]->numOS);
@@ -118,7 +118,8 @@ hipError_t enqueue_batch_iteration(OsJob* const* jobs, int n, int mode, int view
             const Subset sub{(int)k, (int)nO, J.cnt[k]};
             const int epi = (J.fista && k == nO - 1) ? BACK_UPDATE_FISTA : BACK_UPDATE;
             sv.push_back(SatVolJob{g, P->nzc, P->ZT, J.x, P->mask.as<uint8_t>(), P->satX.p, P->satY.p});
-            pj.push_back(ProjJob{g, sub, P->nzc, P->ZT, P->slabTile, PROJ_RATIO, P->cs.as<float2>(),
+            pj.push_back(ProjJob{g, sub, P->nzc, P->ZT, P->slabTile, PROJ_RATIO, P->viewsPerWave,
+                                 P->cs.as<float2>(),
                                  P->bx.as<float>(), P->by.as<float>(), P->xds.as<float>(),
                                  P->yds.as<float>(), P->satX.p, P->satY.p, P->rbuf.as<float>(),
                                  J.meas, P->rowSum.as<float>() + (size_t)k * J.maxc * g.DNU});
@@ -133,11 +134,13 @@ hipError_t enqueue_batch_iteration(OsJob* const* jobs, int n, int mode, int view
         if (m == 0) continue;
         hipError_t e = T(0, [&] { return launch_sat_vol_batch(mode, sv.data(), m, st); });
         if (e != hipSuccess) return e;
-        e = T(1, [&] { return launch_proj_batch(mode, viewsPerWave, pj.data(), m, st); });
+        e = T(1, [&] {
+            return launch_proj_batch(mode, pj.data(), m, st, shape.projFill);
+        });
         if (e != hipSuccess) return e;
         e = T(2, [&] { return launch_sat_sino_batch(mode, ss.data(), m, st); });
         if (e != hipSuccess) return e;
-        e = T(3, [&] { return launch_back_batch(mode, bj.data(), m, st); });
+        e = T(3, [&] { return launch_back_batch(mode, bj.data(), m, st, shape.backFill); });
         if (e != hipSuccess) return e;
     }
     std::vector<int*> ctr(n);
@@ -145,10 +148,11 @@ hipError_t enqueue_batch_iteration(OsJob* const* jobs, int n, int mode, int view
     return launch_counter_inc_batch(ctr.data(), n, st);
 }
 
-int IterGraph::capture(OsJob* const* jobs, int n, int mode, int viewsPerWave, hipStream_t st) {
+int IterGraph::capture(OsJob* const* jobs, int n, int mode, const BatchShape& shape,
+                       hipStream_t st) {
     reset();
     HIPCALL(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    const hipError_t ce = enqueue_batch_iteration(jobs, n, mode, viewsPerWave, st, nullptr);
+    const hipError_t ce = enqueue_batch_iteration(jobs, n, mode, shape, st, nullptr);
     const hipError_t ee = hipStreamEndCapture(st, &graph);
     if (ce != hipSuccess) {
         reset();
@@ -184,19 +188,19 @@ struct SyncOnExit {
 
 // Runs `iters` iterations of every job on `st`: eagerly, or as replays of one
 // captured iteration; returns after the stream has drained.
-int run_iterations(OsJob* const* jobs, int n, int mode, int V, int iters, bool graph,
-                   hipStream_t st, ddfan_plan* timingPlan) {
+int run_iterations(OsJob* const* jobs, int n, int mode, const BatchShape& shape, int iters,
+                   bool graph, hipStream_t st, ddfan_plan* timingPlan) {
     if (iters <= 0 || n == 0) return DDFAN_OK;
     if (graph) {
         IterGraph G;
         SyncOnExit sync{st};  // before G is destroyed
-        TRY(G.capture(jobs, n, mode, V, st));
+        TRY(G.capture(jobs, n, mode, shape, st));
         for (int it = 0; it < iters; ++it) TRY(G.launch(st));
         HIPCALL(hipStreamSynchronize(st));
         return DDFAN_OK;
     }
     for (int it = 0; it < iters; ++it)
-        HIPCALL(enqueue_batch_iteration(jobs, n, mode, V, st, timingPlan));
+        HIPCALL(enqueue_batch_iteration(jobs, n, mode, shape, st, timingPlan));
     HIPCALL(hipStreamSynchronize(st));
     return DDFAN_OK;
 }
@@ -206,7 +210,7 @@ int run_iterations(OsJob* const* jobs, int n, int mode, int V, int iters, bool g
 struct ddfan_batch {
     int device = 0;
     int mode = 0;
-    int viewsPerWave = 4;
+    BatchShape shape;
     hipStream_t stream = nullptr;
     std::vector<OsJob> jobs;
     ~ddfan_batch() {
@@ -234,23 +238,20 @@ int ddfan_ossart(ddfan_plan* P, const float* d_meas, float* d_x, int numIter, in
     TRY(ossart_prepare(J, P->stream));
     OsJob* jp = &J;
     const bool graph = (flags & DDFAN_OSSART_GRAPH) != 0;
-    return run_iterations(&jp, 1, P->mode, P->viewsPerWave, numIter, graph, P->stream,
+    return run_iterations(&jp, 1, P->mode, BatchShape{}, numIter, graph, P->stream,
                           (P->timing && !graph) ? P : nullptr);
 }
 
-int ddfan_batch_create(int sat_mode, int views_per_wave, ddfan_batch** out) {
+int ddfan_batch_create(int sat_mode, ddfan_batch** out) {
     CHECK(out, DDFAN_EINVAL, "out is NULL");
     *out = nullptr;
     CHECK(sat_mode >= 0 && sat_mode <= 2, DDFAN_EINVAL, "sat_mode must be 0, 1 or 2");
-    CHECK(views_per_wave == 0 || views_per_wave == 1 || views_per_wave == 2 || views_per_wave == 4,
-          DDFAN_EINVAL, "views_per_wave must be 0, 1, 2 or 4");
     int dev;
     TRY(check_device(&dev));
     ddfan_batch* B = new (std::nothrow) ddfan_batch();
     CHECK(B, DDFAN_ENOMEM, "host allocation failed");
     B->device = dev;
     B->mode = sat_mode;
-    B->viewsPerWave = views_per_wave ? views_per_wave : 4;
     const hipError_t e = hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete B;
@@ -282,6 +283,22 @@ int ddfan_batch_add(ddfan_batch* B, ddfan_plan* P, const float* d_meas, float* d
     return DDFAN_OK;
 }
 
+int ddfan_batch_set_param(ddfan_batch* B, int param, long long value) {
+    CHECK(B, DDFAN_EINVAL, "batch is NULL");
+    switch (param) {
+        case DDFAN_BATCH_PROJ_FILL:
+            CHECK(value >= 1, DDFAN_EINVAL, "projector launch limit must be >= 1");
+            B->shape.projFill = value;
+            return DDFAN_OK;
+        case DDFAN_BATCH_BACK_FILL:
+            CHECK(value >= 1, DDFAN_EINVAL, "backprojector launch limit must be >= 1");
+            B->shape.backFill = value;
+            return DDFAN_OK;
+        default:
+            return fail(DDFAN_EINVAL, "unknown batch parameter %d", param);
+    }
+}
+
 int ddfan_batch_size(const ddfan_batch* B, int* n) {
     CHECK(B && n, DDFAN_EINVAL, "NULL argument");
     *n = (int)B->jobs.size();
@@ -308,8 +325,8 @@ int ddfan_batch_run(ddfan_batch* B, int flags) {
                 seg = seg == 0 ? left[i] : std::min(seg, left[i]);
             }
         if (act.empty()) break;
-        TRY(run_iterations(act.data(), (int)act.size(), B->mode, B->viewsPerWave, seg, graph,
-                           B->stream, nullptr));
+        TRY(run_iterations(act.data(), (int)act.size(), B->mode, B->shape, seg, graph, B->stream,
+                           nullptr));
         for (int i = 0; i < n; ++i)
             if (left[i] > 0) left[i] -= seg;
     }

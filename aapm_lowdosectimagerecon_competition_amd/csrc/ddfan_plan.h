@@ -154,13 +154,20 @@ struct OsJob {
 // reset and the FISTA copy of the initial iterate.
 int ossart_prepare(OsJob& J, hipStream_t st);
 
+// Launch-group limits (waves / blocks) of a batched iteration's
+// launch_proj_batch / launch_back_batch calls.
+struct BatchShape {
+    long long projFill = kProjFillWaves;
+    long long backFill = kBackFillBlocks;
+};
+
 // Enqueues one OS-SART iteration of every job on `st`.  The iteration is
 // S = max(numOS) rounds; job j runs its subset k in round floor(k*S/numOS_j),
 // so a job's subsets keep their order whatever else shares the launches.
 // Each round is one batched launch per kernel kind; the iteration ends with
 // every job's counter increment.  `timingPlan`, when not null, brackets the
 // launches of each kind with its events.
-hipError_t enqueue_batch_iteration(OsJob* const* jobs, int n, int mode, int viewsPerWave,
+hipError_t enqueue_batch_iteration(OsJob* const* jobs, int n, int mode, const BatchShape& shape,
                                    hipStream_t st, ddfan_plan* timingPlan);
 
 // One batch iteration captured as a hipGraph (stream capture of
@@ -180,7 +187,7 @@ struct IterGraph {
         exec = nullptr;
         graph = nullptr;
     }
-    int capture(OsJob* const* jobs, int n, int mode, int viewsPerWave, hipStream_t st);
+    int capture(OsJob* const* jobs, int n, int mode, const BatchShape& shape, hipStream_t st);
     int launch(hipStream_t st);
 };
 

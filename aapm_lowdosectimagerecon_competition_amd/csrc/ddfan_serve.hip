@@ -73,7 +73,8 @@ struct Request {
 }  // namespace
 
 struct ddfan_server {
-    int device = 0, mode = 0, viewsPerWave = 4;
+    int device = 0, mode = 0;
+    BatchShape shape;
     hipStream_t stream = nullptr, copyStream = nullptr;
     std::mutex mu;
     std::condition_variable cvWork, cvDone;
@@ -111,6 +112,8 @@ int admit_request(Request& R, int mode, hipStream_t st) {
     G.yds = R.yds.data();
     TRY(ddfan_plan_create(&G, R.hangs.data(), R.PN, R.mask.empty() ? nullptr : R.mask.data(), mode,
                           0, &R.plan));
+    int tuned = 0;  // recorded launch shapes for this size of subset, if any
+    TRY(ddfan_plan_apply_tuned(R.plan, subset_views(R.PN, R.numOS, 0, nullptr), &tuned));
     const size_t nv = (size_t)G.XN * G.YN * G.SLN;
     HIPCALL(R.d_meas.reserve(R.meas.size() * sizeof(float)));
     HIPCALL(R.d_x.reserve(nv * sizeof(float)));
@@ -245,8 +248,7 @@ void ddfan_server::loop() {
             while (g == graphIdx[0] || g == graphIdx[1]) ++g;
             std::vector<OsJob*> jobs;
             for (Request* R : running) jobs.push_back(&R->job);
-            const int rc = graphs[g].capture(jobs.data(), (int)jobs.size(), mode, viewsPerWave,
-                                             stream);
+            const int rc = graphs[g].capture(jobs.data(), (int)jobs.size(), mode, shape, stream);
             if (rc != DDFAN_OK) {
                 (void)hipStreamSynchronize(stream);
                 fail_all_running(rc, ddfan_last_error());

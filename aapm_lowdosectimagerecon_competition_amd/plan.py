@@ -138,6 +138,21 @@ class Plan:
     def back_chunk(self, v):
         self.set_param(N.PARAM_BACK_CHUNK, v)
 
+    def autotune(self, views=None, reps=3, use_cache=True) -> dict:
+        """Times launch shapes for calls of `views` views (default PN), sets
+        the fastest and records it (see tune_save / tune_load)."""
+        prm = (ctypes.c_int * 4)()
+        ms = (ctypes.c_double * 2)()
+        N.call("ddfan_plan_autotune", self.handle, int(views or 0), int(reps),
+               N.TUNE_USE_CACHE if use_cache else 0, prm, ms)
+        return {"proj_chunk": prm[0], "views_per_wave": prm[1], "slab_tile": prm[2],
+                "back_chunk": prm[3], "proj_ms": ms[0], "back_ms": ms[1]}
+
+    def apply_tuned(self, views=None) -> bool:
+        found = ctypes.c_int()
+        N.call("ddfan_plan_apply_tuned", self.handle, int(views or 0), ctypes.byref(found))
+        return bool(found.value)
+
     # timing and traces
     def timing(self, enable=True):
         N.call("ddfan_plan_timing", self.handle, 1 if enable else 0)
@@ -196,3 +211,19 @@ def plan_footprint(geom: DetectorGeometry, PN, sat_mode="f32", slice_chunk=0, nu
     N.call("ddfan_plan_footprint", ctypes.byref(gs), int(PN), _sat_mode(sat_mode),
            int(slice_chunk), int(num_os), ctypes.byref(b))
     return b.value
+
+
+def tune_save(path: str) -> None:
+    """Writes the process-wide launch-shape table."""
+    N.call("ddfan_tune_save", str(path).encode())
+
+
+def tune_load(path: str) -> int:
+    """Merges a saved launch-shape table; returns its entry count."""
+    n = ctypes.c_int()
+    N.call("ddfan_tune_load", str(path).encode(), ctypes.byref(n))
+    return n.value
+
+
+def tune_clear() -> None:
+    N.call("ddfan_tune_clear")

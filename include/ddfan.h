@@ -154,6 +154,26 @@ int ddfan_plan_query(const ddfan_plan* plan, int* slice_chunk, int* n_chunks, in
 #define DDFAN_PARAM_BACK_CHUNK 4
 int ddfan_plan_set_param(ddfan_plan* plan, int param, int value);
 int ddfan_plan_get_param(const ddfan_plan* plan, int param, int* value);
+/* Autotuner: times the projector over slice chunks x views per wave x slab
+ * tilings and the backprojector over slice chunks, for launches of `views`
+ * views (0: the plan's PN) with this plan's shape, sets the fastest on the
+ * plan and records it in a process-wide table (key: SAT mode, XN, YN, SLN,
+ * DNU, widest uniform chunk, views).  flags DDFAN_TUNE_USE_CACHE takes a
+ * recorded entry without timing.  out_params (may be NULL) receives
+ * {proj chunk, views per wave, slab tile, back chunk}; out_ms (may be NULL)
+ * the projector and backprojector milliseconds per call.  No parameter
+ * changes results beyond the projector's slab-pass summation grouping. */
+#define DDFAN_TUNE_USE_CACHE 1
+int ddfan_plan_autotune(ddfan_plan* plan, int views, int reps, int flags, int* out_params,
+                        double* out_ms);
+/* Applies a recorded entry for launches of `views` views; *found = 0 if none. */
+int ddfan_plan_apply_tuned(ddfan_plan* plan, int views, int* found);
+/* The table as text, one entry per line (load merges; malformed files are
+ * rejected whole). */
+int ddfan_tune_save(const char* path);
+int ddfan_tune_load(const char* path, int* entries);
+int ddfan_tune_clear(void);
+
 /* Device bytes a plan of this shape holds for full-view operators plus an
  * OS-SART run with numOS subsets (0: operators only). */
 int ddfan_plan_footprint(const ddfan_geom* geom, int PN, int sat_mode, int slice_chunk, int numOS,
@@ -184,12 +204,18 @@ int ddfan_ossart(ddfan_plan* plan, const float* d_meas, float* d_x, int numIter,
  * captured hipGraph.  The reference has no batched form; per reconstruction
  * the arithmetic is that of ddfan_ossart (bit-identical results).  Each plan
  * holds one reconstruction's state, so a plan is added once; all plans share
- * the batch's SAT mode and device.  views_per_wave 0 picks the default (4). */
+ * the batch's SAT mode and device; each keeps its own launch parameters. */
 typedef struct ddfan_batch ddfan_batch;
-int ddfan_batch_create(int sat_mode, int views_per_wave, ddfan_batch** out);
+int ddfan_batch_create(int sat_mode, ddfan_batch** out);
 int ddfan_batch_add(ddfan_batch* batch, ddfan_plan* plan, const float* d_meas, float* d_x,
                     int numIter, int numOS, int useFISTA, int* index);
 int ddfan_batch_size(const ddfan_batch* batch, int* n);
+/* Launch grouping: reconstructions share a projector (backprojector) launch
+ * while it holds at most this many 64-lane waves (256-thread blocks); larger
+ * ones get launches of their own.  Grouping never changes results. */
+#define DDFAN_BATCH_PROJ_FILL 1
+#define DDFAN_BATCH_BACK_FILL 2
+int ddfan_batch_set_param(ddfan_batch* batch, int param, long long value);
 /* Runs every reconstruction to completion (flags: DDFAN_OSSART_GRAPH). */
 int ddfan_batch_run(ddfan_batch* batch, int flags);
 int ddfan_batch_destroy(ddfan_batch* batch);

@@ -1,0 +1,48 @@
+"""Autotuner: the chosen launch shape is valid, is recorded and re-applied
+from the table, and changes no result (slice chunks and views per wave are
+bit-identical; a slab tiling regroups the projector's sums, <= 1e-6 relative
+L2 against the untuned call)."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from helpers import small_conf
+
+pytestmark = pytest.mark.gpu
+
+
+def test_autotune_keeps_results_and_caches():
+    import aapm_lowdosectimagerecon_competition_amd as dd
+    from aapm_lowdosectimagerecon_competition_amd import phantoms
+    conf = small_conf(XN=96, YN=96, SLN=16)
+    g = dd.detector_geometry(conf)
+    PN = 60
+    hangs = phantoms.uniform_angles(PN, 16)
+    vol = phantoms.ellipsoid_phantom(96, 96, 16, conf.recon.dx, seed=3)
+    mask = dd.recon_mask(96, 96)
+    dd.tune_clear()
+    print("\n[tune] 96x96x16, 20-view launches", flush=True)
+    with dd.Plan(g, hangs, mask, sat_mode="twofloat") as plan:
+        p0 = plan.proj_host(vol, first=1, stride=3, count=20)
+        b0 = plan.back_host(p0, first=1, stride=3, count=20)
+        t = plan.autotune(views=20, reps=2, use_cache=False)
+        print(f"[tune] chosen {t}", flush=True)
+        assert t["proj_chunk"] in (1, 2, 4, 8, 16) and t["back_chunk"] in (1, 2, 4, 8, 16)
+        assert t["views_per_wave"] in (1, 2, 4) and t["slab_tile"] in (0, 64)
+        assert t["proj_ms"] > 0 and t["back_ms"] > 0
+        assert (plan.proj_chunk, plan.views_per_wave, plan.slab_tile, plan.back_chunk) == \
+            (t["proj_chunk"], t["views_per_wave"], t["slab_tile"], t["back_chunk"])
+        p1 = plan.proj_host(vol, first=1, stride=3, count=20)
+        b1 = plan.back_host(p0, first=1, stride=3, count=20)
+    if t["slab_tile"] == 0:
+        assert np.array_equal(p1, p0)
+    else:
+        assert rel_l2(p1, p0) <= 1e-6
+    assert np.array_equal(b1, b0)
+    with dd.Plan(g, hangs, mask, sat_mode="twofloat") as plan2:
+        assert plan2.apply_tuned(views=20)
+        assert plan2.back_chunk == t["back_chunk"] and plan2.views_per_wave == t["views_per_wave"]
+        assert not plan2.apply_tuned(views=30)
+        again = plan2.autotune(views=20, use_cache=True)
+        assert again["proj_chunk"] == t["proj_chunk"] and again["proj_ms"] == t["proj_ms"]
+    dd.tune_clear()
