@@ -421,6 +421,37 @@ struct BackLaunch {
     BackDev j[kBatch];
 };
 
+// Writes one voxel column's ZT sums: plain (the operator) or the OS-SART
+// update x += sum / colSum with the FISTA extrapolation on a subset's last
+// launch (OSSART_AAPM.m:98-115).
+__device__ __forceinline__ void back_store(const BackDev& J, const GeomDev& g, int x, int y, int z0,
+                                           const float* acc, int ZT) {
+    const size_t pix = (size_t)y * g.XN + x;
+    const size_t o0 = pix * g.SLN + z0;
+    float* vol = J.vol;
+    if (J.epi == BACK_PLAIN) {
+        for (int q = 0; q < ZT; ++q)
+            if (z0 + q < g.SLN) vol[o0 + q] = acc[q];
+        return;
+    }
+    const float w = J.colSum[pix];
+    const bool fista = J.epi == BACK_UPDATE_FISTA;
+    const float mom = fista ? J.momTab[*J.counter] : 0.0f;
+    for (int q = 0; q < ZT; ++q) {
+        if (z0 + q < g.SLN) {
+            const float upd = (w != 0.0f) ? acc[q] / w : acc[q];
+            const float xn = vol[o0 + q] + upd;
+            if (!fista) {
+                vol[o0 + q] = xn;
+            } else {
+                const float xo = xn + mom * (xn - J.xprev[o0 + q]);
+                vol[o0 + q] = xo;
+                J.xprev[o0 + q] = xo;
+            }
+        }
+    }
+}
+
 template <int MODE, int ZT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DDF_BACK_WAVES)))
 void k_back(const BackLaunch L) {
@@ -470,32 +501,7 @@ void k_back(const BackLaunch L) {
         }
     }
     if (!inb) return;
-    const size_t pix = (size_t)y * g.XN + x;
-    const size_t o0 = pix * g.SLN + z0;
-    float* vol = J.vol;
-    if (J.epi == BACK_PLAIN) {
-#pragma unroll
-        for (int q = 0; q < ZT; ++q)
-            if (z0 + q < g.SLN) vol[o0 + q] = acc[q];
-        return;
-    }
-    const float w = J.colSum[pix];
-    const bool fista = J.epi == BACK_UPDATE_FISTA;
-    const float mom = fista ? J.momTab[*J.counter] : 0.0f;
-#pragma unroll
-    for (int q = 0; q < ZT; ++q) {
-        if (z0 + q < g.SLN) {
-            const float upd = (w != 0.0f) ? acc[q] / w : acc[q];
-            const float xn = vol[o0 + q] + upd;
-            if (!fista) {
-                vol[o0 + q] = xn;
-            } else {
-                const float xo = xn + mom * (xn - J.xprev[o0 + q]);
-                vol[o0 + q] = xo;
-                J.xprev[o0 + q] = xo;
-            }
-        }
-    }
+    back_store(J, g, x, y, z0, acc, ZT);
 }
 
 // ------------------------------------------------------------------ index traces
