@@ -236,6 +236,97 @@ def bench_serve(dd, jobs, alone_s, sat, rate=4.0, nreq=24, iters=10, seed=4):
             "failed": st["failed"]}
 
 
+# ---------------------------------------------------------------- helical workload
+def bench_helical(dd, sat, iters, nos=48, SLN2=200, rotations=20, seed=5):
+    """Helical scan (pitch 0.8, 64 rows x 1.09472 mm at the detector, 736
+    channels, 1152 views per rotation) as uint16 counts of a water cylinder
+    with a documented gain of 0.5 at I0 = 1e5, rebinned on the device to
+    200 fan-beam slices in the sinogram layout (counts -> line integrals fused
+    into the gather), then OS-SART on 512x512x200 slices with the rebinned
+    per-slice angles inside the scanned field of view."""
+    import numpy as np
+
+    from aapm_lowdosectimagerecon_competition_amd import rebin
+    W, H, DefTimes, pitch = DNU, 64, 1152, 0.8
+    SD, SO = SDD, SID
+    PerDetW = float(np.float32(2 * SD * np.tan(STEP / 2)))
+    PerDetH = 0.6 * SD / SO
+    TV = DefTimes * rotations
+    I0, gain = 1e5, 0.5
+    # counts of a 200 mm-radius water cylinder, modulated within each rotation
+    beta = (np.arange(W) - (DETC - 1)) * STEP
+    d = SO * np.abs(np.sin(beta))
+    chord = 2.0 * np.sqrt(np.maximum(200.0 ** 2 - d ** 2, 0.0))
+    base = gain * I0 * np.exp(-0.019 * chord)  # per channel
+    rng = np.random.default_rng(seed)
+    log(f"[bench] helical: {TV} raw views x {W} x {H} uint16")
+    mod = 1.0 + 0.05 * np.sin(6 * np.pi * np.arange(DefTimes) / DefTimes)  # per rotation
+    c = base[None, :, None] * mod[:, None, None] * rng.uniform(0.98, 1.02, (1, 1, H))
+    one = np.clip(np.rint(c), 1, 65535).astype(np.uint16)
+    raw = np.empty((TV, W, H), np.uint16)  # raw[(view*W + ch)*H + row]
+    for r0 in range(0, TV, DefTimes):
+        raw[r0:r0 + DefTimes] = one
+    del c, one
+    h = pitch * PerDetH * H * SO / SD
+    deltaZ = h / DefTimes
+    zPos = np.linspace(h, deltaZ * TV - h, SLN2).astype(np.float32)
+    d_raw = dd.DeviceBuffer.from_array(raw)
+    del raw
+    d_prj = dd.DeviceBuffer(4 * DefTimes * W * SLN2)
+    d_views = dd.DeviceBuffer(4 * DefTimes * SLN2)
+
+    def rebin_call():
+        rebin.helical_to_fan_device(d_raw, 1, gain * I0, 1, TV, zPos, SD, SO, 0.0, W, H, PerDetW,
+                                    PerDetH, DefTimes, DETC, pitch, d_prj, d_views)
+
+    rebin_call()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        rebin_call()
+    rebin_s = (time.perf_counter() - t0) / reps
+    samples = DefTimes * W * SLN2
+    out = {"raw": f"{TV}x{W}x{H} uint16 (gain {gain}, I0 {I0:g})",
+           "rebinned": f"{SLN2} slices x {DefTimes} views x {W} channels",
+           "rebin_ms_per_call": rebin_s * 1e3,
+           "rebin_GBs_algorithmic": samples * 8 / rebin_s / 1e9}
+    log(f"[bench] helical rebin {out}")
+    views = d_views.to_numpy((DefTimes, SLN2))
+    XN2 = 512
+    conf = dd.Conf.make(XN=XN2, YN=XN2, SLN=SLN2, dx=500.0 / XN2, sid=SID, sdd=SDD, DNU=DNU,
+                        detCellWidth=PerDetW, detCntIdx=DETC, PN=DefTimes)
+    g = dd.detector_geometry(conf)
+    # Mask = the scanned field of view (radius SO*sin(half fan angle)).  The
+    # reference's mask (radius 1.3x the half-width, reconscript.m:28-38) keeps
+    # image corners no ray of the fan crosses; with 24-view subsets OS-SART
+    # diverges there within a few iterations.
+    fov = SO * np.sin((DNU / 2) * STEP)
+    yy, xx = np.mgrid[0:XN2, 0:XN2]
+    mask = (np.hypot(xx - (XN2 - 1) / 2, yy - (XN2 - 1) / 2) * g.dx < 0.98 * fov).astype(np.uint8)
+    with dd.Plan(g, views, mask, sat_mode=sat) as plan, \
+            dd.DeviceBuffer(4 * XN2 * XN2 * SLN2) as d_x:
+        plan.autotune(views=DefTimes // nos, reps=2)
+        d_x.upload(np.zeros((XN2, XN2, SLN2), np.float32))
+        plan_chunk = plan.slice_chunk
+        log(f"[bench] helical: OS-SART {iters} it x {nos} subsets, slice chunk {plan_chunk}")
+        t0 = time.perf_counter()
+        plan.ossart(d_prj, d_x, iters, nos, 1, graph=True)
+        os_s = time.perf_counter() - t0
+        x = d_x.to_numpy((XN2, XN2, SLN2))
+    vv = 2.0 * XN2 * XN2 * SLN2 * DefTimes * iters
+    out.update({"ossart_iters": iters, "ossart_subsets": nos, "ossart_s": os_s,
+                "ossart_voxel_views_per_s": vv / os_s, "slice_chunk": plan_chunk,
+                "recon_center_mu_per_mm": float(x[XN2 // 2 - 8:XN2 // 2 + 8,
+                                                  XN2 // 2 - 8:XN2 // 2 + 8].mean()) / g.dx,
+                "recon_max_abs": float(np.abs(x).max()),
+                "recon_nonfinite": int(np.count_nonzero(~np.isfinite(x))),
+                "mask": f"scanned field of view, radius {0.98 * fov:.1f} mm"})
+    d_raw.free()
+    d_prj.free()
+    d_views.free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -247,6 +338,11 @@ def main():
     ap.add_argument("--ossart-iters", type=int, default=2)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched and served mix")
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes, no autotuning")
+    ap.add_argument("--helical-iters", type=int, default=20,
+                    help="OS-SART iterations of the helical workload (0 skips it)")
+    ap.add_argument("--shapes", help="launch-shape table to apply instead of autotuning "
+                                     "(written by --save-shapes)")
+    ap.add_argument("--save-shapes", help="write the autotuned launch-shape table here")
     ap.add_argument("--profile-pass", action="store_true",
                     help="one projector + backprojector call, then one 256 MiB device-to-device "
                          "copy whose counter readings calibrate FETCH_SIZE / WRITE_SIZE; "
@@ -275,7 +371,20 @@ def main():
 
     log(f"[bench] plan sat={args.sat}")
     plan = dd.Plan(g, hangs, mask, sat_mode=args.sat)
-    tuned = plan.autotune(views=PN, reps=2) if not args.no_tune else None
+    if args.shapes:
+        dd.tune_load(args.shapes)
+
+    def shape_for(views):
+        """Launch shape for calls of `views` views: from --shapes, else autotuned."""
+        if args.no_tune:
+            return None
+        if args.shapes:
+            if not plan.apply_tuned(views=views):
+                raise RuntimeError(f"{args.shapes} has no entry for {views}-view calls")
+            return {"from": args.shapes}
+        return plan.autotune(views=views, reps=2)
+
+    tuned = shape_for(PN)
     log(f"[bench] launch shape {tuned}")
     d_vol = dd.DeviceBuffer.from_array(vol)
     d_prj = dd.DeviceBuffer(4 * PN * DNU * SLN)
@@ -338,14 +447,31 @@ def main():
                                        "autotuned": tuned is not None}},
            "roofline": roof, "kernels_ms": kt}
 
+    # the drop-in entry points on host buffers (plan, copies in and out, free
+    # per call, as the reference's MEX calls do): PCIe-inclusive, not `value`
+    from aapm_lowdosectimagerecon_competition_amd import _native as N
+    hprj = np.empty((PN, DNU, SLN), np.float32)
+    hbp = np.empty((YN, XN, SLN), np.float32)
+    t0 = time.perf_counter()
+    N.call("ddfan_proj", g.x0, g.y0, DNU, N.ptr(g.xds), N.ptr(g.yds), g.imgXCenter, g.imgYCenter,
+           N.ptr(hangs), PN, XN, YN, SLN, N.ptr(vol), N.ptr(hprj), g.dx, N.ptr(mask), None, 0)
+    t1 = time.perf_counter()
+    N.call("ddfan_back", g.x0, g.y0, DNU, N.ptr(g.xds), N.ptr(g.yds), g.detCntIdx, g.imgXCenter,
+           g.imgYCenter, N.ptr(hangs), PN, XN, YN, SLN, N.ptr(hbp), N.ptr(hprj), g.dx,
+           N.ptr(mask), None, 0)
+    t2 = time.perf_counter()
+    out["dropin_host_buffers"] = {"proj_ms": (t1 - t0) * 1e3, "back_ms": (t2 - t1) * 1e3,
+                                  "voxel_views_per_s": 2.0 * vv / (t2 - t0),
+                                  "sat": "f32 (drop-in default)"}
+    log(f"[bench] drop-in host buffers {out['dropin_host_buffers']}")
+
     if not args.no_ossart:
         # device-resident fused + graph OS-SART vs the plain per-subset host form
         nos, it = 24, args.ossart_iters
         meas = np.empty((PN, DNU, SLN), np.float32)
         d_prj.download(meas)
         x0 = np.zeros((YN, XN, SLN), np.float32)
-        if not args.no_tune:
-            plan.autotune(views=PN // nos, reps=2)
+        shape_for(PN // nos)
         log(f"[bench] ossart device {it} it x {nos} subsets")
         d_meas = dd.DeviceBuffer.from_array(meas)
         d_x = dd.DeviceBuffer.from_array(x0)
@@ -383,6 +509,13 @@ def main():
         for j in jobs:
             j.close()
         del jobs
+
+    if args.helical_iters > 0 and not args.no_batch:
+        out["helical"] = bench_helical(dd, args.sat, args.helical_iters)
+        log(f"[bench] helical {out['helical']}")
+
+    if args.save_shapes:
+        dd.tune_save(args.save_shapes)
 
     if not args.no_cpu:
         log("[bench] cpu baseline")
