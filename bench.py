@@ -319,6 +319,7 @@ def bench_helical(dd, sat, iters, nos=48, SLN2=200, rotations=20, seed=5):
                 "recon_center_mu_per_mm": float(x[XN2 // 2 - 8:XN2 // 2 + 8,
                                                   XN2 // 2 - 8:XN2 // 2 + 8].mean()) / g.dx,
                 "recon_max_abs": float(np.abs(x).max()),
+                "recon_p99_abs": float(np.percentile(np.abs(x), 99)),
                 "recon_nonfinite": int(np.count_nonzero(~np.isfinite(x))),
                 "mask": f"scanned field of view, radius {0.98 * fov:.1f} mm"})
     d_raw.free()
