@@ -154,13 +154,22 @@ class DeviceJob:
         self.x.free()
 
 
-def bench_batch(dd, sat, n=16, iters=10, seed=3):
-    """16 uneven reconstructions: one at a time (device OS-SART, graph) against
+def small_specs(n, seed):
+    """Small reconstructions (256x256 over 500 mm, 8-16 slices, 720 views) that
+    leave most of the GPU idle when run one at a time."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    return [{"XN": 256, "SLN": int(rng.integers(8, 17)), "PN": 720,
+             "I0": float(np.exp(rng.uniform(np.log(5e4), np.log(5e5))))} for _ in range(n)]
+
+
+def bench_batch(dd, sat, n=16, iters=10, seed=3, specs=None):
+    """Uneven reconstructions: one at a time (device OS-SART, graph) against
     one batch (shared launches, graph per batch iteration); results must agree
     bit for bit."""
     import numpy as np
     log(f"[bench] batch: generating {n} jobs")
-    specs = mix_specs(n, seed)
+    specs = specs or mix_specs(n, seed)
     jobs = [DeviceJob(dd, sp, sat, 1000 + i, iters) for i, sp in enumerate(specs)]
     vv = sum(2.0 * j.vv * j.iters for j in jobs)
     alone_s, ref = [], []
@@ -503,6 +512,11 @@ def main():
         d_prj.free()
         d_bp.free()
         plan.close()
+        small, _, out["batch_small"] = bench_batch(dd, args.sat, specs=small_specs(16, 7))
+        log(f"[bench] batch of small jobs {out['batch_small']}")
+        for j in small:
+            j.close()
+        del small
         jobs, alone_s, out["batch"] = bench_batch(dd, args.sat)
         log(f"[bench] batch {out['batch']}")
         out["serve"] = bench_serve(dd, jobs, alone_s, args.sat)
