@@ -247,7 +247,10 @@ int ddfan_server_wait(ddfan_server* server, long long id, float* x, double* queu
 int ddfan_server_stats(ddfan_server* server, long long* completed, long long* failed,
                        int* running, long long* waiting, size_t* used_bytes,
                        double* voxel_views);
-/* Finishes every queued request, then stops the worker. */
+/* Finishes every queued request, then stops the worker and frees the server.
+ * Submit, poll, wait and stats may be called from any number of threads
+ * concurrently; destroy must be the last call and must not overlap any other
+ * call on the same server. */
 int ddfan_server_destroy(ddfan_server* server);
 
 /* The reference's plain form on host buffers: every subset calls the

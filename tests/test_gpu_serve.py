@@ -90,3 +90,38 @@ def test_failed_request_fails_alone():
         st = S.stats()
     assert st["failed"] == 1 and st["completed"] == 2
     assert np.array_equal(ra.x, ref) and np.array_equal(rc.x, ref)
+
+
+def test_concurrent_clients():
+    """Four client threads submit and collect interleaved requests at once
+    (the C calls release the GIL); every result equals its solo run."""
+    import threading
+
+    import aapm_lowdosectimagerecon_competition_amd as dd
+    specs = [JOBS[i % 5] for i in range(8)]
+    jobs = [make_job(s, 120 + i) for i, s in enumerate(specs)]
+    ref = [alone(dd, J) for J in jobs]
+    print(f"\n[serve] {len(jobs)} requests from 4 client threads", flush=True)
+    got, errors = [None] * len(jobs), []
+    with dd.Server(max_slots=3) as S:
+        def client(idx):
+            try:
+                for i in idx:
+                    J = jobs[i]
+                    rid = S.submit(J["g"], J["hangs"], J["meas"], J["it"], J["nos"], J["fista"],
+                                   mask=J["mask"])
+                    got[i] = S.wait(rid).x
+            except Exception as e:  # pragma: no cover - reported below
+                errors.append(repr(e))
+        threads = [threading.Thread(target=client, args=(range(t, len(jobs), 4),))
+                   for t in range(4)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join(timeout=120)
+        assert not any(t.is_alive() for t in threads)
+        st = S.stats()
+    assert not errors, errors
+    assert st["completed"] == len(jobs) and st["failed"] == 0
+    for i in range(len(jobs)):
+        assert np.array_equal(got[i], ref[i]), i
