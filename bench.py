@@ -199,7 +199,7 @@ def bench_batch(dd, sat, n=16, iters=10, seed=3, specs=None):
     return jobs, alone_s, out
 
 
-def bench_serve(dd, jobs, alone_s, sat, rate=4.0, nreq=24, iters=10, seed=4):
+def bench_serve(dd, jobs, alone_s, sat, rate=4.0, nreq=24, iters=10, seed=4, slots=4):
     """Poisson arrivals (rate per second) of requests drawn from the batch mix
     (plus 1% of 768x768x256, 2304-view requests); each runs `iters` OS-SART
     iterations.  Latency is submission to result on the host."""
@@ -222,7 +222,7 @@ def bench_serve(dd, jobs, alone_s, sat, rate=4.0, nreq=24, iters=10, seed=4):
     known = [per_job[id(j)] for j in picks if id(j) in per_job]
     offered = rate * float(np.mean(known)) if known else None
     log(f"[bench] serve: {nreq} requests at {rate}/s")
-    with dd.Server(hbm_budget=0, max_slots=16, sat_mode=sat) as S:
+    with dd.Server(hbm_budget=0, max_slots=slots, sat_mode=sat) as S:
         t0 = time.perf_counter()
         ids = []
         for i, j in enumerate(picks):
@@ -238,7 +238,7 @@ def bench_serve(dd, jobs, alone_s, sat, rate=4.0, nreq=24, iters=10, seed=4):
         big.x.free()
     lat = np.array([r.latency_ms for r in res])
     q = np.array([r.queue_ms for r in res])
-    return {"requests": nreq, "rate_per_s": rate, "iters": iters,
+    return {"requests": nreq, "rate_per_s": rate, "iters": iters, "slots": slots,
             "offered_load": offered, "p50_ms": float(np.percentile(lat, 50)),
             "p99_ms": float(np.percentile(lat, 99)), "mean_queue_ms": float(q.mean()),
             "wall_s": t1 - t0, "voxel_views_per_s": st["voxel_views"] / (t1 - t0),
