@@ -160,3 +160,26 @@ def test_launch_shape_table_round_trip(tmp_path):
     with pytest.raises(N.DDFanError):
         dd.tune_load(str(tmp_path / "missing.txt"))
     dd.tune_clear()
+
+
+def test_header_is_c_and_host_entry_points_work_from_c(tmp_path):
+    """include/ddfan.h compiles as C99 (-Wall -Wextra -Werror) and a C program
+    linked against libddfan.so gets the same host tables as Python."""
+    import subprocess
+
+    import aapm_lowdosectimagerecon_competition_amd as dd
+    from aapm_lowdosectimagerecon_competition_amd import _native as N
+    exe = tmp_path / "abi_c_check"
+    pkg = os.path.dirname(N.LIB_PATH)
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-Werror",
+                    f"-I{os.path.join(ROOT, 'include')}",
+                    os.path.join(ROOT, "tests", "native", "abi_c_check.c"), f"-L{pkg}", "-lddfan",
+                    f"-Wl,-rpath,{pkg}", "-o", str(exe)], check=True, capture_output=True, text=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = dict(l.split(" ", 1) for l in r.stdout.strip().splitlines())
+    xds, yds = dd.detector_arc(8, 4.5, 1.2, 1085.6, 595.0)
+    assert lines["arc"] == f"{float(xds[0]):.6f} {float(yds[7]):.6f}"
+    assert int(lines["mask"]) == int(dd.recon_mask(6, 5).sum())
+    assert lines["subset"] == "3 2 7 12"
+    assert "numOS" in lines["einval"] or "subset" in lines["einval"]
