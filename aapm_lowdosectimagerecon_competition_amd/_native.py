@@ -116,8 +116,8 @@ _SIGS = {
     "ddfan_tune_save": [ctypes.c_char_p],
     "ddfan_tune_load": [ctypes.c_char_p, _ip],
     "ddfan_tune_clear": [],
-    "ddfan_helical_to_fan_dev": [_p, _p, _p, _i, _f, _i, _ll, _p, _i, _f, _f, _f, _i, _i, _f, _f,
-                                 _i, _f, _f, _p, _p],
+    "ddfan_helical_to_fan_dev": [_p, _p, _p, _i, _i, _f, _i, _ll, _p, _i, _f, _f, _f, _i, _i, _f,
+                                 _f, _i, _f, _f, _p, _p],
 }
 _RESTYPES = {"ddfan_status_string": ctypes.c_char_p, "ddfan_last_error": ctypes.c_char_p}
 

@@ -515,14 +515,17 @@ int ddfan_back(float x0, float y0, int DNU, const float* xds, const float* yds, 
 // ------------------------------------------------------------ rebin
 namespace {
 int rebin_common(float* Proj, float* Views, const void* raw, bool rawOnDevice, int rawKind,
-                 float countScale, int layout, long long TotalViews, const float* zPos, int SLN,
+                 int rawOrder, float countScale, int layout, long long TotalViews,
+                 const float* zPos, int SLN,
                  float SD, float SO, float BVAngle, int W, int H, float PerDetW, float PerDetH,
                  int DefTimes, float DetCenterW, float pitch, bool outOnDevice,
                  unsigned char* d_cut, int32_t* d_idv) {
     CHECK(Proj && Views && raw && (zPos || SLN == 0), DDFAN_EINVAL, "NULL buffer");
     CHECK(SLN >= 0 && W >= 1 && W <= 2048 && H >= 2 && DefTimes >= 1 && TotalViews >= 0,
           DDFAN_EINVAL, "bad rebin shape");
-    CHECK(rawKind == 0 || rawKind == 1, DDFAN_EINVAL, "raw_kind must be 0 or 1");
+    CHECK(rawKind >= 0 && rawKind <= 2, DDFAN_EINVAL, "raw_kind must be 0, 1 or 2");
+    CHECK(rawOrder == 0 || (rawOrder == 1 && rawOnDevice), DDFAN_EINVAL,
+          "raw_order must be 0, or 1 with device-resident raw data");
     CHECK(layout == 0 || layout == 1, DDFAN_EINVAL, "layout must be 0 or 1");
     CHECK(isfinite(BVAngle) && fabsf(BVAngle) < 1.0e5f && isfinite(SD) && isfinite(SO) &&
               SD > 0.0f && SO > 0.0f && PerDetH > 0.0f && PerDetW > 0.0f && pitch > 0.0f,
@@ -571,6 +574,7 @@ int rebin_common(float* Proj, float* Views, const void* raw, bool rawOnDevice, i
         TRY(upload(dstart, start.data(), sizeof(int) * SLN, st));
         TRY(upload(dsm, SM.data(), sizeof(float) * W, st));
         const size_t esz = rawKind == 0 ? 4 : 2;
+        p.totalViews = TotalViews;
         const size_t perView = (size_t)W * H * esz;
         const void* rawDev = raw;
         if (vmax < vmin) {  // no slice reads raw data
@@ -597,7 +601,7 @@ int rebin_common(float* Proj, float* Views, const void* raw, bool rawOnDevice, i
             outP = dP.as<float>();
             outV = dV.as<float>();
         }
-        HIPCALL(launch_rebin(p, rawKind, layout, rawDev, countScale, dz.as<float>(),
+        HIPCALL(launch_rebin(p, rawKind, rawOrder, layout, rawDev, countScale, dz.as<float>(),
                              dstart.as<int>(), dsm.as<float>(), outP, outV, d_cut, d_idv, st));
         if (!outOnDevice) {
             HIPCALL(hipMemcpyAsync(Proj, outP, sizeof(float) * nOut, hipMemcpyDeviceToHost, st));
@@ -615,19 +619,20 @@ int ddfan_helical_to_fan(float* Proj, float* Views, const float* proj, long long
                          const float* zPos, int SLN, float SD, float SO, float BVAngle,
                          int DetWidth, int DetHeight, float PerDetW, float PerDetH, int DefTimes,
                          float DetCenterW, float SpiralPitchFactor) {
-    return rebin_common(Proj, Views, proj, false, 0, 1.0f, 0, TotalViews, zPos, SLN, SD, SO,
+    return rebin_common(Proj, Views, proj, false, 0, 0, 1.0f, 0, TotalViews, zPos, SLN, SD, SO,
                         BVAngle, DetWidth, DetHeight, PerDetW, PerDetH, DefTimes, DetCenterW,
                         SpiralPitchFactor, false, nullptr, nullptr);
 }
 
 int ddfan_helical_to_fan_dev(float* d_Proj, float* d_Views, const void* d_raw, int raw_kind,
-                             float count_scale, int layout, long long TotalViews,
+                             int raw_order, float count_scale, int layout, long long TotalViews,
                              const float* zPos, int SLN, float SD, float SO, float BVAngle,
                              int DetWidth, int DetHeight, float PerDetW, float PerDetH,
                              int DefTimes, float DetCenterW, float SpiralPitchFactor,
                              unsigned char* d_cut, int32_t* d_idv) {
     CHECK(raw_kind == 0 || count_scale > 0.0f, DDFAN_EINVAL, "count_scale must be positive");
-    return rebin_common(d_Proj, d_Views, d_raw, true, raw_kind, count_scale, layout, TotalViews,
+    return rebin_common(d_Proj, d_Views, d_raw, true, raw_kind, raw_order, count_scale, layout,
+                        TotalViews,
                         zPos, SLN, SD, SO, BVAngle, DetWidth, DetHeight, PerDetW, PerDetH,
                         DefTimes, DetCenterW, SpiralPitchFactor, true, d_cut, d_idv);
 }

@@ -131,16 +131,21 @@ hipError_t launch_back_trace(const GeomDev& g, const float2* cossin, Subset sub,
 struct RebinDev {
     float SD, SO, BVAngle, PerDetH, DetCenterHreal, delta, deltaZ, twopi;
     int W, H, DefTimes, SLN;
-    long long viewOffset;  // raw view index (0-based) held at raw[0]
+    long long viewOffset;  // raw view index (0-based) held at raw[0] (raw order 0)
+    long long totalViews;  // raw views held (raw order 1 counts them from the end)
 };
 
 // rawKind 0: float raw line integrals; 1: uint16 counts converted to
-// -log(max(c,1)/(gain*I0)) on the fly.  outLayout 0: reference layout
-// Proj[(s*DefTimes+v)*W+j], Views[s*DefTimes+v]; 1: sinogram layout
+// -log(max(c,1)/(gain*I0)) on the fly; 2: uint16 line integrals times
+// countScale.  rawOrder 0: raw[(view*W+ch)*H+row] (the rebin's own input,
+// HelicalToFanFunc_mex.cpp:45,74); 1: per-view detector frames
+// raw[(f*H+row)*W+ch] in reverse view order, f = TotalViews-1-view (the
+// stored order readProj.m:43-54 permutes and flips).  outLayout 0: reference
+// layout Proj[(s*DefTimes+v)*W+j], Views[s*DefTimes+v]; 1: sinogram layout
 // prj[(v*W+j)*SLN+s], hangs[v*SLN+s].
-hipError_t launch_rebin(const RebinDev& p, int rawKind, int outLayout, const void* raw,
-                        float countScale, const float* zPos, const int* start, const float* SM,
-                        float* Proj, float* Views, uint8_t* cut, int32_t* idvTrace,
-                        hipStream_t st);
+hipError_t launch_rebin(const RebinDev& p, int rawKind, int rawOrder, int outLayout,
+                        const void* raw, float countScale, const float* zPos, const int* start,
+                        const float* SM, float* Proj, float* Views, uint8_t* cut,
+                        int32_t* idvTrace, hipStream_t st);
 
 }  // namespace ddf

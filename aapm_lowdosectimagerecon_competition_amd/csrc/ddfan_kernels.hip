@@ -571,9 +571,9 @@ __global__ __launch_bounds__(256) void k_fill(float* __restrict__ p, float v, si
 }
 
 // ------------------------------------------------------------------ rebin
-template <int RAW, int LAYOUT>
+template <int RAW, int LAYOUT, int ORDER>
 __global__ __launch_bounds__(256) void k_rebin(const RebinDev p, const void* __restrict__ raw,
-                                               const float countScaleLog,
+                                               const float rawScale,
                                                const float* __restrict__ zPos,
                                                const int* __restrict__ start,
                                                const float* __restrict__ SM,
@@ -596,7 +596,10 @@ __global__ __launch_bounds__(256) void k_rebin(const RebinDev p, const void* __r
         const float lambda = (float)(i - 1) * p.delta + p.BVAngle;
         angle = mod_f(lambda, p.twopi);
     }
-    const size_t viewBase = valid ? (size_t)(i - 1 - p.viewOffset) * p.W * p.H : 0;
+    // ORDER 0: raw[(view*W + ch)*H + row]; ORDER 1: frames raw[(f*H + row)*W + ch]
+    // stored in reverse view order, f = TotalViews - view (readProj.m:43-54)
+    const long long rv = ORDER == 0 ? i - 1 - p.viewOffset : p.totalViews - i;
+    const size_t viewBase = valid ? (size_t)rv * p.W * p.H : 0;
     float vals[MAXC];
     int myCut = 0;
 #pragma unroll
@@ -612,16 +615,22 @@ __global__ __launch_bounds__(256) void k_rebin(const RebinDev p, const void* __r
                 if (idv >= 1 && idv <= p.H - 1) {
                     const float t = dv - (float)idv;
                     const float cosA = p.SD / sqrtf(p.SD * p.SD + v * v);
-                    const size_t o = viewBase + (size_t)jj * p.H + (idv - 1);
+                    const size_t o = ORDER == 0 ? viewBase + (size_t)jj * p.H + (idv - 1)
+                                                : viewBase + (size_t)(idv - 1) * p.W + jj;
+                    const size_t o1 = ORDER == 0 ? o + 1 : o + p.W;  // next detector row
                     float d0, d1;
                     if constexpr (RAW == 0) {
                         const float* r = reinterpret_cast<const float*>(raw);
                         d0 = r[o];
-                        d1 = r[o + 1];
-                    } else {
+                        d1 = r[o1];
+                    } else if constexpr (RAW == 1) {  // counts -> log(scale / max(c, 1))
                         const uint16_t* r = reinterpret_cast<const uint16_t*>(raw);
-                        d0 = countScaleLog - logf(fmaxf((float)r[o], 1.0f));
-                        d1 = countScaleLog - logf(fmaxf((float)r[o + 1], 1.0f));
+                        d0 = rawScale - logf(fmaxf((float)r[o], 1.0f));
+                        d1 = rawScale - logf(fmaxf((float)r[o1], 1.0f));
+                    } else {  // stored line integrals times a scale (1: used unscaled)
+                        const uint16_t* r = reinterpret_cast<const uint16_t*>(raw);
+                        d0 = (float)r[o] * rawScale;
+                        d1 = (float)r[o1] * rawScale;
                     }
                     vals[c] = cosA * (d0 * (1.0f - t) + d1 * t);
                     mk = 1;
@@ -993,23 +1002,39 @@ hipError_t launch_fill(float* p, float v, size_t n, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_rebin(const RebinDev& p, int rawKind, int outLayout, const void* raw,
-                        float countScale, const float* zPos, const int* start, const float* SM,
-                        float* Proj, float* Views, uint8_t* cut, int32_t* idvTrace,
-                        hipStream_t st) {
+hipError_t launch_rebin(const RebinDev& p, int rawKind, int rawOrder, int outLayout,
+                        const void* raw, float countScale, const float* zPos, const int* start,
+                        const float* SM, float* Proj, float* Views, uint8_t* cut,
+                        int32_t* idvTrace, hipStream_t st) {
     if (p.W > 8 * 256) return hipErrorInvalidValue;
     const long long nb = (long long)p.SLN * p.DefTimes;
     if (nb == 0) return hipSuccess;
     if (nb > 0x7fffffffLL) return hipErrorInvalidValue;
-    const float csl = rawKind == 1 ? logf(countScale) : 0.0f;
-#define DDF_REBIN(RK, LY)                                                                     \
-    k_rebin<RK, LY><<<(unsigned)nb, 256, 0, st>>>(p, raw, csl, zPos, start, SM, Proj, Views, cut, \
-                                                  idvTrace)
-    if (rawKind == 0 && outLayout == 0) DDF_REBIN(0, 0);
-    else if (rawKind == 0 && outLayout == 1) DDF_REBIN(0, 1);
-    else if (rawKind == 1 && outLayout == 0) DDF_REBIN(1, 0);
-    else if (rawKind == 1 && outLayout == 1) DDF_REBIN(1, 1);
-    else return hipErrorInvalidValue;
+    if (rawOrder == 1 && p.viewOffset != 0) return hipErrorInvalidValue;
+    const float scale = rawKind == 1 ? logf(countScale) : countScale;
+#define DDF_REBIN(RK, LY, OR)                                                                   \
+    k_rebin<RK, LY, OR><<<(unsigned)nb, 256, 0, st>>>(p, raw, scale, zPos, start, SM, Proj, Views, \
+                                                      cut, idvTrace)
+#define DDF_REBIN_O(RK, LY)                  \
+    do {                                     \
+        if (rawOrder == 0)                   \
+            DDF_REBIN(RK, LY, 0);            \
+        else if (rawOrder == 1)              \
+            DDF_REBIN(RK, LY, 1);            \
+        else                                 \
+            return hipErrorInvalidValue;     \
+    } while (0)
+    if (outLayout != 0 && outLayout != 1) return hipErrorInvalidValue;
+    if (rawKind == 0) {
+        if (outLayout == 0) DDF_REBIN_O(0, 0); else DDF_REBIN_O(0, 1);
+    } else if (rawKind == 1) {
+        if (outLayout == 0) DDF_REBIN_O(1, 0); else DDF_REBIN_O(1, 1);
+    } else if (rawKind == 2) {
+        if (outLayout == 0) DDF_REBIN_O(2, 0); else DDF_REBIN_O(2, 1);
+    } else {
+        return hipErrorInvalidValue;
+    }
+#undef DDF_REBIN_O
 #undef DDF_REBIN
     return hipGetLastError();
 }

@@ -78,13 +78,22 @@ def rebin_starts(zPos, SD, SO, DetHeight, PerDetW, PerDetH, DefTimes, pitch) -> 
     return out
 
 
+RAW_FLOAT, RAW_COUNTS_U16, RAW_LINE_INTEGRALS_U16 = 0, 1, 2
+ORDER_REBIN, ORDER_FRAMES_REVERSED = 0, 1
+
+
 def helical_to_fan_device(d_raw, raw_kind, count_scale, layout, total_views, zPos, SD, SO,
                           BVAngle, DetWidth, DetHeight, PerDetW, PerDetH, DefTimes, DetCenterW,
-                          pitch, d_out, d_views, d_cut=None, d_idv=None):
-    """Device-resident rebin (raw and outputs are device buffers)."""
+                          pitch, d_out, d_views, d_cut=None, d_idv=None, raw_order=ORDER_REBIN):
+    """Device-resident rebin (raw and outputs are device buffers).  raw_kind:
+    RAW_FLOAT, RAW_COUNTS_U16 (log(count_scale/max(c,1))) or
+    RAW_LINE_INTEGRALS_U16 (value*count_scale); raw_order: ORDER_REBIN
+    (view, channel, row) or ORDER_FRAMES_REVERSED (stored detector frames
+    (view, row, channel), last view first, as readProj.m reads them)."""
     z = np.ascontiguousarray(zPos, np.float32)
     N.call("ddfan_helical_to_fan_dev", N.ptr(d_out), N.ptr(d_views), N.ptr(d_raw), int(raw_kind),
-           float(count_scale), int(layout), int(total_views), N.ptr(z), int(z.size), float(SD),
+           int(raw_order), float(count_scale), int(layout), int(total_views), N.ptr(z),
+           int(z.size), float(SD),
            float(SO), float(BVAngle), int(DetWidth), int(DetHeight), float(PerDetW),
            float(PerDetH), int(DefTimes), float(DetCenterW), float(pitch), N.ptr(d_cut),
            N.ptr(d_idv))

@@ -278,13 +278,19 @@ int ddfan_plan_trace_proj(ddfan_plan* plan, int first, int stride, int count, in
 int ddfan_plan_trace_back(ddfan_plan* plan, int first, int stride, int count, int32_t* d_trace);
 
 /* Device-resident rebin.  raw_kind 0: d_raw float line integrals; 1: d_raw
- * uint16 counts, converted to log(count_scale / max(count, 1)) on the fly.
+ * uint16 counts, converted to log(count_scale / max(count, 1)) on the fly;
+ * 2: d_raw uint16 line integrals times count_scale (1: used unscaled, as
+ * readProj.m:47 does).  raw_order 0: raw[(view*DetWidth+ch)*DetHeight+row]
+ * (HelicalToFan's input); 1: the detector frames as stored,
+ * raw[(f*DetHeight+row)*DetWidth+ch] with views in reverse order
+ * (f = TotalViews-1-view), which readProj.m:52-53 permutes and flips -- the
+ * gather reads them in place.
  * layout 0: reference layout (Proj[(s*DefTimes+v)*W+ch], Views[s*DefTimes+v]);
  * layout 1: sinogram layout (prj[(v*W+ch)*SLN+s], hangs[v*SLN+s]).
  * d_cut (optional) [SLN][DefTimes] receives 1 for rows zeroed as cut views;
  * d_idv (optional, layout-independent) int32 [SLN][DefTimes][W] source rows. */
 int ddfan_helical_to_fan_dev(float* d_Proj, float* d_Views, const void* d_raw, int raw_kind,
-                             float count_scale, int layout, long long TotalViews,
+                             int raw_order, float count_scale, int layout, long long TotalViews,
                              const float* zPos, int SLN, float SD, float SO, float BVAngle,
                              int DetWidth, int DetHeight, float PerDetW, float PerDetH,
                              int DefTimes, float DetCenterW, float SpiralPitchFactor,

@@ -183,3 +183,20 @@ def test_header_is_c_and_host_entry_points_work_from_c(tmp_path):
     assert int(lines["mask"]) == int(dd.recon_mask(6, 5).sum())
     assert lines["subset"] == "3 2 7 12"
     assert "numOS" in lines["einval"] or "subset" in lines["einval"]
+
+
+def test_rebin_raw_order_needs_device_data():
+    """The stored-frame order is only accepted for device-resident raw data,
+    and unknown kinds/orders are rejected before any device work."""
+    import ctypes
+
+    from aapm_lowdosectimagerecon_competition_amd import _native as N
+    L = N.lib()
+    z = np.zeros(1, np.float32)
+    buf = ctypes.c_void_p(16)  # never dereferenced: argument checks fail first
+    for kind, order in ((3, 0), (0, 2)):
+        st = L.ddfan_helical_to_fan_dev(buf, buf, buf, kind, order, F32(1.0), 0, 100,
+                                        z.ctypes.data_as(ctypes.c_void_p), 1, F32(1085.6),
+                                        F32(595.0), F32(0.0), 16, 8, F32(1.0), F32(1.0), 48,
+                                        F32(8.5), F32(0.8), None, None)
+        assert st == N.DDFAN_EINVAL, (kind, order)
