@@ -266,11 +266,14 @@ int ddfan_plan_create(const ddfan_geom* G, const float* hangs, int PN, const uns
         P->ZTmax = choose_slice_chunk(hangs, PN, SLN, slice_chunk);
         // Launch-shape defaults measured on MI355X at 512x512x64, 2304 views:
         // the projector runs best on chunks of <= 8 slices with 4 views per
-        // wave in one slab pass, the backprojector on the widest chunk.
+        // wave in one slab pass, the backprojector on the widest chunk with
+        // its detector windows staged through LDS one view at a time (narrow
+        // chunks stage 4 / chunk views per group, so each wave copies a row).
         P->ZT = std::min(P->ZTmax, 8);
         P->nzc = (SLN + P->ZT - 1) / P->ZT;
         P->ZTb = P->ZTmax;
         P->nzcb = (SLN + P->ZTb - 1) / P->ZTb;
+        P->backStage = std::max(1, 4 / P->ZTb);
         P->viewsPerWave = 4;
         P->slabTile = 0;
         P->h_hangs.assign(hangs, hangs + (size_t)PN * SLN);
@@ -341,9 +344,17 @@ int ddfan_plan_set_param(ddfan_plan* P, int param, int value) {
             } else {
                 P->ZTb = value;
                 P->nzcb = n;
+                while (P->backStage * value > 32) P->backStage /= 2;  // LDS budget
             }
             return DDFAN_OK;
         }
+        case DDFAN_PARAM_BACK_STAGE:
+            CHECK((value == 0 || value == 1 || value == 2 || value == 4) && value * P->ZTb <= 32,
+                  DDFAN_EINVAL, "staged views must be 0, 1, 2 or 4 with views x chunk <= 32");
+            HIPCALL(hipSetDevice(P->device));
+            HIPCALL(hipStreamSynchronize(P->stream));
+            P->backStage = value;
+            return DDFAN_OK;
         default:
             return fail(DDFAN_EINVAL, "unknown plan parameter %d", param);
     }
@@ -356,6 +367,7 @@ int ddfan_plan_get_param(const ddfan_plan* P, int param, int* value) {
         case DDFAN_PARAM_VIEWS_PER_WAVE: *value = P->viewsPerWave; return DDFAN_OK;
         case DDFAN_PARAM_PROJ_CHUNK: *value = P->ZT; return DDFAN_OK;
         case DDFAN_PARAM_BACK_CHUNK: *value = P->ZTb; return DDFAN_OK;
+        case DDFAN_PARAM_BACK_STAGE: *value = P->backStage; return DDFAN_OK;
         default: return fail(DDFAN_EINVAL, "unknown plan parameter %d", param);
     }
 }
@@ -406,7 +418,7 @@ int ddfan_plan_back(ddfan_plan* P, const float* d_prj, float* d_vol, int first, 
     HIPCALL(P->timed(3, [&] {
         return launch_back(P->mode, P->ZTb, BACK_PLAIN, P->g, P->nzcb, P->cs.as<float2>(),
                            P->mask.as<uint8_t>(), P->satP.p, sub, d_vol, nullptr, nullptr, nullptr,
-                           nullptr, P->stream);
+                           nullptr, P->backStage, P->stream);
     }));
     return DDFAN_OK;
 }

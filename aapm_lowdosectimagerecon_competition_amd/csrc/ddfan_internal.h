@@ -88,6 +88,7 @@ struct BackJob {
     float* xprev;
     const float* momTab;
     const int* counter;
+    int NV = 0;  // views per LDS-staged group (k_back_lds); 0 = per-lane gathers (k_back)
 };
 
 hipError_t launch_sat_vol_batch(int mode, const SatVolJob* jobs, int n, hipStream_t st);
@@ -115,7 +116,7 @@ hipError_t launch_proj(int mode, int ZT, int epi, const GeomDev& g, int nzc, con
 hipError_t launch_back(int mode, int ZT, int epi, const GeomDev& g, int nzc, const float2* cossin,
                        const uint8_t* mask, const void* P, Subset sub, float* vol,
                        const float* colSum, float* xprev, const float* momTab,
-                       const int* iterCounter, hipStream_t st);
+                       const int* iterCounter, int stageViews, hipStream_t st);
 
 hipError_t launch_counter_inc(int* counter, hipStream_t st);
 hipError_t launch_fill(float* p, float v, size_t n, hipStream_t st);

@@ -504,6 +504,210 @@ void k_back(const BackLaunch L) {
     back_store(J, g, x, y, z0, acc, ZT);
 }
 
+// Staged backprojector: the sums of k_back, with the table reads served from
+// LDS.  For each group of NV consecutive views the block copies, per (view,
+// slice), the 1 KiB run of detector entries around the tile's footprint
+// (64 two-float / f64 entries, 128 f32 entries) into LDS with one 64-lane
+// global-to-LDS load, and the lanes gather from LDS instead of through the
+// texture-address path (one address per lane per slice, the bound of k_back).
+// Two buffers alternate: the copy of the next group is in flight while the
+// current group is summed.  A view's window is centred on the footprint of
+// the tile's four corner voxels, which every wave computes the same way (no
+// exchange needed); a lane whose entries fall outside it reads the table in
+// global memory, so the window decides speed only.  Arithmetic and summation
+// order are those of k_back, so the results are bit-identical.
+template <class E>
+__device__ __forceinline__ void stage_row(const E* src, E* dst, int nEntries, int lane) {
+    constexpr int PER = 16 / (int)sizeof(E);  // entries one lane copies
+    if (lane * PER < nEntries)
+        __builtin_amdgcn_global_load_lds(static_cast<const void*>(src + lane * PER),
+                                         (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
+}
+// LDS entry read.  A two-float entry is read as one native 16-byte vector:
+// the compiler keeps that a single ds_read_b128, where the float4 of the
+// global path narrows to a 12-byte ds_read_b96 (measured on MI355X at
+// 512x512x64, 2304 views: 20.3 ms with 16-byte reads, 25.4 ms with 12-byte).
+template <class E>
+__device__ __forceinline__ E lds_entry(const E* p) {
+    return *p;
+}
+typedef float f4v __attribute__((ext_vector_type(4)));
+template <>
+__device__ __forceinline__ float4 lds_entry<float4>(const float4* p) {
+    const f4v v = *reinterpret_cast<const f4v*>(p);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+__device__ __forceinline__ int quad_min(int v) {
+    v = min(v, __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false));  // quad_perm [1,0,3,2]
+    return min(v, __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false));  // quad_perm [2,3,0,1]
+}
+__device__ __forceinline__ int quad_max(int v) {
+    v = max(v, __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false));
+    return max(v, __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false));
+}
+
+// Blocks per CU the two staging buffers (2 * NV * ZT KiB) leave room for in
+// 160 KiB of LDS; each block puts one wave on each SIMD.
+constexpr int back_lds_waves(int NV, int ZT) {
+    return 160 / (2 * NV * ZT) < 8 ? 160 / (2 * NV * ZT) : 8;
+}
+
+template <int MODE, int ZT, int NV>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(back_lds_waves(NV, ZT))))
+void k_back_lds(const BackLaunch L) {
+    using E = typename Sat<MODE>::E;
+    constexpr int WENT = 1024 / (int)sizeof(E);  // entries per staged row
+    constexpr int ROWS = NV * ZT;
+    static_assert(NV >= 1 && NV <= 16, "four corner lanes per view");
+    __shared__ __attribute__((aligned(16))) E bufA[ROWS * WENT];
+    __shared__ __attribute__((aligned(16))) E bufB[ROWS * WENT];
+    const int lin = xcd_linear(blockIdx.x, gridDim.x);
+    if (lin >= L.nwork) return;
+    int ji = 0;
+    for (int t = 1; t < L.njobs; ++t)
+        if (lin >= L.j[t].blk0) ji = t;
+    const BackDev& J = L.j[ji];
+    const GeomDev& g = J.g;
+    const int local = lin - J.blk0;
+    const int tile = local % (J.ntx * J.nty);
+    const int zc = local / (J.ntx * J.nty);
+    const int tx0 = (tile % J.ntx) * 16, ty0 = (tile / J.ntx) * 16;
+    const int x = tx0 + (threadIdx.x & 15);
+    const int y = ty0 + (threadIdx.x >> 4);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const bool inb = x < g.XN && y < g.YN;
+    const int m = inb ? J.mask[(size_t)y * g.XN + x] : 0;
+    const int z0 = zc * ZT;
+    const int zg = min(z0, g.SLN - 1);
+    const int DNU = g.DNU;
+    const int K = DNU + 1;
+    const int count = J.sub.count;
+    const size_t viewStride = (size_t)ZT * K;
+    const E* Pc = reinterpret_cast<const E*>(J.P) + (size_t)zc * count * viewStride;
+
+    float acc[ZT];
+#pragma unroll
+    for (int q = 0; q < ZT; ++q) acc[q] = 0.0f;
+    // Any unmasked voxel in the tile?  (Reduced through the first staging
+    // buffer, so the block's LDS is exactly the two buffers.)
+    {
+        const unsigned long long b = __ballot(m != 0);
+        if (lane == 0) reinterpret_cast<int*>(bufA)[wave] = b != 0ull;
+    }
+    __syncthreads();
+    const int* fl = reinterpret_cast<const int*>(bufA);
+    const bool any = (fl[0] | fl[1] | fl[2] | fl[3]) != 0;
+    __syncthreads();
+    if (any && count > 0) {
+        // Window starts of 16 consecutive views v0.., one per quad of lanes
+        // (lane = 4*(view - v0) + corner of the tile).
+        const int cx = (lane & 1) ? min(tx0 + 15, g.XN - 1) : tx0;
+        const int cy = (lane & 2) ? min(ty0 + 15, g.YN - 1) : ty0;
+        const float cvx = ((float)cx - g.ocxB) * g.dx, cvy = ((float)cy - g.ocyB) * g.dx;
+        int wl = 0;
+        auto windows16 = [&](int v0) {
+            const int j = min(v0 + (lane >> 2), count - 1);
+            const int vg = J.sub.first + j * J.sub.stride;
+            const float2 cs = J.cossin[(size_t)vg * g.SLN + zg];
+            const BackSample bs = back_sample(cvx, cvy, cs.x, cs.y, g.x0, g.y0, g.inv_sid, g.invdb,
+                                              g.detCntIdx, g.dx);
+            const int lo = quad_min(texel(fminf(bs.idxL, bs.idxR), DNU));
+            const int hi = quad_max(texel(fmaxf(bs.idxL, bs.idxR), DNU));
+            wl = max(0, min(((lo + hi) >> 1) - WENT / 2, K - WENT));
+        };
+        auto window = [&](int grp, int* w0) {
+            const int j0 = grp * NV;  // NV divides 16: a group never straddles two sets
+            if ((j0 & 15) == 0) windows16(j0);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) w0[v] = __builtin_amdgcn_readlane(wl, 4 * ((j0 + v) & 15));
+        };
+        // Row (view v, slice q) is copied by wave (v*ZT + q) mod 4.
+        auto stage = [&](int grp, E* buf, const int* w0) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const int j = grp * NV + v;
+                if (j >= count) break;
+                const E* src = Pc + (size_t)j * viewStride + w0[v];
+                if constexpr (ZT % 4 == 0) {
+#pragma unroll
+                    for (int i = 0; i < ZT / 4; ++i) {
+                        const int q = 4 * i + wave;
+                        stage_row(src + (size_t)q * K, buf + (v * ZT + q) * WENT, K - w0[v], lane);
+                    }
+                } else {
+#pragma unroll
+                    for (int q = 0; q < ZT; ++q)
+                        if (((v * ZT + q) & 3) == wave)
+                            stage_row(src + (size_t)q * K, buf + (v * ZT + q) * WENT, K - w0[v],
+                                      lane);
+                }
+            }
+        };
+        const float vx = ((float)x - g.ocxB) * g.dx;
+        const float vy = ((float)y - g.ocyB) * g.dx;
+        auto sum = [&](int grp, const E* buf, const int* w0) {
+            if (m == 0) return;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const int j = grp * NV + v;
+                if (j >= count) break;
+                const int vg = J.sub.first + j * J.sub.stride;
+                const float2 cs = J.cossin[(size_t)vg * g.SLN + zg];
+                const BackSample bs = back_sample(vx, vy, cs.x, cs.y, g.x0, g.y0, g.inv_sid,
+                                                  g.invdb, g.detCntIdx, g.dx);
+                const Fetch fL = fetch_pos<E>(bs.idxL, DNU), fR = fetch_pos<E>(bs.idxR, DNU);
+                const float cosVal = g.dx * __builtin_amdgcn_rcpf(bs.ddv * bs.l);
+                const int iL = (int)(fL.off / (unsigned)sizeof(E)) - w0[v];
+                const int iR = (int)(fR.off / (unsigned)sizeof(E)) - w0[v];
+                if ((unsigned)iL < (unsigned)WENT && (unsigned)iR < (unsigned)WENT) {
+                    const E* lrow = buf + v * ZT * WENT;
+#pragma unroll
+                    for (int q = 0; q < ZT; ++q) {
+                        const Samp<MODE> TL(lds_entry(lrow + q * WENT + iL), fL.t);
+                        const Samp<MODE> TR(lds_entry(lrow + q * WENT + iR), fR.t);
+                        acc[q] = fmaf(TR.minus(TL), cosVal, acc[q]);
+                    }
+                } else {
+                    const E* row = Pc + (size_t)j * viewStride;
+#pragma unroll
+                    for (int q = 0; q < ZT; ++q) {
+                        const Samp<MODE> TL(entry_at(row + (size_t)q * K, fL.off), fL.t);
+                        const Samp<MODE> TR(entry_at(row + (size_t)q * K, fR.off), fR.t);
+                        acc[q] = fmaf(TR.minus(TL), cosVal, acc[q]);
+                    }
+                }
+            }
+        };
+        int wA[NV], wB[NV];
+        const int ng = (count + NV - 1) / NV;
+        window(0, wA);
+        stage(0, bufA, wA);
+        wait_vmem();
+        __syncthreads();
+        for (int grp = 0; grp < ng; grp += 2) {
+            if (grp + 1 < ng) {
+                window(grp + 1, wB);
+                stage(grp + 1, bufB, wB);
+            }
+            sum(grp, bufA, wA);
+            wait_vmem();
+            __syncthreads();
+            if (grp + 1 >= ng) break;
+            if (grp + 2 < ng) {
+                window(grp + 2, wA);
+                stage(grp + 2, bufA, wA);
+            }
+            sum(grp + 1, bufB, wB);
+            wait_vmem();
+            __syncthreads();
+        }
+    }
+    if (!inb) return;
+    back_store(J, g, x, y, z0, acc, ZT);
+}
+
 // ------------------------------------------------------------------ index traces
 __global__ __launch_bounds__(256) void k_proj_trace(const GeomDev g, const float2* __restrict__ cossin,
                                                     const float* __restrict__ bx,
@@ -895,7 +1099,31 @@ hipError_t launch_proj(int mode, int ZT, int epi, const GeomDev& g, int nzc, con
     return launch_proj_batch(mode, &j, 1, st);
 }
 
-static hipError_t launch_back_group(int mode, int ZT, const BackJob* const* jobs, int n,
+// Views per staged group of k_back_lds: 1, 2 or 4, with the two LDS buffers of
+// 2 * NV * ZT KiB within 64 KiB.
+static bool back_stage_ok(int ZT, int NV) {
+    return NV == 0 || ((NV == 1 || NV == 2 || NV == 4) && NV * ZT <= 32);
+}
+
+template <int MC, int ZTC>
+static hipError_t launch_back_kernel(int NV, dim3 grid, const BackLaunch& L, hipStream_t st) {
+    switch (NV) {
+        case 0: k_back<MC, ZTC><<<grid, 256, 0, st>>>(L); break;
+        case 1: k_back_lds<MC, ZTC, 1><<<grid, 256, 0, st>>>(L); break;
+        case 2:
+            if constexpr (2 * ZTC <= 32) k_back_lds<MC, ZTC, 2><<<grid, 256, 0, st>>>(L);
+            else return hipErrorInvalidValue;
+            break;
+        case 4:
+            if constexpr (4 * ZTC <= 32) k_back_lds<MC, ZTC, 4><<<grid, 256, 0, st>>>(L);
+            else return hipErrorInvalidValue;
+            break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+static hipError_t launch_back_group(int mode, int ZT, int NV, const BackJob* const* jobs, int n,
                                     hipStream_t st) {
     BackLaunch L{};
     long long nw = 0;
@@ -925,32 +1153,38 @@ static hipError_t launch_back_group(int mode, int ZT, const BackJob* const* jobs
     L.njobs = m;
     L.nwork = (int)nw;
     dim3 grid((unsigned)xcd_grid(L.nwork));
-    DDF_MODE_SWITCH(mode, DDF_ZT_SWITCH(ZT, (k_back<MC, ZTC><<<grid, 256, 0, st>>>(L))));
-    return hipGetLastError();
+    hipError_t e = hipErrorInvalidValue;
+    DDF_MODE_SWITCH(mode, DDF_ZT_SWITCH(ZT, (e = launch_back_kernel<MC, ZTC>(NV, grid, L, st))));
+    return e;
 }
 
 hipError_t launch_back_batch(int mode, const BackJob* jobs, int n, hipStream_t st,
                              long long fillBlocks) {
+    for (int i = 0; i < n; ++i)
+        if (!back_stage_ok(jobs[i].ZT, jobs[i].NV)) return hipErrorInvalidValue;
     for (int ZT : {1, 2, 4, 8, 16}) {
-        const BackJob* grp[kBatch];
-        int m = 0;
-        long long work = 0;
-        for (int i = 0; i < n; ++i) {
-            const BackJob& J = jobs[i];
-            if (J.ZT != ZT) continue;
-            const long long w = (long long)((J.g.XN + 15) / 16) * ((J.g.YN + 15) / 16) * J.nzc;
-            if (m == kBatch || (m > 0 && work + w > fillBlocks)) {
-                const hipError_t e = launch_back_group(mode, ZT, grp, m, st);
-                if (e != hipSuccess) return e;
-                m = 0;
-                work = 0;
+        for (int NV : {0, 1, 2, 4}) {
+            const BackJob* grp[kBatch];
+            int m = 0;
+            long long work = 0;
+            for (int i = 0; i < n; ++i) {
+                const BackJob& J = jobs[i];
+                if (J.ZT != ZT || J.NV != NV) continue;
+                const long long w =
+                    (long long)((J.g.XN + 15) / 16) * ((J.g.YN + 15) / 16) * J.nzc;
+                if (m == kBatch || (m > 0 && work + w > fillBlocks)) {
+                    const hipError_t e = launch_back_group(mode, ZT, NV, grp, m, st);
+                    if (e != hipSuccess) return e;
+                    m = 0;
+                    work = 0;
+                }
+                grp[m++] = &J;
+                work += w;
             }
-            grp[m++] = &J;
-            work += w;
-        }
-        if (m) {
-            const hipError_t e = launch_back_group(mode, ZT, grp, m, st);
-            if (e != hipSuccess) return e;
+            if (m) {
+                const hipError_t e = launch_back_group(mode, ZT, NV, grp, m, st);
+                if (e != hipSuccess) return e;
+            }
         }
     }
     return hipSuccess;
@@ -959,8 +1193,9 @@ hipError_t launch_back_batch(int mode, const BackJob* jobs, int n, hipStream_t s
 hipError_t launch_back(int mode, int ZT, int epi, const GeomDev& g, int nzc, const float2* cossin,
                        const uint8_t* mask, const void* P, Subset sub, float* vol,
                        const float* colSum, float* xprev, const float* momTab,
-                       const int* iterCounter, hipStream_t st) {
-    const BackJob j{g, sub, nzc, ZT, epi, cossin, mask, P, vol, colSum, xprev, momTab, iterCounter};
+                       const int* iterCounter, int stageViews, hipStream_t st) {
+    const BackJob j{g,      sub,   nzc,    ZT,          epi,       cossin, mask, P,
+                    vol,    colSum, xprev, momTab,      iterCounter, stageViews};
     return launch_back_batch(mode, &j, 1, st);
 }
 

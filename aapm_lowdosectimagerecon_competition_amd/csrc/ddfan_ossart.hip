@@ -68,7 +68,7 @@ int ossart_prepare(OsJob& J, hipStream_t st) {
         HIPCALL(launch_sat_sino(mode, 1, DNU, 1, 1, J.cnt[k], P->onesP.as<float>(), P->satP0.p, st));
         HIPCALL(launch_back(mode, 1, BACK_PLAIN, g0, 1, P->cs0.as<float2>(), P->mask.as<uint8_t>(),
                             P->satP0.p, sub, P->colSum.as<float>() + (size_t)k * npix, nullptr,
-                            nullptr, nullptr, nullptr, st));
+                            nullptr, nullptr, nullptr, 0, st));
     }
     // FISTA momenta (OSSART_AAPM.m:111-115): MATLAB double scalars, applied to
     // single arrays as single((t0-1)/t1).
@@ -128,7 +128,7 @@ hipError_t enqueue_batch_iteration(OsJob* const* jobs, int n, int mode, const Ba
             bj.push_back(BackJob{g, sub, P->nzcb, P->ZTb, epi, P->cs.as<float2>(),
                                  P->mask.as<uint8_t>(), P->satP.p, J.x,
                                  P->colSum.as<float>() + (size_t)k * npix, P->xprev.as<float>(),
-                                 P->momTab.as<float>(), P->counter.as<int>()});
+                                 P->momTab.as<float>(), P->counter.as<int>(), P->backStage});
         }
         const int m = (int)sv.size();
         if (m == 0) continue;

@@ -3,13 +3,14 @@
 // For one plan shape and one launch size (views per operator call) it times,
 // with HIP events on the plan's stream, the projector over slice chunks x
 // views per wave (one slab pass), then slab tilings of the winner, and the
-// backprojector over its slice chunks, and keeps the fastest.  Results live in
+// backprojector over slice chunks x LDS-staged view groups, and keeps the
+// fastest.  Results live in
 // a process-wide table keyed by (SAT mode, XN, YN, SLN, DNU, widest uniform
 // chunk, views), which can be saved to and loaded from a text file and is
 // consulted by ddfan_plan_apply_tuned (the request loop applies it to every
-// admitted reconstruction).  No parameter changes results: slice chunks and
-// views per wave give bit-identical output, slab passes change only the
-// summation grouping of the projector.
+// admitted reconstruction).  No parameter changes results: slice chunks,
+// views per wave and staged groups give bit-identical output, slab passes
+// change only the summation grouping of the projector.
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 
@@ -33,9 +34,13 @@ struct TuneKey {
     }
 };
 struct TuneVal {
-    int ZT, V, tile, ZTb;
+    int ZT, V, tile, ZTb, stage;
     double projMs, backMs;
 };
+
+bool stage_fits(int stage, int ZTb) {
+    return (stage == 0 || stage == 1 || stage == 2 || stage == 4) && stage * ZTb <= 32;
+}
 
 std::mutex g_tuneMu;
 std::map<TuneKey, TuneVal> g_tune;
@@ -49,6 +54,7 @@ int apply(ddfan_plan* P, const TuneVal& v) {
     TRY(ddfan_plan_set_param(P, DDFAN_PARAM_VIEWS_PER_WAVE, v.V));
     TRY(ddfan_plan_set_param(P, DDFAN_PARAM_SLAB_TILE, v.tile));
     TRY(ddfan_plan_set_param(P, DDFAN_PARAM_BACK_CHUNK, v.ZTb));
+    TRY(ddfan_plan_set_param(P, DDFAN_PARAM_BACK_STAGE, v.stage));
     return DDFAN_OK;
 }
 
@@ -114,7 +120,7 @@ int ddfan_plan_autotune(ddfan_plan* P, int views, int reps, int flags, int* out_
         HIPCALL(launch_fill(vol.as<float>(), 1.0f, nvox, P->stream));
         HIPCALL(launch_fill(prj.as<float>(), 1.0f, nprj, P->stream));
         // the plan's own settings are restored if tuning fails part-way
-        const TuneVal saved{P->ZT, P->viewsPerWave, P->slabTile, P->ZTb, 0.0, 0.0};
+        const TuneVal saved{P->ZT, P->viewsPerWave, P->slabTile, P->ZTb, P->backStage, 0.0, 0.0};
         int rc = [&]() -> int {
             std::vector<int> chunks;
             for (int z = std::min(P->ZTmax, 16); z >= 1; z >>= 1) chunks.push_back(z);
@@ -156,12 +162,18 @@ int ddfan_plan_autotune(ddfan_plan* P, int views, int reps, int flags, int* out_
             best.projMs = bestP;
             double bestB = 1e30;
             for (int z : chunks) {
+                TRY(ddfan_plan_set_param(P, DDFAN_PARAM_BACK_STAGE, 0));
                 TRY(ddfan_plan_set_param(P, DDFAN_PARAM_BACK_CHUNK, z));
-                double ms;
-                TRY(time_ms(P->stream, reps, back, &ms));
-                if (ms < bestB) {
-                    bestB = ms;
-                    best.ZTb = z;
+                for (int st : {1, 2, 4, 0}) {
+                    if (!stage_fits(st, z)) continue;
+                    TRY(ddfan_plan_set_param(P, DDFAN_PARAM_BACK_STAGE, st));
+                    double ms;
+                    TRY(time_ms(P->stream, reps, back, &ms));
+                    if (ms < bestB) {
+                        bestB = ms;
+                        best.ZTb = z;
+                        best.stage = st;
+                    }
                 }
             }
             best.backMs = bestB;
@@ -181,6 +193,7 @@ int ddfan_plan_autotune(ddfan_plan* P, int views, int reps, int flags, int* out_
         out_params[1] = best.V;
         out_params[2] = best.tile;
         out_params[3] = best.ZTb;
+        out_params[4] = best.stage;
     }
     if (out_ms) {
         out_ms[0] = best.projMs;
@@ -204,18 +217,20 @@ int ddfan_plan_apply_tuned(ddfan_plan* P, int views, int* found) {
 }
 
 // One line per entry:
-// mode XN YN SLN DNU ZTmax views  ZT V tile ZTb  proj_ms back_ms
+// mode XN YN SLN DNU ZTmax views  ZT V tile ZTb stage  proj_ms back_ms
 int ddfan_tune_save(const char* path) {
     CHECK(path, DDFAN_EINVAL, "path is NULL");
     FILE* f = fopen(path, "w");
     CHECK(f, DDFAN_EINVAL, "cannot write %s", path);
-    fprintf(f, "# ddfan launch-shape table: mode XN YN SLN DNU ZTmax views  ZT V tile ZTb  proj_ms back_ms\n");
+    fprintf(f, "# ddfan launch-shape table: mode XN YN SLN DNU ZTmax views  ZT V tile ZTb stage  "
+               "proj_ms back_ms\n");
     std::lock_guard<std::mutex> lk(g_tuneMu);
     for (const auto& kv : g_tune) {
         const TuneKey& k = kv.first;
         const TuneVal& v = kv.second;
-        fprintf(f, "%d %d %d %d %d %d %d  %d %d %d %d  %.4f %.4f\n", k.mode, k.XN, k.YN, k.SLN,
-                k.DNU, k.ZTmax, k.views, v.ZT, v.V, v.tile, v.ZTb, v.projMs, v.backMs);
+        fprintf(f, "%d %d %d %d %d %d %d  %d %d %d %d %d  %.4f %.4f\n", k.mode, k.XN, k.YN,
+                k.SLN, k.DNU, k.ZTmax, k.views, v.ZT, v.V, v.tile, v.ZTb, v.stage, v.projMs,
+                v.backMs);
     }
     const int bad = ferror(f);
     fclose(f);
@@ -235,16 +250,24 @@ int ddfan_tune_load(const char* path, int* entries) {
         if (line[0] == '#' || line[0] == '\n') continue;
         TuneKey k{};
         TuneVal v{};
-        if (sscanf(line, "%d %d %d %d %d %d %d %d %d %d %d %lf %lf", &k.mode, &k.XN, &k.YN, &k.SLN,
-                   &k.DNU, &k.ZTmax, &k.views, &v.ZT, &v.V, &v.tile, &v.ZTb, &v.projMs,
-                   &v.backMs) != 13) {
+        int tokens = 0;
+        for (const char* c = line; *c;) {
+            while (*c == ' ' || *c == '\t' || *c == '\n' || *c == '\r') ++c;
+            if (!*c) break;
+            ++tokens;
+            while (*c && *c != ' ' && *c != '\t' && *c != '\n' && *c != '\r') ++c;
+        }
+        if (tokens != 14 ||
+            sscanf(line, "%d %d %d %d %d %d %d %d %d %d %d %d %lf %lf", &k.mode, &k.XN, &k.YN,
+                   &k.SLN, &k.DNU, &k.ZTmax, &k.views, &v.ZT, &v.V, &v.tile, &v.ZTb, &v.stage,
+                   &v.projMs, &v.backMs) != 14) {
             bad = lineNo;
             break;
         }
         const bool pow2 = v.ZT >= 1 && v.ZT <= 16 && (v.ZT & (v.ZT - 1)) == 0 && v.ZTb >= 1 &&
                           v.ZTb <= 16 && (v.ZTb & (v.ZTb - 1)) == 0;
         if (!pow2 || v.ZT > k.ZTmax || v.ZTb > k.ZTmax || !(v.V == 1 || v.V == 2 || v.V == 4) ||
-            v.tile < 0) {
+            v.tile < 0 || !stage_fits(v.stage, v.ZTb)) {
             bad = lineNo;
             break;
         }

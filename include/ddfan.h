@@ -152,15 +152,23 @@ int ddfan_plan_query(const ddfan_plan* plan, int* slice_chunk, int* n_chunks, in
  * the backprojector. */
 #define DDFAN_PARAM_PROJ_CHUNK 3
 #define DDFAN_PARAM_BACK_CHUNK 4
+/* DDFAN_PARAM_BACK_STAGE: views per group whose detector windows the
+ * backprojector copies into LDS before summing (0 = every lane reads the
+ * table in global memory; 1, 2 or 4, with value x back chunk <= 32).  Results
+ * are bit-identical for every value.  Narrowing the back chunk halves it as
+ * needed. */
+#define DDFAN_PARAM_BACK_STAGE 5
 int ddfan_plan_set_param(ddfan_plan* plan, int param, int value);
 int ddfan_plan_get_param(const ddfan_plan* plan, int param, int* value);
 /* Autotuner: times the projector over slice chunks x views per wave x slab
- * tilings and the backprojector over slice chunks, for launches of `views`
+ * tilings and the backprojector over slice chunks x staged view groups, for
+ * launches of `views`
  * views (0: the plan's PN) with this plan's shape, sets the fastest on the
  * plan and records it in a process-wide table (key: SAT mode, XN, YN, SLN,
  * DNU, widest uniform chunk, views).  flags DDFAN_TUNE_USE_CACHE takes a
- * recorded entry without timing.  out_params (may be NULL) receives
- * {proj chunk, views per wave, slab tile, back chunk}; out_ms (may be NULL)
+ * recorded entry without timing.  out_params (may be NULL, else 5 ints)
+ * receives {proj chunk, views per wave, slab tile, back chunk, back stage};
+ * out_ms (may be NULL)
  * the projector and backprojector milliseconds per call.  No parameter
  * changes results beyond the projector's slab-pass summation grouping. */
 #define DDFAN_TUNE_USE_CACHE 1
