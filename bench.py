@@ -454,6 +454,7 @@ def main():
                                        "views_per_wave": plan.views_per_wave,
                                        "slab_tile": plan.slab_tile,
                                        "back_chunk": plan.back_chunk,
+                                       "back_stage": plan.back_stage,
                                        "autotuned": tuned is not None}},
            "roofline": roof, "kernels_ms": kt}
 
