@@ -41,6 +41,7 @@ PARAM_VIEWS_PER_WAVE = 2
 PARAM_PROJ_CHUNK = 3
 PARAM_BACK_CHUNK = 4
 PARAM_BACK_STAGE = 5
+PARAM_PROJ_STAGE = 6
 TUNE_USE_CACHE = 1
 BATCH_PROJ_FILL = 1
 BATCH_BACK_FILL = 2

@@ -265,11 +265,13 @@ int ddfan_plan_create(const ddfan_geom* G, const float* hangs, int PN, const uns
         P->mode = sat_mode;
         P->ZTmax = choose_slice_chunk(hangs, PN, SLN, slice_chunk);
         // Launch-shape defaults measured on MI355X at 512x512x64, 2304 views:
-        // the projector runs best on chunks of <= 8 slices with 4 views per
-        // wave in one slab pass, the backprojector on the widest chunk with
-        // its detector windows staged through LDS one view at a time (narrow
+        // the projector stages its table rows through LDS on chunks of 4
+        // slices (8 with f32 tables) with 4 views per wave in one slab pass
+        // (narrower chunks read per lane), the backprojector runs on the widest
+        // chunk with its detector windows staged one view at a time (narrow
         // chunks stage 4 / chunk views per group, so each wave copies a row).
-        P->ZT = std::min(P->ZTmax, 8);
+        P->ZT = std::min(P->ZTmax, sat_mode == SAT_F32 ? 8 : 4);
+        P->projStage = P->ZT >= 4 ? 1 : 0;
         P->nzc = (SLN + P->ZT - 1) / P->ZT;
         P->ZTb = P->ZTmax;
         P->nzcb = (SLN + P->ZTb - 1) / P->ZTb;
@@ -348,6 +350,12 @@ int ddfan_plan_set_param(ddfan_plan* P, int param, int value) {
             }
             return DDFAN_OK;
         }
+        case DDFAN_PARAM_PROJ_STAGE:
+            CHECK(value == 0 || value == 1, DDFAN_EINVAL, "projector stage must be 0 or 1");
+            HIPCALL(hipSetDevice(P->device));
+            HIPCALL(hipStreamSynchronize(P->stream));
+            P->projStage = value;
+            return DDFAN_OK;
         case DDFAN_PARAM_BACK_STAGE:
             CHECK((value == 0 || value == 1 || value == 2 || value == 4) && value * P->ZTb <= 32,
                   DDFAN_EINVAL, "staged views must be 0, 1, 2 or 4 with views x chunk <= 32");
@@ -368,6 +376,7 @@ int ddfan_plan_get_param(const ddfan_plan* P, int param, int* value) {
         case DDFAN_PARAM_PROJ_CHUNK: *value = P->ZT; return DDFAN_OK;
         case DDFAN_PARAM_BACK_CHUNK: *value = P->ZTb; return DDFAN_OK;
         case DDFAN_PARAM_BACK_STAGE: *value = P->backStage; return DDFAN_OK;
+        case DDFAN_PARAM_PROJ_STAGE: *value = P->projStage; return DDFAN_OK;
         default: return fail(DDFAN_EINVAL, "unknown plan parameter %d", param);
     }
 }
@@ -395,7 +404,7 @@ int ddfan_plan_proj(ddfan_plan* P, const float* d_vol, float* d_prj, int first, 
         return launch_proj(P->mode, P->ZT, PROJ_PLAIN, P->g, P->nzc, P->cs.as<float2>(),
                            P->bx.as<float>(), P->by.as<float>(), P->xds.as<float>(),
                            P->yds.as<float>(), P->satX.p, P->satY.p, sub, d_prj, nullptr, nullptr,
-                           P->slabTile, P->viewsPerWave, P->stream);
+                           P->slabTile, P->viewsPerWave, P->projStage, P->stream);
     }));
     return DDFAN_OK;
 }

@@ -75,6 +75,7 @@ struct ProjJob {
     float* out;
     const float* meas;
     const float* rowSum;
+    int stage = 0;  // 1: LDS-staged table rows (k_proj_lds); 0: per-lane gathers (k_proj)
 };
 struct BackJob {
     GeomDev g;
@@ -112,7 +113,7 @@ hipError_t launch_proj(int mode, int ZT, int epi, const GeomDev& g, int nzc, con
                        const float* bx, const float* by, const float* xds, const float* yds,
                        const void* satX, const void* satY, Subset sub, float* out,
                        const float* meas, const float* rowSum, int slabTile, int viewsPerWave,
-                       hipStream_t st);
+                       int stage, hipStream_t st);
 hipError_t launch_back(int mode, int ZT, int epi, const GeomDev& g, int nzc, const float2* cossin,
                        const uint8_t* mask, const void* P, Subset sub, float* vol,
                        const float* colSum, float* xprev, const float* momTab,

@@ -287,6 +287,39 @@ struct ProjLaunch {
 #endif
 
 
+// Writes one detector's ZT sums of the V views of view group jg: plain (the
+// operator; later slab passes add to the earlier ones) or, on the last pass,
+// the OS-SART ratio (b - p) / rowSum (OSSART_AAPM.m:81-88).
+template <int ZT, int V>
+__device__ __forceinline__ void proj_store(const ProjDev& J, const GeomDev& g, const Subset& sub,
+                                           int jg, int u, int z0, const float (&acc)[V][ZT],
+                                           const float (&cval)[V]) {
+    const bool ratio = J.epi == PROJ_RATIO && J.last;
+    float* __restrict__ out = J.out;
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        const int j = jg * V + v;
+        if (j >= sub.count) break;
+        const int vg = sub.first + j * sub.stride;
+        const size_t o0 = ((size_t)j * g.DNU + u) * g.SLN + z0;
+        const float w = ratio ? J.rowSum[(size_t)j * g.DNU + u] : 0.0f;
+#pragma unroll
+        for (int q = 0; q < ZT; ++q) {
+            const int z = z0 + q;
+            if (z < g.SLN) {
+                float p = acc[v][q] * cval[v];
+                if (!J.first) p = p + out[o0 + q];
+                if (!ratio) {
+                    out[o0 + q] = p;
+                } else {
+                    const float b = J.meas[((size_t)vg * g.DNU + u) * g.SLN + z];
+                    out[o0 + q] = (w != 0.0f) ? (b - p) / w : p;
+                }
+            }
+        }
+    }
+}
+
 template <int MODE, int ZT, int V>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(DDF_PROJ_WAVES, DDF_PROJ_WAVES)))
 void k_proj(const ProjLaunch L) {
@@ -371,30 +404,7 @@ void k_proj(const ProjLaunch L) {
         }
     }
     if (!act) return;
-    const bool ratio = J.epi == PROJ_RATIO && J.last;
-    float* __restrict__ out = J.out;
-#pragma unroll
-    for (int v = 0; v < V; ++v) {
-        const int j = jg * V + v;
-        if (j >= sub.count) break;
-        const int vg = sub.first + j * sub.stride;
-        const size_t o0 = ((size_t)j * g.DNU + u) * g.SLN + z0;
-        const float w = ratio ? J.rowSum[(size_t)j * g.DNU + u] : 0.0f;
-#pragma unroll
-        for (int q = 0; q < ZT; ++q) {
-            const int z = z0 + q;
-            if (z < g.SLN) {
-                float p = acc[v][q] * cval[v];
-                if (!J.first) p = p + out[o0 + q];
-                if (!ratio) {
-                    out[o0 + q] = p;
-                } else {
-                    const float b = J.meas[((size_t)vg * g.DNU + u) * g.SLN + z];
-                    out[o0 + q] = (w != 0.0f) ? (b - p) / w : p;
-                }
-            }
-        }
-    }
+    proj_store<ZT, V>(J, g, sub, jg, u, z0, acc, cval);
 }
 
 // ------------------------------------------------------------------ backprojector
@@ -607,7 +617,7 @@ void k_back_lds(const BackLaunch L) {
         const int cy = (lane & 2) ? min(ty0 + 15, g.YN - 1) : ty0;
         const float cvx = ((float)cx - g.ocxB) * g.dx, cvy = ((float)cy - g.ocyB) * g.dx;
         int wl = 0;
-        auto windows16 = [&](int v0) {
+        auto windows16 = [&](int v0) __attribute__((always_inline)) {
             const int j = min(v0 + (lane >> 2), count - 1);
             const int vg = J.sub.first + j * J.sub.stride;
             const float2 cs = J.cossin[(size_t)vg * g.SLN + zg];
@@ -617,14 +627,14 @@ void k_back_lds(const BackLaunch L) {
             const int hi = quad_max(texel(fmaxf(bs.idxL, bs.idxR), DNU));
             wl = max(0, min(((lo + hi) >> 1) - WENT / 2, K - WENT));
         };
-        auto window = [&](int grp, int* w0) {
+        auto window = [&](int grp, int* w0) __attribute__((always_inline)) {
             const int j0 = grp * NV;  // NV divides 16: a group never straddles two sets
             if ((j0 & 15) == 0) windows16(j0);
 #pragma unroll
             for (int v = 0; v < NV; ++v) w0[v] = __builtin_amdgcn_readlane(wl, 4 * ((j0 + v) & 15));
         };
         // Row (view v, slice q) is copied by wave (v*ZT + q) mod 4.
-        auto stage = [&](int grp, E* buf, const int* w0) {
+        auto stage = [&](int grp, E* buf, const int* w0) __attribute__((always_inline)) {
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 const int j = grp * NV + v;
@@ -647,7 +657,7 @@ void k_back_lds(const BackLaunch L) {
         };
         const float vx = ((float)x - g.ocxB) * g.dx;
         const float vy = ((float)y - g.ocyB) * g.dx;
-        auto sum = [&](int grp, const E* buf, const int* w0) {
+        auto sum = [&](int grp, const E* buf, const int* w0) __attribute__((always_inline)) {
             if (m == 0) return;
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
@@ -706,6 +716,171 @@ void k_back_lds(const BackLaunch L) {
     }
     if (!inb) return;
     back_store(J, g, x, y, z0, acc, ZT);
+}
+
+// Staged projector: the sums of k_proj, with the table reads served from LDS.
+// A 256-thread block is four waves = four consecutive view groups (4V views) of
+// one detector group and slice chunk; at each slab their boundary rays cross
+// the same stretch of one table row.  For groups of S slabs the block copies,
+// per (slab, slice), the WP-entry run of the row centred on the crossing of
+// one ray (the middle boundary of the block's middle view, which every wave
+// evaluates the same way) into LDS with 64-lane global-to-LDS loads, two
+// buffers alternating, and the lanes gather from LDS.  Views whose slab
+// orientation differs from the middle view's, and lanes whose entry falls
+// outside the run, read the table in global memory.  Arithmetic and summation
+// order are those of k_proj, so the results are bit-identical.
+#ifndef DDF_PROJ_LDS_WAVES
+#define DDF_PROJ_LDS_WAVES 5
+#endif
+template <int MODE, int ZT, int V>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DDF_PROJ_LDS_WAVES)))
+void k_proj_lds(const ProjLaunch L) {
+    using E = typename Sat<MODE>::E;
+    constexpr int WP = 128;                          // entries per staged run
+    constexpr int PER = 16 / (int)sizeof(E);         // entries per lane per copy
+    constexpr int CP = WP / (64 * PER);              // 64-lane copies per run
+    constexpr int S = ZT >= 8 ? 1 : 8 / ZT;          // slabs per group
+    constexpr int ROWS = S * ZT;
+    __shared__ __attribute__((aligned(16))) E buf[2 * ROWS * WP];  // two alternating halves
+    const int lin = xcd_linear(blockIdx.x, gridDim.x);
+    if (lin >= L.nwork) return;  // whole block
+    int ji = 0;
+    for (int t = 1; t < L.njobs; ++t)
+        if (lin >= L.j[t].blk0) ji = t;
+    const ProjDev& J = L.j[ji];
+    const GeomDev g = J.g;
+    const Subset sub = J.sub;
+    const int local = lin - J.blk0;
+    const int ug = local % J.nug;
+    const int rest = local / J.nug;
+    const int jb = rest % J.nvg;  // block of four view groups
+    const int zc = rest / J.nvg;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int jg = jb * 4 + wave;
+    const E* baseX = reinterpret_cast<const E*>(J.satX) + (size_t)zc * g.YN * (size_t)ZT * (g.XN + 1);
+    const E* baseY = reinterpret_cast<const E*>(J.satY) + (size_t)zc * g.XN * (size_t)ZT * (g.YN + 1);
+
+    const int bnd = min(ug * 63 + lane, g.DNU);
+    const int u = ug * 63 + lane - 1;
+    const bool act = lane >= 1 && u < g.DNU;
+    const int uu = min(max(u, 0), g.DNU - 1);
+    const int z0 = zc * ZT;
+    const int zs = min(z0, g.SLN - 1);
+    const float bxb = J.bx[bnd], byb = J.by[bnd], xdu = J.xds[uu], ydu = J.yds[uu];
+
+    float pa[V], pf[V], pbb[V], cval[V];
+    unsigned xs = 0;
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        const int j = min(jg * V + v, sub.count - 1);
+        const int vg = sub.first + j * sub.stride;
+        const float2 cs = J.cossin[(size_t)vg * g.SLN + zs];
+        const ProjBoundary pb = proj_boundary(g.x0, g.y0, cs.x, cs.y, bxb, byb);
+        const float srcx = g.x0 * cs.x - g.y0 * cs.y, srcy = g.x0 * cs.y + g.y0 * cs.x;
+        const float ddx = (xdu * cs.x - ydu * cs.y) - srcx;
+        const float ddy = (xdu * cs.y + ydu * cs.x) - srcy;
+        const float nrm = sqrtf(fmaf(ddx, ddx, ddy * ddy));
+        cval[v] = g.dx * nrm / fabsf(pb.xslab ? ddx : ddy);
+        pa[v] = pb.a;
+        pf[v] = pb.f;
+        pbb[v] = pb.b;
+        if (proj_xslab(cs.x, cs.y)) xs |= 1u << v;
+    }
+
+    // The block's staged table and its window ray.
+    const int jm = min(jb * 4 * V + 2 * V, sub.count - 1);
+    const float2 csm = J.cossin[(size_t)(sub.first + jm * sub.stride) * g.SLN + zs];
+    const bool bxs = proj_xslab(csm.x, csm.y);
+    const int bm = min(ug * 63 + 32, g.DNU);
+    const ProjBoundary pm = proj_boundary(g.x0, g.y0, csm.x, csm.y, J.bx[bm], J.by[bm]);
+    const int NkB = bxs ? g.YN : g.XN, KB = NkB + 1, NsB = bxs ? g.XN : g.YN;
+    const float ocSB = bxs ? g.ocxP : g.ocyP, ocKB = bxs ? g.ocyP : g.ocxP;
+    const E* tabB = bxs ? baseY : baseX;
+    const unsigned xsB = bxs ? 0xFu : 0u;
+
+    float acc[V][ZT];
+#pragma unroll
+    for (int v = 0; v < V; ++v)
+#pragma unroll
+        for (int q = 0; q < ZT; ++q) acc[v][q] = 0.0f;
+
+    const int ii0 = J.ii0;
+    const int iiE = min(J.ii1, max(g.XN, g.YN));  // block-uniform
+    const int ng = (iiE - ii0 + S - 1) / S;
+    auto stage = [&](int grp, int bo, int* w0) __attribute__((always_inline)) {
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int ii = ii0 + grp * S + s;
+            const float r = slab_r(slab_obj(ii, ocSB, g.dx), pm.a, pm.f, pm.b);
+            const int c = __builtin_amdgcn_readfirstlane(texel(slab_coord(r, g.invdx, ocKB), NkB));
+            w0[s] = max(0, min(c - WP / 2, KB - WP));
+            if (ii >= iiE || ii >= NsB) continue;
+#pragma unroll
+            for (int q = 0; q < ZT; ++q)
+#pragma unroll
+                for (int cp = 0; cp < CP; ++cp) {
+                    if ((((s * ZT + q) * CP + cp) & 3) != wave) continue;
+                    const int e0 = w0[s] + cp * 64 * PER;
+                    stage_row(tabB + ((size_t)ii * ZT + q) * KB + e0,
+                              &buf[bo + (s * ZT + q) * WP + cp * 64 * PER], KB - e0, lane);
+                }
+        }
+    };
+    auto sum = [&](int grp, int bo, const int* w0) __attribute__((always_inline)) {
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int ii = ii0 + grp * S + s;
+            if (ii >= iiE) break;
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                const bool xv = (xs >> v) & 1u;
+                if (ii < (xv ? g.XN : g.YN)) {  // uniform across the wave
+                    const int Nk = xv ? g.YN : g.XN;
+                    const float ocS = xv ? g.ocxP : g.ocyP, ocK = xv ? g.ocyP : g.ocxP;
+                    const int K = Nk + 1;
+                    const float r = slab_r(slab_obj(ii, ocS, g.dx), pa[v], pf[v], pbb[v]);
+                    const Fetch f = fetch_pos<E>(slab_coord(r, g.invdx, ocK), Nk);
+                    const float invlen = __builtin_amdgcn_rcpf(r - lane_prev(r));
+                    const int li = (int)(f.off / (unsigned)sizeof(E)) - w0[s];
+                    E e[ZT];
+                    if (((xs ^ xsB) >> v & 1u) == 0u && (unsigned)li < (unsigned)WP) {
+#pragma unroll
+                        for (int q = 0; q < ZT; ++q) e[q] = lds_entry(&buf[bo + (s * ZT + q) * WP + li]);
+                    } else {
+                        const E* row = (xv ? baseY : baseX) + (size_t)ii * ZT * K;
+#pragma unroll
+                        for (int q = 0; q < ZT; ++q) e[q] = entry_at(row + (size_t)q * K, f.off);
+                    }
+#pragma unroll
+                    for (int q = 0; q < ZT; ++q) {
+                        const Samp<MODE> T(e[q], f.t);
+                        acc[v][q] = fmaf(T.minus_prev_lane(), invlen, acc[v][q]);
+                    }
+                }
+            }
+        }
+    };
+    constexpr int HB = ROWS * WP;  // second half
+    int wA[S], wB[S];
+    if (ng > 0) {
+        stage(0, 0, wA);
+        wait_vmem();
+        __syncthreads();
+    }
+    for (int grp = 0; grp < ng; grp += 2) {
+        if (grp + 1 < ng) stage(grp + 1, HB, wB);
+        sum(grp, 0, wA);
+        wait_vmem();
+        __syncthreads();
+        if (grp + 1 >= ng) break;
+        if (grp + 2 < ng) stage(grp + 2, 0, wA);
+        sum(grp + 1, HB, wB);
+        wait_vmem();
+        __syncthreads();
+    }
+    if (!act) return;
+    proj_store<ZT, V>(J, g, sub, jg, u, z0, acc, cval);
 }
 
 // ------------------------------------------------------------------ index traces
@@ -990,8 +1165,8 @@ hipError_t launch_sat_sino(int mode, int ZT, int DNU, int SLN, int nzc, int PNs,
 
 // Projector launches: jobs of one slice chunk ZT go together, kBatch at a time;
 // pass p of every job that still has slabs left shares one launch.
-static hipError_t launch_proj_group(int mode, int ZT, int V, const ProjJob* const* jobs, int n,
-                                    hipStream_t st) {
+static hipError_t launch_proj_group(int mode, int ZT, int V, int stage,
+                                    const ProjJob* const* jobs, int n, hipStream_t st) {
     int maxPass = 0;
     int T[kBatch], Nmax[kBatch];
     for (int i = 0; i < n; ++i) {
@@ -1014,6 +1189,7 @@ static hipError_t launch_proj_group(int mode, int ZT, int V, const ProjJob* cons
             d.nzc = J.nzc;
             d.nug = (J.g.DNU + 62) / 63;  // 63 detectors per wave (64 boundaries)
             d.nvg = (J.sub.count + V - 1) / V;
+            if (stage) d.nvg = (d.nvg + 3) / 4;  // blocks of four view groups
             d.blk0 = (int)nw;
             d.ii0 = ii0;
             d.ii1 = ii0 + T[i];
@@ -1037,8 +1213,12 @@ static hipError_t launch_proj_group(int mode, int ZT, int V, const ProjJob* cons
         L.njobs = m;
         L.nwork = (int)nw;
         dim3 grid((unsigned)xcd_grid(L.nwork));
-#define DDF_PROJ(VV) \
-    DDF_MODE_SWITCH(mode, DDF_ZT_SWITCH(ZT, (k_proj<MC, ZTC, VV><<<grid, 64, 0, st>>>(L))))
+#define DDF_PROJ(VV)                                                                       \
+    if (stage) {                                                                           \
+        DDF_MODE_SWITCH(mode, DDF_ZT_SWITCH(ZT, (k_proj_lds<MC, ZTC, VV><<<grid, 256, 0, st>>>(L)))) \
+    } else {                                                                               \
+        DDF_MODE_SWITCH(mode, DDF_ZT_SWITCH(ZT, (k_proj<MC, ZTC, VV><<<grid, 64, 0, st>>>(L))))      \
+    }
         if (V == 4) {
             DDF_PROJ(4);
         } else if (V == 2) {
@@ -1059,32 +1239,35 @@ static hipError_t launch_proj_group(int mode, int ZT, int V, const ProjJob* cons
 // never changes a job's arithmetic.
 hipError_t launch_proj_batch(int mode, const ProjJob* jobs, int n, hipStream_t st,
                              long long fillWaves) {
-    for (int V : {1, 2, 4}) {
-        for (int ZT : {1, 2, 4, 8, 16}) {
-            const ProjJob* grp[kBatch];
-            int m = 0;
-            long long work = 0;
-            for (int i = 0; i < n; ++i) {
-                const ProjJob& J = jobs[i];
-                const int Vj = J.V == 2 || J.V == 4 ? J.V : 1;
-                if (J.ZT != ZT || Vj != V || J.sub.count <= 0) continue;
-                const long long w =
-                    (long long)((J.g.DNU + 62) / 63) * ((J.sub.count + V - 1) / V) * J.nzc;
-                if (m == kBatch || (m > 0 && work + w > fillWaves)) {
-                    const hipError_t e = launch_proj_group(mode, ZT, V, grp, m, st);
-                    if (e != hipSuccess) return e;
-                    m = 0;
-                    work = 0;
+    for (int i = 0; i < n; ++i)
+        if (jobs[i].stage != 0 && jobs[i].stage != 1) return hipErrorInvalidValue;
+    for (int stage : {0, 1})
+        for (int V : {1, 2, 4}) {
+            for (int ZT : {1, 2, 4, 8, 16}) {
+                const ProjJob* grp[kBatch];
+                int m = 0;
+                long long work = 0;
+                for (int i = 0; i < n; ++i) {
+                    const ProjJob& J = jobs[i];
+                    const int Vj = J.V == 2 || J.V == 4 ? J.V : 1;
+                    if (J.ZT != ZT || Vj != V || J.stage != stage || J.sub.count <= 0) continue;
+                    const long long w =
+                        (long long)((J.g.DNU + 62) / 63) * ((J.sub.count + V - 1) / V) * J.nzc;
+                    if (m == kBatch || (m > 0 && work + w > fillWaves)) {
+                        const hipError_t e = launch_proj_group(mode, ZT, V, stage, grp, m, st);
+                        if (e != hipSuccess) return e;
+                        m = 0;
+                        work = 0;
+                    }
+                    grp[m++] = &J;
+                    work += w;
                 }
-                grp[m++] = &J;
-                work += w;
-            }
-            if (m) {
-                const hipError_t e = launch_proj_group(mode, ZT, V, grp, m, st);
-                if (e != hipSuccess) return e;
+                if (m) {
+                    const hipError_t e = launch_proj_group(mode, ZT, V, stage, grp, m, st);
+                    if (e != hipSuccess) return e;
+                }
             }
         }
-    }
     return hipSuccess;
 }
 
@@ -1092,10 +1275,10 @@ hipError_t launch_proj(int mode, int ZT, int epi, const GeomDev& g, int nzc, con
                        const float* bx, const float* by, const float* xds, const float* yds,
                        const void* satX, const void* satY, Subset sub, float* out,
                        const float* meas, const float* rowSum, int slabTile, int viewsPerWave,
-                       hipStream_t st) {
+                       int stage, hipStream_t st) {
     if (sub.count <= 0) return hipSuccess;
-    const ProjJob j{g, sub, nzc, ZT, slabTile, epi, viewsPerWave, cossin, bx, by, xds, yds, satX, satY,
-                    out, meas, rowSum};
+    const ProjJob j{g,   sub,  nzc, ZT,   slabTile, epi,  viewsPerWave, cossin, bx,    by,
+                    xds, yds,  satX, satY, out,     meas, rowSum,       stage};
     return launch_proj_batch(mode, &j, 1, st);
 }
 

@@ -64,7 +64,7 @@ int ossart_prepare(OsJob& J, hipStream_t st) {
         HIPCALL(launch_proj(mode, 1, PROJ_PLAIN, g0, 1, P->cs0.as<float2>(), P->bx.as<float>(),
                             P->by.as<float>(), P->xds.as<float>(), P->yds.as<float>(), P->satX0.p,
                             P->satY0.p, sub, P->rowSum.as<float>() + (size_t)k * maxc * DNU,
-                            nullptr, nullptr, 0, 1, st));
+                            nullptr, nullptr, 0, 1, 0, st));
         HIPCALL(launch_sat_sino(mode, 1, DNU, 1, 1, J.cnt[k], P->onesP.as<float>(), P->satP0.p, st));
         HIPCALL(launch_back(mode, 1, BACK_PLAIN, g0, 1, P->cs0.as<float2>(), P->mask.as<uint8_t>(),
                             P->satP0.p, sub, P->colSum.as<float>() + (size_t)k * npix, nullptr,
@@ -122,7 +122,8 @@ hipError_t enqueue_batch_iteration(OsJob* const* jobs, int n, int mode, const Ba
                                  P->cs.as<float2>(),
                                  P->bx.as<float>(), P->by.as<float>(), P->xds.as<float>(),
                                  P->yds.as<float>(), P->satX.p, P->satY.p, P->rbuf.as<float>(),
-                                 J.meas, P->rowSum.as<float>() + (size_t)k * J.maxc * g.DNU});
+                                 J.meas, P->rowSum.as<float>() + (size_t)k * J.maxc * g.DNU,
+                                 P->projStage});
             ss.push_back(SatSinoJob{g.DNU, g.SLN, P->nzcb, P->ZTb, J.cnt[k], P->rbuf.as<float>(),
                                     P->satP.p});
             bj.push_back(BackJob{g, sub, P->nzcb, P->ZTb, epi, P->cs.as<float2>(),

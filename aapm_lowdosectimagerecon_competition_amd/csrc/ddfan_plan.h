@@ -68,11 +68,11 @@ struct DBuf {
 int check_device(int* dev);
 int upload(DBuf& b, const void* src, size_t bytes, hipStream_t st);
 
-inline size_t sat_vol_bytes(int mode, int nzc, int ZT, int XN, int YN) {
-    return sat_elem_bytes(mode) * (size_t)nzc * ZT * (size_t)(XN + 1) * YN;
-}
-// Sinogram tables carry 256 bytes of slack: the staged backprojector copies
+// Tables carry 256 bytes of slack: the staged projector and backprojector copy
 // rows in 16-byte pieces, which may end one 8-byte f32 entry past a row.
+inline size_t sat_vol_bytes(int mode, int nzc, int ZT, int XN, int YN) {
+    return sat_elem_bytes(mode) * (size_t)nzc * ZT * (size_t)(XN + 1) * YN + 256;
+}
 inline size_t sat_sino_bytes(int mode, int nzc, int ZT, int DNU, int views) {
     return sat_elem_bytes(mode) * (size_t)nzc * ZT * (size_t)(DNU + 1) * views + 256;
 }
@@ -90,6 +90,7 @@ struct ddfan_plan {
     int slabTile = 0;      // projector slabs per pass (0: all slabs in one pass)
     int viewsPerWave = 4;  // projector views per wave (1, 2 or 4)
     int backStage = 0;     // backprojector views per LDS-staged group (0: per-lane gathers)
+    int projStage = 0;     // 1: projector reads LDS-staged table rows (0: per-lane gathers)
     std::vector<float> h_hangs, h_cs;
     ddf::DBuf cs, bx, by, xds, yds, mask;
     ddf::DBuf satX, satY, satP;

@@ -2,15 +2,15 @@
 //
 // For one plan shape and one launch size (views per operator call) it times,
 // with HIP events on the plan's stream, the projector over slice chunks x
-// views per wave (one slab pass), then slab tilings of the winner, and the
-// backprojector over slice chunks x LDS-staged view groups, and keeps the
-// fastest.  Results live in
+// views per wave x LDS staging (one slab pass), then slab tilings of the
+// winner, and the backprojector over slice chunks x LDS-staged view groups,
+// and keeps the fastest.  Results live in
 // a process-wide table keyed by (SAT mode, XN, YN, SLN, DNU, widest uniform
 // chunk, views), which can be saved to and loaded from a text file and is
 // consulted by ddfan_plan_apply_tuned (the request loop applies it to every
 // admitted reconstruction).  No parameter changes results: slice chunks,
-// views per wave and staged groups give bit-identical output, slab passes
-// change only the summation grouping of the projector.
+// views per wave and staging give bit-identical output, slab passes change
+// only the summation grouping of the projector.
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 
@@ -34,7 +34,7 @@ struct TuneKey {
     }
 };
 struct TuneVal {
-    int ZT, V, tile, ZTb, stage;
+    int ZT, V, tile, ZTb, stage, pstage;
     double projMs, backMs;
 };
 
@@ -55,6 +55,7 @@ int apply(ddfan_plan* P, const TuneVal& v) {
     TRY(ddfan_plan_set_param(P, DDFAN_PARAM_SLAB_TILE, v.tile));
     TRY(ddfan_plan_set_param(P, DDFAN_PARAM_BACK_CHUNK, v.ZTb));
     TRY(ddfan_plan_set_param(P, DDFAN_PARAM_BACK_STAGE, v.stage));
+    TRY(ddfan_plan_set_param(P, DDFAN_PARAM_PROJ_STAGE, v.pstage));
     return DDFAN_OK;
 }
 
@@ -120,7 +121,8 @@ int ddfan_plan_autotune(ddfan_plan* P, int views, int reps, int flags, int* out_
         HIPCALL(launch_fill(vol.as<float>(), 1.0f, nvox, P->stream));
         HIPCALL(launch_fill(prj.as<float>(), 1.0f, nprj, P->stream));
         // the plan's own settings are restored if tuning fails part-way
-        const TuneVal saved{P->ZT, P->viewsPerWave, P->slabTile, P->ZTb, P->backStage, 0.0, 0.0};
+        const TuneVal saved{P->ZT,        P->viewsPerWave, P->slabTile, P->ZTb,
+                            P->backStage, P->projStage,    0.0,         0.0};
         int rc = [&]() -> int {
             std::vector<int> chunks;
             for (int z = std::min(P->ZTmax, 16); z >= 1; z >>= 1) chunks.push_back(z);
@@ -136,18 +138,23 @@ int ddfan_plan_autotune(ddfan_plan* P, int views, int reps, int flags, int* out_
                 TRY(ddfan_plan_set_param(P, DDFAN_PARAM_PROJ_CHUNK, z));
                 for (int v : {4, 2, 1}) {
                     TRY(ddfan_plan_set_param(P, DDFAN_PARAM_VIEWS_PER_WAVE, v));
-                    double ms;
-                    TRY(time_ms(P->stream, reps, proj, &ms));
-                    if (ms < bestP) {
-                        bestP = ms;
-                        best.ZT = z;
-                        best.V = v;
-                        best.tile = 0;
+                    for (int ps : {1, 0}) {
+                        TRY(ddfan_plan_set_param(P, DDFAN_PARAM_PROJ_STAGE, ps));
+                        double ms;
+                        TRY(time_ms(P->stream, reps, proj, &ms));
+                        if (ms < bestP) {
+                            bestP = ms;
+                            best.ZT = z;
+                            best.V = v;
+                            best.tile = 0;
+                            best.pstage = ps;
+                        }
                     }
                 }
             }
             TRY(ddfan_plan_set_param(P, DDFAN_PARAM_PROJ_CHUNK, best.ZT));
             TRY(ddfan_plan_set_param(P, DDFAN_PARAM_VIEWS_PER_WAVE, best.V));
+            TRY(ddfan_plan_set_param(P, DDFAN_PARAM_PROJ_STAGE, best.pstage));
             const int nmax = std::max(g.XN, g.YN);
             for (int t : {256, 128, 64}) {
                 if (t >= nmax) continue;
@@ -194,6 +201,7 @@ int ddfan_plan_autotune(ddfan_plan* P, int views, int reps, int flags, int* out_
         out_params[2] = best.tile;
         out_params[3] = best.ZTb;
         out_params[4] = best.stage;
+        out_params[5] = best.pstage;
     }
     if (out_ms) {
         out_ms[0] = best.projMs;
@@ -216,21 +224,22 @@ int ddfan_plan_apply_tuned(ddfan_plan* P, int views, int* found) {
     return apply(P, v);
 }
 
-// One line per entry:
-// mode XN YN SLN DNU ZTmax views  ZT V tile ZTb stage  proj_ms back_ms
+// One line per entry (bstage: backprojector staged views, pstage: projector
+// staging):
+// mode XN YN SLN DNU ZTmax views  ZT V tile ZTb bstage pstage  proj_ms back_ms
 int ddfan_tune_save(const char* path) {
     CHECK(path, DDFAN_EINVAL, "path is NULL");
     FILE* f = fopen(path, "w");
     CHECK(f, DDFAN_EINVAL, "cannot write %s", path);
-    fprintf(f, "# ddfan launch-shape table: mode XN YN SLN DNU ZTmax views  ZT V tile ZTb stage  "
-               "proj_ms back_ms\n");
+    fprintf(f, "# ddfan launch-shape table: mode XN YN SLN DNU ZTmax views  ZT V tile ZTb bstage "
+               "pstage  proj_ms back_ms\n");
     std::lock_guard<std::mutex> lk(g_tuneMu);
     for (const auto& kv : g_tune) {
         const TuneKey& k = kv.first;
         const TuneVal& v = kv.second;
-        fprintf(f, "%d %d %d %d %d %d %d  %d %d %d %d %d  %.4f %.4f\n", k.mode, k.XN, k.YN,
-                k.SLN, k.DNU, k.ZTmax, k.views, v.ZT, v.V, v.tile, v.ZTb, v.stage, v.projMs,
-                v.backMs);
+        fprintf(f, "%d %d %d %d %d %d %d  %d %d %d %d %d %d  %.4f %.4f\n", k.mode, k.XN, k.YN,
+                k.SLN, k.DNU, k.ZTmax, k.views, v.ZT, v.V, v.tile, v.ZTb, v.stage, v.pstage,
+                v.projMs, v.backMs);
     }
     const int bad = ferror(f);
     fclose(f);
@@ -257,17 +266,17 @@ int ddfan_tune_load(const char* path, int* entries) {
             ++tokens;
             while (*c && *c != ' ' && *c != '\t' && *c != '\n' && *c != '\r') ++c;
         }
-        if (tokens != 14 ||
-            sscanf(line, "%d %d %d %d %d %d %d %d %d %d %d %d %lf %lf", &k.mode, &k.XN, &k.YN,
+        if (tokens != 15 ||
+            sscanf(line, "%d %d %d %d %d %d %d %d %d %d %d %d %d %lf %lf", &k.mode, &k.XN, &k.YN,
                    &k.SLN, &k.DNU, &k.ZTmax, &k.views, &v.ZT, &v.V, &v.tile, &v.ZTb, &v.stage,
-                   &v.projMs, &v.backMs) != 14) {
+                   &v.pstage, &v.projMs, &v.backMs) != 15) {
             bad = lineNo;
             break;
         }
         const bool pow2 = v.ZT >= 1 && v.ZT <= 16 && (v.ZT & (v.ZT - 1)) == 0 && v.ZTb >= 1 &&
                           v.ZTb <= 16 && (v.ZTb & (v.ZTb - 1)) == 0;
         if (!pow2 || v.ZT > k.ZTmax || v.ZTb > k.ZTmax || !(v.V == 1 || v.V == 2 || v.V == 4) ||
-            v.tile < 0 || !stage_fits(v.stage, v.ZTb)) {
+            v.tile < 0 || !stage_fits(v.stage, v.ZTb) || (v.pstage != 0 && v.pstage != 1)) {
             bad = lineNo;
             break;
         }

@@ -139,6 +139,15 @@ class Plan:
         self.set_param(N.PARAM_BACK_CHUNK, v)
 
     @property
+    def proj_stage(self):
+        """1: projector table rows staged through LDS (0: per-lane table reads)."""
+        return self.get_param(N.PARAM_PROJ_STAGE)
+
+    @proj_stage.setter
+    def proj_stage(self, v):
+        self.set_param(N.PARAM_PROJ_STAGE, v)
+
+    @property
     def back_stage(self):
         """Views per LDS-staged backprojector group (0: per-lane table reads)."""
         return self.get_param(N.PARAM_BACK_STAGE)
@@ -150,12 +159,13 @@ class Plan:
     def autotune(self, views=None, reps=3, use_cache=True) -> dict:
         """Times launch shapes for calls of `views` views (default PN), sets
         the fastest and records it (see tune_save / tune_load)."""
-        prm = (ctypes.c_int * 5)()
+        prm = (ctypes.c_int * 6)()
         ms = (ctypes.c_double * 2)()
         N.call("ddfan_plan_autotune", self.handle, int(views or 0), int(reps),
                N.TUNE_USE_CACHE if use_cache else 0, prm, ms)
         return {"proj_chunk": prm[0], "views_per_wave": prm[1], "slab_tile": prm[2],
-                "back_chunk": prm[3], "back_stage": prm[4], "proj_ms": ms[0], "back_ms": ms[1]}
+                "back_chunk": prm[3], "back_stage": prm[4], "proj_stage": prm[5],
+                "proj_ms": ms[0], "back_ms": ms[1]}
 
     def apply_tuned(self, views=None) -> bool:
         found = ctypes.c_int()

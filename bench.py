@@ -455,6 +455,7 @@ def main():
                                        "slab_tile": plan.slab_tile,
                                        "back_chunk": plan.back_chunk,
                                        "back_stage": plan.back_stage,
+                                       "proj_stage": plan.proj_stage,
                                        "autotuned": tuned is not None}},
            "roofline": roof, "kernels_ms": kt}
 

@@ -158,16 +158,21 @@ int ddfan_plan_query(const ddfan_plan* plan, int* slice_chunk, int* n_chunks, in
  * are bit-identical for every value.  Narrowing the back chunk halves it as
  * needed. */
 #define DDFAN_PARAM_BACK_STAGE 5
+/* DDFAN_PARAM_PROJ_STAGE: 1 = the projector copies each slab's table run
+ * into LDS for four view groups at a time; 0 = every lane reads the table in
+ * global memory.  Results are bit-identical. */
+#define DDFAN_PARAM_PROJ_STAGE 6
 int ddfan_plan_set_param(ddfan_plan* plan, int param, int value);
 int ddfan_plan_get_param(const ddfan_plan* plan, int param, int* value);
-/* Autotuner: times the projector over slice chunks x views per wave x slab
- * tilings and the backprojector over slice chunks x staged view groups, for
- * launches of `views`
+/* Autotuner: times the projector over slice chunks x views per wave x LDS
+ * staging, then slab tilings, and the backprojector over slice chunks x
+ * staged view groups, for launches of `views`
  * views (0: the plan's PN) with this plan's shape, sets the fastest on the
  * plan and records it in a process-wide table (key: SAT mode, XN, YN, SLN,
  * DNU, widest uniform chunk, views).  flags DDFAN_TUNE_USE_CACHE takes a
- * recorded entry without timing.  out_params (may be NULL, else 5 ints)
- * receives {proj chunk, views per wave, slab tile, back chunk, back stage};
+ * recorded entry without timing.  out_params (may be NULL, else 6 ints)
+ * receives {proj chunk, views per wave, slab tile, back chunk, back stage,
+ * proj stage};
  * out_ms (may be NULL)
  * the projector and backprojector milliseconds per call.  No parameter
  * changes results beyond the projector's slab-pass summation grouping. */

@@ -142,20 +142,22 @@ def test_launch_shape_table_round_trip(tmp_path):
     dd.tune_clear()
     src = tmp_path / "shapes.txt"
     src.write_text("# comment\n"
-                   "1 512 512 64 736 16 2304  8 4 0 16 1  29.9 20.3\n"
-                   "0 256 256 1 736 1 48  1 2 128 1 4  0.31 0.40\n")
+                   "1 512 512 64 736 16 2304  4 4 0 16 1 1  20.6 19.3\n"
+                   "0 256 256 1 736 1 48  1 2 128 1 4 0  0.31 0.40\n")
     assert dd.tune_load(str(src)) == 2
     out = tmp_path / "out.txt"
     dd.tune_save(str(out))
-    rows = sorted(l.split()[:12] for l in out.read_text().splitlines() if not l.startswith("#"))
-    assert rows == sorted([["1", "512", "512", "64", "736", "16", "2304", "8", "4", "0", "16", "1"],
-                           ["0", "256", "256", "1", "736", "1", "48", "1", "2", "128", "1", "4"]])
+    rows = sorted(l.split()[:13] for l in out.read_text().splitlines() if not l.startswith("#"))
+    assert rows == sorted(
+        [["1", "512", "512", "64", "736", "16", "2304", "4", "4", "0", "16", "1", "1"],
+         ["0", "256", "256", "1", "736", "1", "48", "1", "2", "128", "1", "4", "0"]])
     bad = tmp_path / "bad.txt"
-    for line in ("1 512 512 64 736 16 2304  32 4 0 16 1  1 1",  # chunk 32 > widest 16
-                 "1 512 512 64 736 16 2304  8 3 0 16 1  1 1",   # 3 views per wave
-                 "1 512 512 64 736 16 2304  8 4 0 16 4  1 1",   # 4 staged views x chunk 16 > 32
-                 "1 512 512 64 736 16 2304  8 4 0 16 3  1 1",   # 3 staged views
-                 "1 512 512 64 736 16 2304  8 4 0 16  1.5 2"):  # no stage column
+    for line in ("1 512 512 64 736 16 2304  32 4 0 16 1 1  1 1",  # chunk 32 > widest 16
+                 "1 512 512 64 736 16 2304  8 3 0 16 1 1  1 1",   # 3 views per wave
+                 "1 512 512 64 736 16 2304  8 4 0 16 4 1  1 1",   # 4 staged views x chunk 16 > 32
+                 "1 512 512 64 736 16 2304  8 4 0 16 3 1  1 1",   # 3 staged views
+                 "1 512 512 64 736 16 2304  8 4 0 16 1 2  1 1",   # projector stage 2
+                 "1 512 512 64 736 16 2304  8 4 0 16 1  1.5 2"):  # no projector stage column
         bad.write_text(line + "\n")
         with pytest.raises(N.DDFanError):
             dd.tune_load(str(bad))

@@ -1,7 +1,7 @@
 """Autotuner: the chosen launch shape is valid, is recorded and re-applied
 from the table, and changes no result (slice chunks, views per wave and
-staged backprojector groups are bit-identical; a slab tiling regroups the
-projector's sums, <= 1e-6 relative L2 against the untuned call)."""
+LDS staging are bit-identical; a slab tiling regroups the projector's sums,
+<= 1e-6 relative L2 against the untuned call)."""
 import numpy as np
 import pytest
 
@@ -31,9 +31,11 @@ def test_autotune_keeps_results_and_caches():
         assert t["views_per_wave"] in (1, 2, 4) and t["slab_tile"] in (0, 64)
         assert t["proj_ms"] > 0 and t["back_ms"] > 0
         assert t["back_stage"] in (0, 1, 2, 4) and t["back_stage"] * t["back_chunk"] <= 32
+        assert t["proj_stage"] in (0, 1)
         assert (plan.proj_chunk, plan.views_per_wave, plan.slab_tile, plan.back_chunk,
-                plan.back_stage) == (t["proj_chunk"], t["views_per_wave"], t["slab_tile"],
-                                     t["back_chunk"], t["back_stage"])
+                plan.back_stage, plan.proj_stage) == (t["proj_chunk"], t["views_per_wave"],
+                                                      t["slab_tile"], t["back_chunk"],
+                                                      t["back_stage"], t["proj_stage"])
         p1 = plan.proj_host(vol, first=1, stride=3, count=20)
         b1 = plan.back_host(p0, first=1, stride=3, count=20)
     if t["slab_tile"] == 0:
@@ -44,7 +46,7 @@ def test_autotune_keeps_results_and_caches():
     with dd.Plan(g, hangs, mask, sat_mode="twofloat") as plan2:
         assert plan2.apply_tuned(views=20)
         assert plan2.back_chunk == t["back_chunk"] and plan2.views_per_wave == t["views_per_wave"]
-        assert plan2.back_stage == t["back_stage"]
+        assert plan2.back_stage == t["back_stage"] and plan2.proj_stage == t["proj_stage"]
         assert not plan2.apply_tuned(views=30)
         again = plan2.autotune(views=20, use_cache=True)
         assert again["proj_chunk"] == t["proj_chunk"] and again["proj_ms"] == t["proj_ms"]
