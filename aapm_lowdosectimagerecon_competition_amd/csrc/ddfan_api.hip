@@ -265,12 +265,12 @@ int ddfan_plan_create(const ddfan_geom* G, const float* hangs, int PN, const uns
         P->mode = sat_mode;
         P->ZTmax = choose_slice_chunk(hangs, PN, SLN, slice_chunk);
         // Launch-shape defaults measured on MI355X at 512x512x64, 2304 views:
-        // the projector stages its table rows through LDS on chunks of 4
-        // slices (8 with f32 tables) with 4 views per wave in one slab pass
-        // (narrower chunks read per lane), the backprojector runs on the widest
-        // chunk with its detector windows staged one view at a time (narrow
-        // chunks stage 4 / chunk views per group, so each wave copies a row).
-        P->ZT = std::min(P->ZTmax, sat_mode == SAT_F32 ? 8 : 4);
+        // the projector stages its table rows through LDS on chunks of 8
+        // slices with 4 views per wave in one slab pass (narrower chunks read
+        // per lane), the backprojector runs on the widest chunk with its
+        // detector windows staged one view at a time (narrow chunks stage
+        // 4 / chunk views per group, so each wave copies a row).
+        P->ZT = std::min(P->ZTmax, 8);
         P->projStage = P->ZT >= 4 ? 1 : 0;
         P->nzc = (SLN + P->ZT - 1) / P->ZT;
         P->ZTb = P->ZTmax;

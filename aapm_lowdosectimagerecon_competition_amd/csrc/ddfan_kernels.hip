@@ -729,11 +729,15 @@ void k_back_lds(const BackLaunch L) {
 // orientation differs from the middle view's, and lanes whose entry falls
 // outside the run, read the table in global memory.  Arithmetic and summation
 // order are those of k_proj, so the results are bit-identical.
-#ifndef DDF_PROJ_LDS_WAVES
-#define DDF_PROJ_LDS_WAVES 5
-#endif
+// Register budget: 5 waves per SIMD (the projector's), fewer where the ZT x V
+// accumulators and loads would spill.  Measured at 512x512x64, 2304 views,
+// chunk 8 x 4 views per wave: two-float tables 16.7 ms at 4 waves vs 22.6 ms at
+// 5 (spilling); f32 tables 10.5 ms at 5 waves (a 16-byte spill) vs 11.5 at 4.
+constexpr int proj_lds_waves(int MODE, int ZT, int V) {
+    return ZT * V >= 64 ? 2 : (ZT * V >= 32 && MODE != SAT_F32) ? 4 : 5;
+}
 template <int MODE, int ZT, int V>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DDF_PROJ_LDS_WAVES)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(proj_lds_waves(MODE, ZT, V))))
 void k_proj_lds(const ProjLaunch L) {
     using E = typename Sat<MODE>::E;
     constexpr int WP = 128;                          // entries per staged run
@@ -843,8 +847,15 @@ void k_proj_lds(const ProjLaunch L) {
                     const Fetch f = fetch_pos<E>(slab_coord(r, g.invdx, ocK), Nk);
                     const float invlen = __builtin_amdgcn_rcpf(r - lane_prev(r));
                     const int li = (int)(f.off / (unsigned)sizeof(E)) - w0[s];
+                    // Lanes outside the run (or views of the other slab orientation)
+                    // read global memory; the sums run after the branch with every
+                    // lane active, as the DPP neighbour read needs.  (A wave-uniform
+                    // choice with the sums duplicated in both paths, which avoids the
+                    // wait for in-flight copies before the LDS reads, measured slower:
+                    // 21.5 vs 20.6 ms at chunk 4, 17.9 vs 16.7 ms at chunk 8.)
+                    const bool inw = ((xs ^ xsB) >> v & 1u) == 0u && (unsigned)li < (unsigned)WP;
                     E e[ZT];
-                    if (((xs ^ xsB) >> v & 1u) == 0u && (unsigned)li < (unsigned)WP) {
+                    if (inw) {
 #pragma unroll
                         for (int q = 0; q < ZT; ++q) e[q] = lds_entry(&buf[bo + (s * ZT + q) * WP + li]);
                     } else {

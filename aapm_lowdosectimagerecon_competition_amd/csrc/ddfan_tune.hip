@@ -1,6 +1,7 @@
 // ddfan_tune.hip -- launch-shape autotuner.
 //
-// For one plan shape and one launch size (views per operator call) it times,
+// For one plan shape and one launch size (views per operator call, spaced
+// PN / views apart like an OS-SART subset) it times,
 // with HIP events on the plan's stream, the projector over slice chunks x
 // views per wave x LDS staging (one slab pass), then slab tilings of the
 // winner, and the backprojector over slice chunks x LDS-staged view groups,
@@ -126,11 +127,15 @@ int ddfan_plan_autotune(ddfan_plan* P, int views, int reps, int flags, int* out_
         int rc = [&]() -> int {
             std::vector<int> chunks;
             for (int z = std::min(P->ZTmax, 16); z >= 1; z >>= 1) chunks.push_back(z);
+            // launches of `views` views spaced PN / views apart: an OS-SART
+            // subset (consecutive views when views = PN); the spacing decides
+            // how much of a table row neighbouring views of a launch share
+            const int stride = std::max(1, P->PN / views);
             auto proj = [&]() {
-                return ddfan_plan_proj(P, vol.as<float>(), prj.as<float>(), 0, 1, views);
+                return ddfan_plan_proj(P, vol.as<float>(), prj.as<float>(), 0, stride, views);
             };
             auto back = [&]() {
-                return ddfan_plan_back(P, prj.as<float>(), vol.as<float>(), 0, 1, views);
+                return ddfan_plan_back(P, prj.as<float>(), vol.as<float>(), 0, stride, views);
             };
             double bestP = 1e30;
             TRY(ddfan_plan_set_param(P, DDFAN_PARAM_SLAB_TILE, 0));

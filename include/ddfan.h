@@ -167,7 +167,8 @@ int ddfan_plan_get_param(const ddfan_plan* plan, int param, int* value);
 /* Autotuner: times the projector over slice chunks x views per wave x LDS
  * staging, then slab tilings, and the backprojector over slice chunks x
  * staged view groups, for launches of `views`
- * views (0: the plan's PN) with this plan's shape, sets the fastest on the
+ * views (0: the plan's PN) spaced PN / views apart, as in an OS-SART subset,
+ * with this plan's shape, sets the fastest on the
  * plan and records it in a process-wide table (key: SAT mode, XN, YN, SLN,
  * DNU, widest uniform chunk, views).  flags DDFAN_TUNE_USE_CACHE takes a
  * recorded entry without timing.  out_params (may be NULL, else 6 ints)
