@@ -11,6 +11,8 @@ from . import _native
 from ._native import DDFanError, DeviceBuffer, device_count
 from .batch import Batch
 from .dd2 import dd2_multi_slices
+from .dicom import (ImageCfg, ReconCfg, collect_image_cfg, collect_recon_cfg,
+                    convert_to_recon_conf, read_proj, read_proj_frames)
 from .geometry import Conf, detector_arc, detector_geometry, recon_mask, subset_views
 from .ossart import ossart_aapm
 from .plan import Plan, plan_footprint, tune_clear, tune_load, tune_save
@@ -18,7 +20,9 @@ from .rebin import HelicalCfg, helical_to_fan_func, helical_to_fan_routine
 from .serve import Result, Server
 
 __all__ = [
-    "Batch", "Conf", "DDFanError", "DeviceBuffer", "HelicalCfg", "Plan", "dd2_multi_slices",
+    "Batch", "Conf", "DDFanError", "DeviceBuffer", "HelicalCfg", "ImageCfg", "Plan", "ReconCfg",
+    "collect_image_cfg", "collect_recon_cfg", "convert_to_recon_conf", "dd2_multi_slices",
+    "read_proj", "read_proj_frames",
     "detector_arc", "detector_geometry", "device_count", "helical_to_fan_func",
     "helical_to_fan_routine", "ossart_aapm", "plan_footprint", "recon_mask", "Result", "Server",
     "subset_views", "tune_clear", "tune_load", "tune_save",

@@ -29,6 +29,7 @@ DDFAN_EBOUNDS = -3
 DDFAN_EDEVICE = -4
 DDFAN_ENODEV = -5
 DDFAN_EBUSY = -6
+DDFAN_EIO = -7
 
 SAT_F32 = 0
 SAT_TWOFLOAT = 1
@@ -120,6 +121,10 @@ _SIGS = {
     "ddfan_tune_clear": [],
     "ddfan_helical_to_fan_dev": [_p, _p, _p, _i, _i, _f, _i, _ll, _p, _i, _f, _f, _f, _i, _i, _f,
                                  _f, _i, _f, _f, _p, _p],
+    "ddfan_ctpd_read_header": [ctypes.c_char_p, _p, _p, _i],
+    "ddfan_ct_image_read_header": [ctypes.c_char_p, _p],
+    "ddfan_dicom_read_u16": [ctypes.c_char_p, _i, _i, _p],
+    "ddfan_ctpd_read_frames": [ctypes.POINTER(ctypes.c_char_p), _i, _i, _i, _p, _i],
 }
 _RESTYPES = {"ddfan_status_string": ctypes.c_char_p, "ddfan_last_error": ctypes.c_char_p}
 

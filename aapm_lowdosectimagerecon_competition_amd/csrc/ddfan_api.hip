@@ -129,6 +129,7 @@ const char* ddfan_status_string(int s) {
         case DDFAN_EDEVICE: return "device error";
         case DDFAN_ENODEV: return "no device";
         case DDFAN_EBUSY: return "rejected by memory budget";
+        case DDFAN_EIO: return "file missing, unreadable or malformed";
         default: return "unknown status";
     }
 }

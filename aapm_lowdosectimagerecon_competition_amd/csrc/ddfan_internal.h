@@ -140,9 +140,9 @@ struct RebinDev {
 // rawKind 0: float raw line integrals; 1: uint16 counts converted to
 // -log(max(c,1)/(gain*I0)) on the fly; 2: uint16 line integrals times
 // countScale.  rawOrder 0: raw[(view*W+ch)*H+row] (the rebin's own input,
-// HelicalToFanFunc_mex.cpp:45,74); 1: per-view detector frames
-// raw[(f*H+row)*W+ch] in reverse view order, f = TotalViews-1-view (the
-// stored order readProj.m:43-54 permutes and flips).  outLayout 0: reference
+// HelicalToFanFunc_mex.cpp:45,74); 1: the same per-frame layout with the
+// frames in reverse view order, f = TotalViews-1-view (the projection files
+// in file order, which readProj.m:43-54 reads and flips).  outLayout 0: reference
 // layout Proj[(s*DefTimes+v)*W+j], Views[s*DefTimes+v]; 1: sinogram layout
 // prj[(v*W+j)*SLN+s], hangs[v*SLN+s].
 hipError_t launch_rebin(const RebinDev& p, int rawKind, int rawOrder, int outLayout,

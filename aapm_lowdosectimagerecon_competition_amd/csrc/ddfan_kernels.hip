@@ -986,8 +986,8 @@ __global__ __launch_bounds__(256) void k_rebin(const RebinDev p, const void* __r
         const float lambda = (float)(i - 1) * p.delta + p.BVAngle;
         angle = mod_f(lambda, p.twopi);
     }
-    // ORDER 0: raw[(view*W + ch)*H + row]; ORDER 1: frames raw[(f*H + row)*W + ch]
-    // stored in reverse view order, f = TotalViews - view (readProj.m:43-54)
+    // raw[(view*W + ch)*H + row]; ORDER 1: the frames in reverse view order,
+    // frame f = TotalViews - view (readProj.m:43-54)
     const long long rv = ORDER == 0 ? i - 1 - p.viewOffset : p.totalViews - i;
     const size_t viewBase = valid ? (size_t)rv * p.W * p.H : 0;
     float vals[MAXC];
@@ -1005,9 +1005,8 @@ __global__ __launch_bounds__(256) void k_rebin(const RebinDev p, const void* __r
                 if (idv >= 1 && idv <= p.H - 1) {
                     const float t = dv - (float)idv;
                     const float cosA = p.SD / sqrtf(p.SD * p.SD + v * v);
-                    const size_t o = ORDER == 0 ? viewBase + (size_t)jj * p.H + (idv - 1)
-                                                : viewBase + (size_t)(idv - 1) * p.W + jj;
-                    const size_t o1 = ORDER == 0 ? o + 1 : o + p.W;  // next detector row
+                    const size_t o = viewBase + (size_t)jj * p.H + (idv - 1);
+                    const size_t o1 = o + 1;  // next detector row
                     float d0, d1;
                     if constexpr (RAW == 0) {
                         const float* r = reinterpret_cast<const float*>(raw);

@@ -88,8 +88,9 @@ def helical_to_fan_device(d_raw, raw_kind, count_scale, layout, total_views, zPo
     """Device-resident rebin (raw and outputs are device buffers).  raw_kind:
     RAW_FLOAT, RAW_COUNTS_U16 (log(count_scale/max(c,1))) or
     RAW_LINE_INTEGRALS_U16 (value*count_scale); raw_order: ORDER_REBIN
-    (view, channel, row) or ORDER_FRAMES_REVERSED (stored detector frames
-    (view, row, channel), last view first, as readProj.m reads them)."""
+    (view, channel, row) or ORDER_FRAMES_REVERSED (the projection files'
+    frames in file order, (channel, row) each, last view first: readProj.m
+    flips them)."""
     z = np.ascontiguousarray(zPos, np.float32)
     N.call("ddfan_helical_to_fan_dev", N.ptr(d_out), N.ptr(d_views), N.ptr(d_raw), int(raw_kind),
            int(raw_order), float(count_scale), int(layout), int(total_views), N.ptr(z),

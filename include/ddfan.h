@@ -45,6 +45,7 @@ extern "C" {
 #define DDFAN_EDEVICE (-4) /* HIP runtime error */
 #define DDFAN_ENODEV (-5)  /* no usable GPU */
 #define DDFAN_EBUSY (-6)   /* request rejected by the HBM budget */
+#define DDFAN_EIO (-7)     /* file missing, unreadable or not in the expected format */
 
 /* Summed-area-table precision. */
 #define DDFAN_SAT_F32 0      /* FP32 sequential prefix: the reference's arithmetic */
@@ -295,10 +296,11 @@ int ddfan_plan_trace_back(ddfan_plan* plan, int first, int stride, int count, in
  * uint16 counts, converted to log(count_scale / max(count, 1)) on the fly;
  * 2: d_raw uint16 line integrals times count_scale (1: used unscaled, as
  * readProj.m:47 does).  raw_order 0: raw[(view*DetWidth+ch)*DetHeight+row]
- * (HelicalToFan's input); 1: the detector frames as stored,
- * raw[(f*DetHeight+row)*DetWidth+ch] with views in reverse order
- * (f = TotalViews-1-view), which readProj.m:52-53 permutes and flips -- the
- * gather reads them in place.
+ * (HelicalToFan's input); 1: the projection frames in file order, as the
+ * DICOM-CT-PD files store them (DetWidth rows of DetHeight samples each,
+ * raw[(f*DetWidth+ch)*DetHeight+row]), which readProj.m:43-53 reads into the
+ * same per-frame order and flips to the reverse view order
+ * (view = TotalViews-1-f) -- the gather reads them in place.
  * layout 0: reference layout (Proj[(s*DefTimes+v)*W+ch], Views[s*DefTimes+v]);
  * layout 1: sinogram layout (prj[(v*W+ch)*SLN+s], hangs[v*SLN+s]).
  * d_cut (optional) [SLN][DefTimes] receives 1 for rows zeroed as cut views;
@@ -309,6 +311,78 @@ int ddfan_helical_to_fan_dev(float* d_Proj, float* d_Views, const void* d_raw, i
                              int DetWidth, int DetHeight, float PerDetW, float PerDetH,
                              int DefTimes, float DetCenterW, float SpiralPitchFactor,
                              unsigned char* d_cut, int32_t* d_idv);
+
+/* ---- DICOM-CT-PD projection files and CT image headers (what readProj.m,
+ * CollectReconCfg.m and CollectImageCfg.m read through MATLAB's dicominfo /
+ * dicomread and the cov_* byte converters).  Little-endian files, explicit or
+ * implicit VR; private elements are matched by tag number and decoded from
+ * their bytes as the reference's converters do.  Host-only (no GPU). */
+
+/* Projection header, CollectReconCfg.m:5-92.  The fields marked (opt) are
+ * NaN (or empty) when the file lacks them; the others are required. */
+typedef struct ddfan_ctpd_header {
+    double HUCalibrationFactor;              /* (0018,0061) decimal string (opt) */
+    double DataCollectionDiameter;           /* (0018,0090) decimal string (opt) */
+    double SpiralPitchFactor;                /* (0018,9311) float64 */
+    int NumberofDetectorRows;                /* (7029,1010) uint16 */
+    int NumberofDetectorColumns;             /* (7029,1011) uint16 */
+    float DetectorElementTransverseSpacing;  /* (7029,1002) float32, mm */
+    float DetectorElementAxialSpacing;       /* (7029,1006) float32, mm */
+    char DetectorShape[17];                  /* (7029,100B) text (opt) */
+    float DetectorFocalCenterAngularPosition; /* (7031,1001) float32 */
+    float DetectorFocalCenterAxialPosition;   /* (7031,1002) float32 (opt) */
+    float DetectorFocalCenterRadialDistance;  /* (7031,1003) float32, mm */
+    float DetectorCentralElementX;            /* (7031,1033) first float32, 1-based */
+    float DetectorCentralElementY;            /* (7031,1033) second float32, 1-based */
+    float ConstantRadialDistance;             /* (7031,1031) float32, mm */
+    float SourceAngularPositionShift;         /* (7033,100B) float32 (opt) */
+    float SourceAxialPositionShift;           /* (7033,100C) float32 (opt) */
+    float SourceRadialDistanceShift;          /* (7033,100D) float32 (opt) */
+    int NumberofSourceAngularSteps;           /* (7033,1013) uint16 */
+    int Rows, Columns;                        /* (0028,0010/0011): the stored frame (0 if absent) */
+    int PhotonStatisticsCount;                /* float32 values in (7033,1065) */
+} ddfan_ctpd_header;
+/* Reads one file's header; photon_stats (capacity floats, may be NULL with
+ * capacity 0) receives the first values of (7033,1065). */
+int ddfan_ctpd_read_header(const char* path, ddfan_ctpd_header* out, float* photon_stats,
+                           int capacity);
+
+/* CT image header, CollectImageCfg.m:3-31 (MATLAB's dicominfo names).  Width,
+ * Height and ReconstructionDiameter are required; absent numbers are NaN
+ * (BitDepth and the pixel value bounds 0). */
+typedef struct ddfan_ct_image_header {
+    int Width, Height;                       /* (0028,0011) Columns, (0028,0010) Rows */
+    int BitDepth;                            /* (0028,0101) */
+    double SliceThickness;                   /* (0018,0050) */
+    double KVP;                              /* (0018,0060) */
+    double DataCollectionDiameter;           /* (0018,0090) */
+    double ReconstructionDiameter;           /* (0018,1100) */
+    double SingleCollimationWidth;           /* (0018,9306) */
+    double TotalCollimationWidth;            /* (0018,9307) */
+    double TableSpeed;                       /* (0018,9309) */
+    double TableFeedPerRotation;             /* (0018,9310) */
+    double SpiralPitchFactor;                /* (0018,9311) */
+    double DataCollectionCenterPatient[3];   /* (0018,9313) */
+    double ReconstructionTargetCenterPatient[3]; /* (0018,9318) */
+    double PixelSpacing[2];                  /* (0028,0030) */
+    int SmallestImagePixelValue;             /* (0028,0106) */
+    int LargestImagePixelValue;              /* (0028,0107) */
+    double WindowCenter[2], WindowWidth[2];  /* (0028,1050/1051), first two values */
+    double RescaleIntercept, RescaleSlope;   /* (0028,1052/1053) */
+    double PixelSize;                        /* ReconstructionDiameter / Width */
+} ddfan_ct_image_header;
+int ddfan_ct_image_read_header(const char* path, ddfan_ct_image_header* out);
+
+/* The 16-bit pixel data of one file (rows x cols samples, row-major, one
+ * frame, uncompressed) into out.  A file of another size is DDFAN_EIO. */
+int ddfan_dicom_read_u16(const char* path, int rows, int cols, uint16_t* out);
+/* The frames of nfiles files, in the given order, into out[f*rows*cols ...]
+ * with `threads` host threads (0: one per hardware thread, at most 64).  For
+ * DICOM-CT-PD projections rows = detector columns and cols = detector rows,
+ * so out is the rebin's raw input with the views reversed (raw_order 1 of
+ * ddfan_helical_to_fan_dev; readProj.m:41-53). */
+int ddfan_ctpd_read_frames(const char* const* paths, int nfiles, int rows, int cols,
+                           uint16_t* out, int threads);
 
 #ifdef __cplusplus
 }

@@ -96,16 +96,16 @@ def test_rebin_uint16_counts_fused(oracle):
 
 
 def test_rebin_reads_stored_frames_and_unscaled_integers(oracle):
-    """Raw ingestion (readProj.m:43-54): detector frames as stored
-    (view, row, channel), last view first, read in place, give the rebin of
-    the permuted and flipped array bit for bit; uint16 line integrals used
-    unscaled give the float rebin of the same values bit for bit."""
+    """Raw ingestion (readProj.m:43-54): the projection frames in file order
+    ((channel, row) each, last view first), read in place, give the rebin of
+    the flipped array bit for bit; uint16 line integrals used unscaled give the
+    float rebin of the same values bit for bit."""
     import aapm_lowdosectimagerecon_competition_amd as dd
     from aapm_lowdosectimagerecon_competition_amd import rebin as R
     rng = np.random.default_rng(11)
     ints = rng.integers(0, 5000, (DT * TURNS, W, H)).astype(np.uint16)  # [view, ch, row]
     flo = ints.astype(np.float32)
-    frames = np.ascontiguousarray(np.transpose(ints, (0, 2, 1))[::-1])  # [f, row, ch], reversed
+    frames = np.ascontiguousarray(ints[::-1])  # [f, ch, row], files in reverse view order
     z = _zpos(oracle, 6)
     SLN, TV = z.size, ints.shape[0]
     print("\n[rebin-ingest] stored frames / unscaled uint16 vs float input", flush=True)
@@ -120,8 +120,8 @@ def test_rebin_reads_stored_frames_and_unscaled_integers(oracle):
     ref_p, ref_v = run(flo, R.RAW_FLOAT, R.ORDER_REBIN)
     for raw, kind, order in ((ints, R.RAW_LINE_INTEGRALS_U16, R.ORDER_REBIN),
                              (frames, R.RAW_LINE_INTEGRALS_U16, R.ORDER_FRAMES_REVERSED),
-                             (np.ascontiguousarray(np.transpose(flo, (0, 2, 1))[::-1]),
-                              R.RAW_FLOAT, R.ORDER_FRAMES_REVERSED)):
+                             (np.ascontiguousarray(flo[::-1]), R.RAW_FLOAT,
+                              R.ORDER_FRAMES_REVERSED)):
         p, v = run(raw, kind, order)
         assert np.array_equal(p, ref_p), (kind, order)
         assert np.array_equal(v, ref_v)
