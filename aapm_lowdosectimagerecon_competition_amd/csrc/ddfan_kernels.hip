@@ -61,21 +61,26 @@ __device__ __forceinline__ double lane_prev(double v) {
 }
 
 // Fetch position of a texture coordinate c on a table with entries 0..N:
-// i0 = clamp(floor(c - 0.5), 0, N) as a 32-bit byte offset of entry type E,
-// and the weight t, forced to 0 where the clamp-to-edge fetch reads one texel
-// twice.  Rows are addressed as (uniform row base) + (32-bit lane offset) so
-// loads use the scalar-base addressing form.
+// i0 = clamp(floor(c - 0.5), 0, N) (also as a 32-bit byte offset of entry
+// type E) and the weight t = clamp(c - 0.5 - i0, 0, 1).  This is the
+// clamp-to-edge fetch: below the first texel t clamps to 0 and the entry
+// reads S[0]; at or above the last, v[N] = 0 makes t irrelevant; in between
+// t = (c - 0.5) - floor(c - 0.5) exactly; a NaN coordinate reads entry 0
+// with t = 0.  Rows are addressed as (uniform row base) + (32-bit lane
+// offset) so loads use the scalar-base addressing form.
 struct Fetch {
     unsigned off;
+    int i0;
     float t;
 };
 template <class E>
 __device__ __forceinline__ Fetch fetch_pos(float c, int N) {
-    const int i = texel(c, N);
-    const int i0 = min(max(i, 0), N);
+    const float cm = c - 0.5f;
+    const float fc = fminf(fmaxf(floorf(cm), 0.0f), (float)N);
     Fetch f;
-    f.off = (unsigned)i0 * (unsigned)sizeof(E);
-    f.t = (i < 0 || i >= N) ? 0.0f : (c - 0.5f) - (float)i;
+    f.i0 = (int)fc;
+    f.off = (unsigned)f.i0 * (unsigned)sizeof(E);
+    f.t = fminf(fmaxf(cm - fc, 0.0f), 1.0f);
     return f;
 }
 template <class E>
@@ -533,6 +538,16 @@ __device__ __forceinline__ void stage_row(const E* src, E* dst, int nEntries, in
         __builtin_amdgcn_global_load_lds(static_cast<const void*>(src + lane * PER),
                                          (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
 }
+// The same into a destination typed as LDS (no generic-to-LDS conversion left
+// for the compiler to check).
+#define DDF_LDS __attribute__((address_space(3)))
+template <class E>
+__device__ __forceinline__ void stage_row_lds(const E* src, DDF_LDS void* dst, int nEntries,
+                                              int lane) {
+    constexpr int PER = 16 / (int)sizeof(E);
+    if (lane * PER < nEntries)
+        __builtin_amdgcn_global_load_lds(static_cast<const void*>(src + lane * PER), dst, 16, 0, 0);
+}
 // LDS entry read.  A two-float entry is read as one native 16-byte vector:
 // the compiler keeps that a single ds_read_b128, where the float4 of the
 // global path narrows to a 12-byte ds_read_b96 (measured on MI355X at
@@ -547,7 +562,29 @@ __device__ __forceinline__ float4 lds_entry<float4>(const float4* p) {
     const f4v v = *reinterpret_cast<const f4v*>(p);
     return make_float4(v.x, v.y, v.z, v.w);
 }
+// Entry reads through LDS-typed pointers, as native vectors.
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef double d2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 lds_entry(const DDF_LDS float2* p) {
+    const f2v v = *(const DDF_LDS f2v*)p;
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ float4 lds_entry(const DDF_LDS float4* p) {
+    const f4v v = *(const DDF_LDS f4v*)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ double2 lds_entry(const DDF_LDS double2* p) {
+    const d2v v = *(const DDF_LDS d2v*)p;
+    return make_double2(v.x, v.y);
+}
 __device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+// Wave-uniform element of a table the kernel only reads, through the scalar
+// cache (s_load, counted in lgkmcnt): waiting for it does not wait for the
+// global-to-LDS copies still in flight, which a vector load's vmcnt would.
+__device__ __forceinline__ float2 uniform_ld(const float2* p, size_t i) {
+    const f2v v = ((const __attribute__((address_space(4))) f2v*)p)[i];
+    return make_float2(v.x, v.y);
+}
 __device__ __forceinline__ int quad_min(int v) {
     v = min(v, __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false));  // quad_perm [1,0,3,2]
     return min(v, __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false));  // quad_perm [2,3,0,1]
@@ -664,13 +701,13 @@ void k_back_lds(const BackLaunch L) {
                 const int j = grp * NV + v;
                 if (j >= count) break;
                 const int vg = J.sub.first + j * J.sub.stride;
-                const float2 cs = J.cossin[(size_t)vg * g.SLN + zg];
+                const float2 cs = uniform_ld(J.cossin, (size_t)vg * g.SLN + zg);
                 const BackSample bs = back_sample(vx, vy, cs.x, cs.y, g.x0, g.y0, g.inv_sid,
                                                   g.invdb, g.detCntIdx, g.dx);
                 const Fetch fL = fetch_pos<E>(bs.idxL, DNU), fR = fetch_pos<E>(bs.idxR, DNU);
                 const float cosVal = g.dx * __builtin_amdgcn_rcpf(bs.ddv * bs.l);
-                const int iL = (int)(fL.off / (unsigned)sizeof(E)) - w0[v];
-                const int iR = (int)(fR.off / (unsigned)sizeof(E)) - w0[v];
+                const int iL = fL.i0 - w0[v];
+                const int iR = fR.i0 - w0[v];
                 if ((unsigned)iL < (unsigned)WENT && (unsigned)iR < (unsigned)WENT) {
                     const E* lrow = buf + v * ZT * WENT;
 #pragma unroll
@@ -745,7 +782,14 @@ void k_proj_lds(const ProjLaunch L) {
     constexpr int CP = WP / (64 * PER);              // 64-lane copies per run
     constexpr int S = ZT >= 8 ? 1 : 8 / ZT;          // slabs per group
     constexpr int ROWS = S * ZT;
-    __shared__ __attribute__((aligned(16))) E buf[2 * ROWS * WP];  // two alternating halves
+    // Two buffers as separate LDS objects: the compiler then knows that reads of
+    // one do not alias the global-to-LDS copies in flight into the other and
+    // puts no wait for those copies in front of them.
+    __shared__ __attribute__((aligned(16))) E bufA[ROWS * WP];
+    __shared__ __attribute__((aligned(16))) E bufB[ROWS * WP];
+    using LE = DDF_LDS E;
+    LE* const ldsA = (LE*)bufA;
+    LE* const ldsB = (LE*)bufB;
     const int lin = xcd_linear(blockIdx.x, gridDim.x);
     if (lin >= L.nwork) return;  // whole block
     int ji = 0;
@@ -812,7 +856,7 @@ void k_proj_lds(const ProjLaunch L) {
     const int ii0 = J.ii0;
     const int iiE = min(J.ii1, max(g.XN, g.YN));  // block-uniform
     const int ng = (iiE - ii0 + S - 1) / S;
-    auto stage = [&](int grp, int bo, int* w0) __attribute__((always_inline)) {
+    auto stage = [&](int grp, LE* buf, int* w0) __attribute__((always_inline)) {
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             const int ii = ii0 + grp * S + s;
@@ -826,12 +870,13 @@ void k_proj_lds(const ProjLaunch L) {
                 for (int cp = 0; cp < CP; ++cp) {
                     if ((((s * ZT + q) * CP + cp) & 3) != wave) continue;
                     const int e0 = w0[s] + cp * 64 * PER;
-                    stage_row(tabB + ((size_t)ii * ZT + q) * KB + e0,
-                              &buf[bo + (s * ZT + q) * WP + cp * 64 * PER], KB - e0, lane);
+                    stage_row_lds(tabB + ((size_t)ii * ZT + q) * KB + e0,
+                                  (DDF_LDS void*)&buf[(s * ZT + q) * WP + cp * 64 * PER], KB - e0,
+                                  lane);
                 }
         }
     };
-    auto sum = [&](int grp, int bo, const int* w0) __attribute__((always_inline)) {
+    auto sum = [&](int grp, const LE* buf, const int* w0) __attribute__((always_inline)) {
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             const int ii = ii0 + grp * S + s;
@@ -846,22 +891,24 @@ void k_proj_lds(const ProjLaunch L) {
                     const float r = slab_r(slab_obj(ii, ocS, g.dx), pa[v], pf[v], pbb[v]);
                     const Fetch f = fetch_pos<E>(slab_coord(r, g.invdx, ocK), Nk);
                     const float invlen = __builtin_amdgcn_rcpf(r - lane_prev(r));
-                    const int li = (int)(f.off / (unsigned)sizeof(E)) - w0[s];
-                    // Lanes outside the run (or views of the other slab orientation)
-                    // read global memory; the sums run after the branch with every
-                    // lane active, as the DPP neighbour read needs.  (A wave-uniform
-                    // choice with the sums duplicated in both paths, which avoids the
-                    // wait for in-flight copies before the LDS reads, measured slower:
-                    // 21.5 vs 20.6 ms at chunk 4, 17.9 vs 16.7 ms at chunk 8.)
+                    const int li = f.i0 - w0[s];
+                    // Every lane reads the run (index clamped into it); lanes outside
+                    // the run, and views of the other slab orientation, then replace
+                    // their entries with global reads in a branch (skipped when no
+                    // lane needs it) that waits for its own loads, so the common path
+                    // waits for no vector memory (copies of the next group stay in
+                    // flight).  The sums run after the branch with every lane active,
+                    // as the DPP neighbour read needs.
                     const bool inw = ((xs ^ xsB) >> v & 1u) == 0u && (unsigned)li < (unsigned)WP;
+                    const int lc = min(max(li, 0), WP - 1);
                     E e[ZT];
-                    if (inw) {
 #pragma unroll
-                        for (int q = 0; q < ZT; ++q) e[q] = lds_entry(&buf[bo + (s * ZT + q) * WP + li]);
-                    } else {
+                    for (int q = 0; q < ZT; ++q) e[q] = lds_entry(&buf[(s * ZT + q) * WP + lc]);
+                    if (!inw) {
                         const E* row = (xv ? baseY : baseX) + (size_t)ii * ZT * K;
 #pragma unroll
                         for (int q = 0; q < ZT; ++q) e[q] = entry_at(row + (size_t)q * K, f.off);
+                        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0), inside the branch
                     }
 #pragma unroll
                     for (int q = 0; q < ZT; ++q) {
@@ -872,21 +919,20 @@ void k_proj_lds(const ProjLaunch L) {
             }
         }
     };
-    constexpr int HB = ROWS * WP;  // second half
     int wA[S], wB[S];
     if (ng > 0) {
-        stage(0, 0, wA);
+        stage(0, ldsA, wA);
         wait_vmem();
         __syncthreads();
     }
     for (int grp = 0; grp < ng; grp += 2) {
-        if (grp + 1 < ng) stage(grp + 1, HB, wB);
-        sum(grp, 0, wA);
+        if (grp + 1 < ng) stage(grp + 1, ldsB, wB);
+        sum(grp, ldsA, wA);
         wait_vmem();
         __syncthreads();
         if (grp + 1 >= ng) break;
-        if (grp + 2 < ng) stage(grp + 2, 0, wA);
-        sum(grp + 1, HB, wB);
+        if (grp + 2 < ng) stage(grp + 2, ldsA, wA);
+        sum(grp + 1, ldsB, wB);
         wait_vmem();
         __syncthreads();
     }
