@@ -531,38 +531,24 @@ void k_back(const BackLaunch L) {
 // exchange needed); a lane whose entries fall outside it reads the table in
 // global memory, so the window decides speed only.  Arithmetic and summation
 // order are those of k_back, so the results are bit-identical.
-template <class E>
-__device__ __forceinline__ void stage_row(const E* src, E* dst, int nEntries, int lane) {
-    constexpr int PER = 16 / (int)sizeof(E);  // entries one lane copies
-    if (lane * PER < nEntries)
-        __builtin_amdgcn_global_load_lds(static_cast<const void*>(src + lane * PER),
-                                         (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
-}
-// The same into a destination typed as LDS (no generic-to-LDS conversion left
-// for the compiler to check).
+// Staging: one 64-lane global-to-LDS copy of 16 bytes per lane (one two-float
+// or f64 entry, two f32 entries) from a table row into LDS.  LDS operands are
+// typed as LDS pointers throughout, so the compiler tracks each staging buffer
+// as its own object and leaves no generic-to-LDS conversion to check.
 #define DDF_LDS __attribute__((address_space(3)))
 template <class E>
-__device__ __forceinline__ void stage_row_lds(const E* src, DDF_LDS void* dst, int nEntries,
-                                              int lane) {
-    constexpr int PER = 16 / (int)sizeof(E);
+__device__ __forceinline__ void stage_row(const E* src, DDF_LDS void* dst, int nEntries, int lane) {
+    constexpr int PER = 16 / (int)sizeof(E);  // entries one lane copies
     if (lane * PER < nEntries)
         __builtin_amdgcn_global_load_lds(static_cast<const void*>(src + lane * PER), dst, 16, 0, 0);
 }
-// LDS entry read.  A two-float entry is read as one native 16-byte vector:
-// the compiler keeps that a single ds_read_b128, where the float4 of the
-// global path narrows to a 12-byte ds_read_b96 (measured on MI355X at
-// 512x512x64, 2304 views: 20.3 ms with 16-byte reads, 25.4 ms with 12-byte).
-template <class E>
-__device__ __forceinline__ E lds_entry(const E* p) {
-    return *p;
-}
+// LDS entry reads as native vectors.  A two-float entry is read as one 16-byte
+// vector: the compiler keeps that a single ds_read_b128, where the float4 of
+// the global path narrows to a 12-byte ds_read_b96.  Measured on MI355X at
+// 512x512x64, 2304 views: backprojector 20.3 ms with 16-byte reads, 25.4 ms
+// with 12-byte ones; reading the 12 bytes used as ds_read_b64 + ds_read_b32:
+// projector 15.6 -> 18.2 ms, backprojector 19.2 -> 21.2 ms.
 typedef float f4v __attribute__((ext_vector_type(4)));
-template <>
-__device__ __forceinline__ float4 lds_entry<float4>(const float4* p) {
-    const f4v v = *reinterpret_cast<const f4v*>(p);
-    return make_float4(v.x, v.y, v.z, v.w);
-}
-// Entry reads through LDS-typed pointers, as native vectors.
 typedef float f2v __attribute__((ext_vector_type(2)));
 typedef double d2v __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ float2 lds_entry(const DDF_LDS float2* p) {
@@ -609,6 +595,9 @@ void k_back_lds(const BackLaunch L) {
     static_assert(NV >= 1 && NV <= 16, "four corner lanes per view");
     __shared__ __attribute__((aligned(16))) E bufA[ROWS * WENT];
     __shared__ __attribute__((aligned(16))) E bufB[ROWS * WENT];
+    using LE = DDF_LDS E;
+    LE* const ldsA = (LE*)bufA;
+    LE* const ldsB = (LE*)bufB;
     const int lin = xcd_linear(blockIdx.x, gridDim.x);
     if (lin >= L.nwork) return;
     int ji = 0;
@@ -671,7 +660,7 @@ void k_back_lds(const BackLaunch L) {
             for (int v = 0; v < NV; ++v) w0[v] = __builtin_amdgcn_readlane(wl, 4 * ((j0 + v) & 15));
         };
         // Row (view v, slice q) is copied by wave (v*ZT + q) mod 4.
-        auto stage = [&](int grp, E* buf, const int* w0) __attribute__((always_inline)) {
+        auto stage = [&](int grp, LE* buf, const int* w0) __attribute__((always_inline)) {
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 const int j = grp * NV + v;
@@ -681,20 +670,22 @@ void k_back_lds(const BackLaunch L) {
 #pragma unroll
                     for (int i = 0; i < ZT / 4; ++i) {
                         const int q = 4 * i + wave;
-                        stage_row(src + (size_t)q * K, buf + (v * ZT + q) * WENT, K - w0[v], lane);
+                        stage_row(src + (size_t)q * K, (DDF_LDS void*)(buf + (v * ZT + q) * WENT),
+                                      K - w0[v], lane);
                     }
                 } else {
 #pragma unroll
                     for (int q = 0; q < ZT; ++q)
                         if (((v * ZT + q) & 3) == wave)
-                            stage_row(src + (size_t)q * K, buf + (v * ZT + q) * WENT, K - w0[v],
-                                      lane);
+                            stage_row(src + (size_t)q * K,
+                                          (DDF_LDS void*)(buf + (v * ZT + q) * WENT), K - w0[v],
+                                          lane);
                 }
             }
         };
         const float vx = ((float)x - g.ocxB) * g.dx;
         const float vy = ((float)y - g.ocyB) * g.dx;
-        auto sum = [&](int grp, const E* buf, const int* w0) __attribute__((always_inline)) {
+        auto sum = [&](int grp, const LE* buf, const int* w0) __attribute__((always_inline)) {
             if (m == 0) return;
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
@@ -709,7 +700,7 @@ void k_back_lds(const BackLaunch L) {
                 const int iL = fL.i0 - w0[v];
                 const int iR = fR.i0 - w0[v];
                 if ((unsigned)iL < (unsigned)WENT && (unsigned)iR < (unsigned)WENT) {
-                    const E* lrow = buf + v * ZT * WENT;
+                    const LE* lrow = buf + v * ZT * WENT;
 #pragma unroll
                     for (int q = 0; q < ZT; ++q) {
                         const Samp<MODE> TL(lds_entry(lrow + q * WENT + iL), fL.t);
@@ -730,23 +721,23 @@ void k_back_lds(const BackLaunch L) {
         int wA[NV], wB[NV];
         const int ng = (count + NV - 1) / NV;
         window(0, wA);
-        stage(0, bufA, wA);
+        stage(0, ldsA, wA);
         wait_vmem();
         __syncthreads();
         for (int grp = 0; grp < ng; grp += 2) {
             if (grp + 1 < ng) {
                 window(grp + 1, wB);
-                stage(grp + 1, bufB, wB);
+                stage(grp + 1, ldsB, wB);
             }
-            sum(grp, bufA, wA);
+            sum(grp, ldsA, wA);
             wait_vmem();
             __syncthreads();
             if (grp + 1 >= ng) break;
             if (grp + 2 < ng) {
                 window(grp + 2, wA);
-                stage(grp + 2, bufA, wA);
+                stage(grp + 2, ldsA, wA);
             }
-            sum(grp + 1, bufB, wB);
+            sum(grp + 1, ldsB, wB);
             wait_vmem();
             __syncthreads();
         }
@@ -870,7 +861,7 @@ void k_proj_lds(const ProjLaunch L) {
                 for (int cp = 0; cp < CP; ++cp) {
                     if ((((s * ZT + q) * CP + cp) & 3) != wave) continue;
                     const int e0 = w0[s] + cp * 64 * PER;
-                    stage_row_lds(tabB + ((size_t)ii * ZT + q) * KB + e0,
+                    stage_row(tabB + ((size_t)ii * ZT + q) * KB + e0,
                                   (DDF_LDS void*)&buf[(s * ZT + q) * WP + cp * 64 * PER], KB - e0,
                                   lane);
                 }

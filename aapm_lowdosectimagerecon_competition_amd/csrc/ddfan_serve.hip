@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 #include <string.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -38,11 +39,53 @@ double ms_between(Clock::time_point a, Clock::time_point b) {
     return std::chrono::duration<double, std::milli>(b - a).count();
 }
 
+// memcpy split over up to 8 threads for large buffers.  A fresh destination
+// is faulted in by the copying threads, so the first touch of its pages runs
+// in parallel too (a 434 MB sinogram: 87 ms single-threaded into a new vector).
+void parallel_copy(void* dst, const void* src, size_t bytes) {
+    constexpr size_t kMinPerThread = 16u << 20;
+    const size_t hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t nt = std::min<size_t>({8, hw, bytes / kMinPerThread});
+    if (nt <= 1) {
+        if (bytes) memcpy(dst, src, bytes);
+        return;
+    }
+    const size_t step = ((bytes + nt - 1) / nt + 4095) & ~(size_t)4095;
+    std::vector<std::thread> th;
+    th.reserve(nt);
+    for (size_t i = 0; i < nt; ++i) {
+        const size_t a = i * step;
+        if (a >= bytes) break;
+        const size_t n = std::min(step, bytes - a);
+        th.emplace_back([=] { memcpy((char*)dst + a, (const char*)src + a, n); });
+    }
+    for (auto& t : th) t.join();
+}
+
+// Host float buffer without value-initialisation (filled by a copy).
+struct HostFloats {
+    std::unique_ptr<float[]> p;
+    size_t n = 0;
+    bool empty() const { return n == 0; }
+    size_t size() const { return n; }
+    float* data() { return p.get(); }
+    const float* data() const { return p.get(); }
+    void alloc(size_t count) {
+        p.reset(count ? new float[count] : nullptr);
+        n = count;
+    }
+    void assign_copy(const float* src, size_t count) {
+        alloc(count);
+        parallel_copy(p.get(), src, count * sizeof(float));
+    }
+};
+
 struct Request {
     long long id = 0;
     // inputs (host copies)
     ddfan_geom G{};
-    std::vector<float> xds, yds, hangs, meas, x0;
+    std::vector<float> xds, yds, hangs;
+    HostFloats meas, x0;
     std::vector<uint8_t> mask;
     int PN = 0, numIter = 0, numOS = 1, fista = 0;
     size_t bytes = 0;
@@ -56,7 +99,7 @@ struct Request {
     int status = DDFAN_OK;
     std::string err;
     bool done = false;
-    std::vector<float> result;
+    HostFloats result;
     Clock::time_point tSubmit, tAdmit, tDone;
 
     ~Request() { release_device(); }
@@ -180,7 +223,7 @@ void ddfan_server::loop() {
                 continue;
             }
             const size_t nv = (size_t)R->G.XN * R->G.YN * R->G.SLN;
-            R->result.resize(nv);
+            R->result.alloc(nv);
             int st = DDFAN_OK;
             std::string err;
             if (hipMemcpyAsync(R->result.data(), R->d_x.p, nv * sizeof(float),
@@ -353,9 +396,9 @@ int ddfan_server_submit(ddfan_server* S, const ddfan_geom* G, const float* hangs
     R->xds.assign(G->xds, G->xds + G->DNU);
     R->yds.assign(G->yds, G->yds + G->DNU);
     R->hangs.assign(hangs, hangs + (size_t)PN * G->SLN);
-    R->meas.assign(meas, meas + (size_t)PN * G->DNU * G->SLN);
+    R->meas.assign_copy(meas, (size_t)PN * G->DNU * G->SLN);
     const size_t nv = (size_t)G->XN * G->YN * G->SLN;
-    if (x0) R->x0.assign(x0, x0 + nv);
+    if (x0) R->x0.assign_copy(x0, nv);
     if (mask) R->mask.assign(mask, mask + (size_t)G->XN * G->YN);
     R->PN = PN;
     R->numIter = numIter;
@@ -403,7 +446,7 @@ int ddfan_server_wait(ddfan_server* S, long long id, float* x, double* queue_ms,
     if (latency_ms) *latency_ms = ms_between(R->tSubmit, R->tDone);
     if (R->status != DDFAN_OK)
         return fail(R->status, "request %lld failed: %s", id, R->err.c_str());
-    if (x) memcpy(x, R->result.data(), R->result.size() * sizeof(float));
+    if (x) parallel_copy(x, R->result.data(), R->result.size() * sizeof(float));
     return DDFAN_OK;
 }
 

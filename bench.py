@@ -502,10 +502,22 @@ def main():
                             mode="plain", sat_mode="f32")
         td = time.perf_counter()
         rd = float(np.linalg.norm(xd - xp.T) / max(np.linalg.norm(xp), 1e-30))
+        # the same job through the request loop, alone: submit (inputs copied
+        # from the host) -> result on the host, like the plain form's host buffers
+        log(f"[bench] ossart served {it} it x {nos} subsets")
+        with dd.Server(hbm_budget=0, max_slots=4, sat_mode=args.sat) as srv:
+            te = time.perf_counter()
+            res = srv.wait(srv.submit(g, hangs, meas, it, nos, 1, mask=mask, x0=x0))
+            tf = time.perf_counter()
         out["ossart"] = {"iters": it, "subsets": nos, "device_s": tb - ta, "plain_s": td - tc,
+                         "served_s": tf - te,
                          "speedup_vs_plain": (td - tc) / (tb - ta),
+                         "speedup_served_vs_plain": (td - tc) / (tf - te),
                          "device_voxel_views_per_s": 2.0 * vv * it / (tb - ta),
-                         "rel_diff_device_twofloat_vs_plain_f32": rd}
+                         "rel_diff_device_twofloat_vs_plain_f32": rd,
+                         "served_equals_device": bool(np.array_equal(res.x, xd)),
+                         "rel_diff_served_vs_device": float(np.linalg.norm(res.x - xd) /
+                                                            max(np.linalg.norm(xd), 1e-30))}
         d_meas.free()
         d_x.free()
 
