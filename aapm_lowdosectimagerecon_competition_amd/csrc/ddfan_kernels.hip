@@ -155,7 +155,7 @@ struct SatDev {
     int nrows, Nk, rs, ks, SLN, ZT, nzc, nzg, blk0;
 };
 struct SatLaunch {
-    int njobs;
+    int njobs, nwork;
     SatDev j[2 * kBatch];
 };
 
@@ -166,7 +166,10 @@ __global__ __launch_bounds__(256) void k_sat_rows(const SatLaunch L) {
     using E = typename Sat<MODE>::E;
     constexpr int NSEG = 256 / G;
     extern __shared__ __align__(16) unsigned char smem[];
-    const int bid = blockIdx.x;
+    // The blocks of one row (its slice groups, which share the input's cache
+    // lines) run back to back on one XCD.
+    const int bid = xcd_linear(blockIdx.x, gridDim.x);
+    if (bid >= L.nwork) return;  // whole block
     int ji = 0;
     for (int t = 1; t < L.njobs; ++t)
         if (bid >= L.j[t].blk0) ji = t;
@@ -1133,11 +1136,13 @@ static hipError_t launch_sat_group(int mode, int G, SatDev* t, int n, hipStream_
     }
     L.njobs = m;
     if (nb == 0) return hipSuccess;
-    if (nb > 0x7fffffffLL) return hipErrorInvalidValue;
+    if (nb > 0x7fffff00LL) return hipErrorInvalidValue;
+    L.nwork = (int)nb;
+    const unsigned grid = (unsigned)xcd_grid((int)nb);
     const size_t lsz = mode == SAT_F32 ? 4 : 8;
     const size_t lds = (size_t)G * maxK * (lsz + 4) + (size_t)256 * lsz;
 #define DDF_SATV(GG) \
-    DDF_MODE_SWITCH(mode, k_sat_rows<MC, GG><<<(unsigned)nb, 256, lds, st>>>(L))
+    DDF_MODE_SWITCH(mode, k_sat_rows<MC, GG><<<grid, 256, lds, st>>>(L))
     switch (G) {
         case 16: DDF_SATV(16); break;
         case 8: DDF_SATV(8); break;
