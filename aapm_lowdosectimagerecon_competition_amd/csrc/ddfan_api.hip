@@ -588,13 +588,49 @@ int rebin_common(float* Proj, float* Views, const void* raw, bool rawOnDevice, i
     p.H = H;
     p.DefTimes = DefTimes;
     p.SLN = SLN;
+    // Raw-view block order: with nondecreasing slice starts the slices that
+    // read raw view i are a contiguous range lo..hi; used when the grid it
+    // needs (views x the widest range) is at most twice the work.
+    std::vector<int> lo, hi;
+    {
+        bool sorted = true;
+        for (int s = 1; s < SLN; ++s) sorted = sorted && start[s] >= start[s - 1];
+        const long long imin = start[0], imax = (long long)start[SLN - 1] + DefTimes - 1;
+        const long long NI = imax - imin + 1;
+        if (sorted && NI <= 0x3fffffffLL) {
+            lo.resize(NI);
+            hi.resize(NI);
+            int a = 0, b = -1, R = 0;
+            for (long long k = 0; k < NI; ++k) {
+                const long long i = imin + k;
+                while (b + 1 < SLN && start[b + 1] <= i) ++b;              // last s with start <= i
+                while (a < SLN && (long long)start[a] + DefTimes - 1 < i) ++a;  // first s whose window reaches i
+                lo[k] = a;
+                hi[k] = b;
+                R = std::max(R, b - a + 1);
+            }
+            p.imin = imin;
+            p.NI = (int)NI;
+            p.R = R;
+            if (R == 0 || (double)NI * R > 2.0 * SLN * DefTimes + 64.0) {
+                lo.clear();
+                hi.clear();
+            }
+        }
+    }
     hipStream_t st = nullptr;
     HIPCALL(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     int rc = [&]() -> int {
-        DBuf dz, dstart, dsm, draw, dP, dV;
+        DBuf dz, dstart, dsm, draw, dP, dV, dlo, dhi, dtmp;
         TRY(upload(dz, zPos, sizeof(float) * SLN, st));
         TRY(upload(dstart, start.data(), sizeof(int) * SLN, st));
         TRY(upload(dsm, SM.data(), sizeof(float) * W, st));
+        if (!lo.empty()) {
+            TRY(upload(dlo, lo.data(), sizeof(int) * lo.size(), st));
+            TRY(upload(dhi, hi.data(), sizeof(int) * hi.size(), st));
+            p.lo = dlo.as<int>();
+            p.hi = dhi.as<int>();
+        }
         const size_t esz = rawKind == 0 ? 4 : 2;
         p.totalViews = TotalViews;
         const size_t perView = (size_t)W * H * esz;
@@ -623,8 +659,10 @@ int rebin_common(float* Proj, float* Views, const void* raw, bool rawOnDevice, i
             outP = dP.as<float>();
             outV = dV.as<float>();
         }
+        if (layout == 1) HIPCALL(dtmp.reserve(sizeof(float) * nOut));
         HIPCALL(launch_rebin(p, rawKind, rawOrder, layout, rawDev, countScale, dz.as<float>(),
-                             dstart.as<int>(), dsm.as<float>(), outP, outV, d_cut, d_idv, st));
+                             dstart.as<int>(), dsm.as<float>(), outP, outV, d_cut, d_idv,
+                             dtmp.as<float>(), st));
         if (!outOnDevice) {
             HIPCALL(hipMemcpyAsync(Proj, outP, sizeof(float) * nOut, hipMemcpyDeviceToHost, st));
             HIPCALL(hipMemcpyAsync(Views, outV, sizeof(float) * nViews, hipMemcpyDeviceToHost, st));

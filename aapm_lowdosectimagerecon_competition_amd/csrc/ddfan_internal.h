@@ -135,6 +135,13 @@ struct RebinDev {
     int W, H, DefTimes, SLN;
     long long viewOffset;  // raw view index (0-based) held at raw[0] (raw order 0)
     long long totalViews;  // raw views held (raw order 1 counts them from the end)
+    // Block order by raw view (slice starts nondecreasing): the slices reading
+    // raw view imin + k are lo[k]..hi[k]; block (k, r) is slice lo[k] + r, r < R.
+    // lo == nullptr: block s * DefTimes + a.
+    const int* lo;
+    const int* hi;
+    long long imin;
+    int NI, R;
 };
 
 // rawKind 0: float raw line integrals; 1: uint16 counts converted to
@@ -145,9 +152,11 @@ struct RebinDev {
 // in file order, which readProj.m:43-54 reads and flips).  outLayout 0: reference
 // layout Proj[(s*DefTimes+v)*W+j], Views[s*DefTimes+v]; 1: sinogram layout
 // prj[(v*W+j)*SLN+s], hangs[v*SLN+s].
+// outLayout 1 goes through `tmp` (SLN * DefTimes * W floats, the reference
+// layout) and a transpose; tmp is unused for layout 0.
 hipError_t launch_rebin(const RebinDev& p, int rawKind, int rawOrder, int outLayout,
                         const void* raw, float countScale, const float* zPos, const int* start,
                         const float* SM, float* Proj, float* Views, uint8_t* cut,
-                        int32_t* idvTrace, hipStream_t st);
+                        int32_t* idvTrace, float* tmp, hipStream_t st);
 
 }  // namespace ddf

@@ -1012,8 +1012,22 @@ __global__ __launch_bounds__(256) void k_rebin(const RebinDev p, const void* __r
                                                int32_t* __restrict__ idvTrace) {
     constexpr int MAXC = 8;  // channels per thread: W <= 2048
     __shared__ int anyCut;
-    const int s = blockIdx.x / p.DefTimes;
-    const int a = blockIdx.x - s * p.DefTimes;
+    // Block -> (slice s, rebinned view a).  In raw-view order the blocks that
+    // read one raw view (one per slice whose window holds it) run back to back
+    // on one XCD, so its frame comes from HBM once.
+    const int bl = xcd_linear(blockIdx.x, gridDim.x);
+    int s, a;
+    if (p.lo) {
+        const int k = bl / p.R;
+        if (k >= p.NI) return;  // whole block
+        s = p.lo[k] + (bl - k * p.R);
+        if (s > p.hi[k]) return;
+        a = (int)(p.imin + k - start[s]);
+    } else {
+        if (bl >= p.SLN * p.DefTimes) return;
+        s = bl / p.DefTimes;
+        a = bl - s * p.DefTimes;
+    }
     if (threadIdx.x == 0) anyCut = 0;
     __syncthreads();
     const int st = start[s];
@@ -1076,11 +1090,7 @@ __global__ __launch_bounds__(256) void k_rebin(const RebinDev p, const void* __r
     for (int c = 0; c < MAXC; ++c) {
         const int jj = threadIdx.x + c * 256;
         if (jj < p.W) {
-            const float val = isCut ? 0.0f : vals[c];
-            if constexpr (LAYOUT == 0)
-                Proj[((size_t)s * p.DefTimes + a) * p.W + jj] = val;
-            else
-                Proj[((size_t)a * p.W + jj) * p.SLN + s] = val;
+            Proj[((size_t)s * p.DefTimes + a) * p.W + jj] = isCut ? 0.0f : vals[c];
         }
     }
     if (threadIdx.x == 0) {
@@ -1089,6 +1099,28 @@ __global__ __launch_bounds__(256) void k_rebin(const RebinDev p, const void* __r
         else
             Views[(size_t)a * p.SLN + s] = angle;
         if (cut) cut[(size_t)s * p.DefTimes + a] = isCut ? 1 : 0;
+    }
+}
+
+// Reference layout T[(s*DefTimes + a)*W + j] -> sinogram layout
+// P[(a*W + j)*SLN + s] through 32 x 32 LDS tiles (rows of 128 bytes read and
+// written).  grid (ceil(W/32), ceil(SLN/32), DefTimes), 256 threads.
+__global__ __launch_bounds__(256) void k_rebin_to_sino(const float* __restrict__ T,
+                                                       float* __restrict__ P, int W, int SLN,
+                                                       int DefTimes) {
+    __shared__ float tile[32][33];
+    const int j0 = blockIdx.x * 32, s0 = blockIdx.y * 32, a = blockIdx.z;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+#pragma unroll
+    for (int r = 0; r < 32; r += 8) {
+        const int s = s0 + ty + r, j = j0 + tx;
+        if (s < SLN && j < W) tile[ty + r][tx] = T[((size_t)s * DefTimes + a) * W + j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 32; r += 8) {
+        const int j = j0 + ty + r, s = s0 + tx;
+        if (s < SLN && j < W) P[((size_t)a * W + j) * SLN + s] = tile[tx][ty + r];
     }
 }
 
@@ -1475,16 +1507,22 @@ hipError_t launch_fill(float* p, float v, size_t n, hipStream_t st) {
 hipError_t launch_rebin(const RebinDev& p, int rawKind, int rawOrder, int outLayout,
                         const void* raw, float countScale, const float* zPos, const int* start,
                         const float* SM, float* Proj, float* Views, uint8_t* cut,
-                        int32_t* idvTrace, hipStream_t st) {
+                        int32_t* idvTrace, float* tmp, hipStream_t st) {
     if (p.W > 8 * 256) return hipErrorInvalidValue;
-    const long long nb = (long long)p.SLN * p.DefTimes;
+    const long long nb = p.lo ? (long long)p.NI * p.R : (long long)p.SLN * p.DefTimes;
     if (nb == 0) return hipSuccess;
-    if (nb > 0x7fffffffLL) return hipErrorInvalidValue;
+    if (nb > 0x7fffff00LL || (long long)p.SLN * p.DefTimes > 0x7fffff00LL)
+        return hipErrorInvalidValue;
     if (rawOrder == 1 && p.viewOffset != 0) return hipErrorInvalidValue;
+    if (outLayout == 1 && !tmp) return hipErrorInvalidValue;
     const float scale = rawKind == 1 ? logf(countScale) : countScale;
+    const unsigned grid = (unsigned)xcd_grid((int)nb);
+    // the gather writes the reference layout (rows of W contiguous samples);
+    // the sinogram layout is a transpose of it
+    float* const gProj = outLayout == 1 ? tmp : Proj;
 #define DDF_REBIN(RK, LY, OR)                                                                   \
-    k_rebin<RK, LY, OR><<<(unsigned)nb, 256, 0, st>>>(p, raw, scale, zPos, start, SM, Proj, Views, \
-                                                      cut, idvTrace)
+    k_rebin<RK, LY, OR><<<grid, 256, 0, st>>>(p, raw, scale, zPos, start, SM, gProj, Views, cut, \
+                                              idvTrace)
 #define DDF_REBIN_O(RK, LY)                  \
     do {                                     \
         if (rawOrder == 0)                   \
@@ -1495,6 +1533,8 @@ hipError_t launch_rebin(const RebinDev& p, int rawKind, int rawOrder, int outLay
             return hipErrorInvalidValue;     \
     } while (0)
     if (outLayout != 0 && outLayout != 1) return hipErrorInvalidValue;
+    // the samples go out in the reference layout either way; LY selects the
+    // layout of the view angles
     if (rawKind == 0) {
         if (outLayout == 0) DDF_REBIN_O(0, 0); else DDF_REBIN_O(0, 1);
     } else if (rawKind == 1) {
@@ -1506,6 +1546,10 @@ hipError_t launch_rebin(const RebinDev& p, int rawKind, int rawOrder, int outLay
     }
 #undef DDF_REBIN_O
 #undef DDF_REBIN
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || outLayout == 0) return e;
+    const dim3 tg((unsigned)((p.W + 31) / 32), (unsigned)((p.SLN + 31) / 32), (unsigned)p.DefTimes);
+    k_rebin_to_sino<<<tg, 256, 0, st>>>(tmp, Proj, p.W, p.SLN, p.DefTimes);
     return hipGetLastError();
 }
 

@@ -127,3 +127,41 @@ def test_rebin_reads_stored_frames_and_unscaled_integers(oracle):
         assert np.array_equal(v, ref_v)
     o = oracle.helical_to_fan(flo, z, SD, SO, BV, PERDETW, PERDETH, DT, DETCW, PITCH, fp64=False)
     assert np.array_equal(ref_p, np.transpose(o["Proj"], (1, 2, 0)))
+
+
+@pytest.mark.parametrize("case", ["sorted", "reversed", "gaps", "edge"])
+def test_rebin_block_orders_and_layouts(oracle, case):
+    """Slice positions that take each block order of the device rebin: sorted
+    starts (blocks in raw-view order), reversed ones (slice-major order),
+    slices whose windows leave raw views unread between them, and slices
+    at the scan's start (first raw view before the data, 0 or 1); both output
+    layouts are bit-identical to the FP32 restatement."""
+    import aapm_lowdosectimagerecon_competition_amd as dd
+    from aapm_lowdosectimagerecon_competition_amd.rebin import helical_to_fan_device
+    raw = _raw(6)
+    p = oracle.rebin_params(SD, SO, H, PERDETW, PERDETH, DT, PITCH)
+    h, dz = float(p["h"]), float(p["deltaZ"])
+    top = dz * DT * TURNS - h
+    z = {"sorted": np.linspace(h, top, 11),
+         "reversed": np.linspace(h, top, 11)[::-1],
+         "gaps": np.array([h / 2, h / 2 + 2.5 * dz * DT, top]),
+         "edge": np.array([h / 4, h / 2, h / 2 + 0.5 * dz, h / 2 + dz, h])}[case].astype(np.float32)
+    SLN = z.size
+    ref = oracle.helical_to_fan(raw, z, SD, SO, BV, PERDETW, PERDETH, DT, DETCW, PITCH, fp64=False)
+    print(f"\n[rebin-order] {case}: starts {ref['start'].tolist()}", flush=True)
+    with dd.DeviceBuffer.from_array(raw) as d_raw, \
+            dd.DeviceBuffer(4 * SLN * DT * W) as d_out, dd.DeviceBuffer(4 * SLN * DT) as d_v, \
+            dd.DeviceBuffer(SLN * DT) as d_cut:
+        for layout in (0, 1):
+            helical_to_fan_device(d_raw, 0, 1.0, layout, raw.shape[0], z, SD, SO, BV, W, H,
+                                  PERDETW, PERDETH, DT, DETCW, PITCH, d_out, d_v, d_cut)
+            cut = d_cut.to_numpy((SLN, DT), np.uint8)
+            if layout == 0:
+                out = d_out.to_numpy((SLN, DT, W))
+                views = d_v.to_numpy((SLN, DT))
+            else:
+                out = np.transpose(d_out.to_numpy((DT, W, SLN)), (2, 0, 1))
+                views = d_v.to_numpy((DT, SLN)).T
+            assert np.array_equal(out, ref["Proj"]), layout
+            assert np.array_equal(views, ref["Views"]), layout
+            assert np.array_equal(cut, ref["cut"]), layout
