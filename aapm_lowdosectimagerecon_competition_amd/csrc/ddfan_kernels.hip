@@ -159,6 +159,11 @@ struct SatLaunch {
     SatDev j[2 * kBatch];
 };
 
+// Longest scan segment a thread holds in registers: the rows sat_group admits
+// (G * K * (prefix + sample bytes) + 256 prefixes within 64 KiB) split over
+// 256 / G segments give at most 32 samples (f32 tables, G = 1, K = 8064).
+constexpr int kSatSeg = 32;
+
 template <int MODE, int G>
 __global__ __launch_bounds__(256) void k_sat_rows(const SatLaunch L) {
     using S = Scan<MODE>;
@@ -184,19 +189,29 @@ __global__ __launch_bounds__(256) void k_sat_rows(const SatLaunch L) {
     Lt* pre = reinterpret_cast<Lt*>(smem);  // [G][K] exclusive prefixes
     Lt* tot = pre + (size_t)G * K;          // [G][NSEG] segment totals
     float* val = reinterpret_cast<float*>(tot + (size_t)G * NSEG);  // [G][K] samples
-    const int Lseg = (jb.Nk + NSEG - 1) / NSEG;
+    const int Lseg = (jb.Nk + NSEG - 1) / NSEG;  // <= kSatSeg (launch_sat_group)
     const int k0 = min(seg * Lseg, jb.Nk), k1 = min(k0 + Lseg, jb.Nk);
-    S acc;
-    for (int k = k0; k < k1; ++k) {
-        float v = 0.0f;
-        if (real) {
+    // the segment's samples are loaded first, all in flight together, then scanned
+    float sv[kSatSeg];
+#pragma unroll
+    for (int q = 0; q < kSatSeg; ++q) {
+        const int k = k0 + q;
+        sv[q] = 0.0f;
+        if (real && k < k1) {
             const size_t p = (size_t)row * jb.rs + (size_t)k * jb.ks;
-            v = jb.in[p * jb.SLN + z];
-            if (jb.mask) v = v * (float)jb.mask[p];
+            sv[q] = jb.in[p * jb.SLN + z];
+            if (jb.mask) sv[q] = sv[q] * (float)jb.mask[p];
         }
-        pre[(size_t)tz * K + k] = acc.get();
-        val[(size_t)tz * K + k] = v;
-        acc.add(v);
+    }
+    S acc;
+#pragma unroll
+    for (int q = 0; q < kSatSeg; ++q) {
+        const int k = k0 + q;
+        if (k < k1) {
+            pre[(size_t)tz * K + k] = acc.get();
+            val[(size_t)tz * K + k] = sv[q];
+            acc.add(sv[q]);
+        }
     }
     tot[tz * NSEG + seg] = acc.get();
     if (seg == NSEG - 1) val[(size_t)tz * K + jb.Nk] = 0.0f;
@@ -1170,6 +1185,7 @@ static hipError_t launch_sat_group(int mode, int G, SatDev* t, int n, hipStream_
     if (nb == 0) return hipSuccess;
     if (nb > 0x7fffff00LL) return hipErrorInvalidValue;
     L.nwork = (int)nb;
+    if ((maxK - 1 + 256 / G - 1) / (256 / G) > kSatSeg) return hipErrorInvalidValue;
     const unsigned grid = (unsigned)xcd_grid((int)nb);
     const size_t lsz = mode == SAT_F32 ? 4 : 8;
     const size_t lds = (size_t)G * maxK * (lsz + 4) + (size_t)256 * lsz;
