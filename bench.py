@@ -300,6 +300,11 @@ def bench_helical(dd, sat, iters, nos=48, SLN2=200, rotations=20, seed=5):
            "rebin_ms_per_call": rebin_s * 1e3,
            "rebin_GBs_algorithmic": samples * 8 / rebin_s / 1e9}
     log(f"[bench] helical rebin {out}")
+    if iters <= 0:  # rebin only
+        d_raw.free()
+        d_prj.free()
+        d_views.free()
+        return out
     views = d_views.to_numpy((DefTimes, SLN2))
     XN2 = 512
     conf = dd.Conf.make(XN=XN2, YN=XN2, SLN=SLN2, dx=500.0 / XN2, sid=SID, sdd=SDD, DNU=DNU,
@@ -353,6 +358,9 @@ def main():
     ap.add_argument("--shapes", help="launch-shape table to apply instead of autotuning "
                                      "(written by --save-shapes)")
     ap.add_argument("--save-shapes", help="write the autotuned launch-shape table here")
+    ap.add_argument("--rebin-pass", action="store_true",
+                    help="only the helical workload's device rebin (23040 uint16 raw views -> "
+                         "200 slices x 1152 views, 6 calls); prints no JSON (for rocprofv3 runs)")
     ap.add_argument("--profile-pass", action="store_true",
                     help="one projector + backprojector call, then one 256 MiB device-to-device "
                          "copy whose counter readings calibrate FETCH_SIZE / WRITE_SIZE; "
@@ -370,6 +378,10 @@ def main():
 
     if dd.device_count() < 1:
         raise RuntimeError("no GPU visible")
+    if args.rebin_pass:
+        bench_helical(dd, args.sat, 0)
+        log("[bench] rebin pass done")
+        return
     conf = dd.Conf.make(XN=XN, YN=YN, SLN=SLN, dx=DX, sid=SID, sdd=SDD, DNU=DNU,
                         detCellWidth=float(np.float32(2 * SDD * np.tan(STEP / 2))),
                         detCntIdx=DETC, PN=PN)
