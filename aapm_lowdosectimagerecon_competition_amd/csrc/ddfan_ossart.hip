@@ -133,15 +133,46 @@ hipError_t enqueue_batch_iteration(OsJob* const* jobs, int n, int mode, const Ba
         }
         const int m = (int)sv.size();
         if (m == 0) continue;
-        hipError_t e = T(0, [&] { return launch_sat_vol_batch(mode, sv.data(), m, st); });
+        // A job whose projector fills a launch group by itself gets launches of
+        // its own anyway: its four kernels run back to back, so the tables it
+        // has just built are read before other jobs' tables evict them.  The
+        // rest share batched launches per kernel kind.
+        int ms = 0;  // shared jobs compacted to the front
+        for (int i = 0; i < m; ++i) {
+            const ProjJob& J = pj[i];
+            const int V = J.V == 2 || J.V == 4 ? J.V : 1;
+            const long long waves =
+                (long long)((J.g.DNU + 62) / 63) * ((J.sub.count + V - 1) / V) * J.nzc;
+            if (waves < shape.projFill) {
+                if (ms != i) {
+                    std::swap(sv[ms], sv[i]);
+                    std::swap(pj[ms], pj[i]);
+                    std::swap(ss[ms], ss[i]);
+                    std::swap(bj[ms], bj[i]);
+                }
+                ++ms;
+            }
+        }
+        for (int i = ms; i < m; ++i) {
+            hipError_t e = T(0, [&] { return launch_sat_vol_batch(mode, &sv[i], 1, st); });
+            if (e != hipSuccess) return e;
+            e = T(1, [&] { return launch_proj_batch(mode, &pj[i], 1, st, shape.projFill); });
+            if (e != hipSuccess) return e;
+            e = T(2, [&] { return launch_sat_sino_batch(mode, &ss[i], 1, st); });
+            if (e != hipSuccess) return e;
+            e = T(3, [&] { return launch_back_batch(mode, &bj[i], 1, st, shape.backFill); });
+            if (e != hipSuccess) return e;
+        }
+        if (ms == 0) continue;
+        hipError_t e = T(0, [&] { return launch_sat_vol_batch(mode, sv.data(), ms, st); });
         if (e != hipSuccess) return e;
         e = T(1, [&] {
-            return launch_proj_batch(mode, pj.data(), m, st, shape.projFill);
+            return launch_proj_batch(mode, pj.data(), ms, st, shape.projFill);
         });
         if (e != hipSuccess) return e;
-        e = T(2, [&] { return launch_sat_sino_batch(mode, ss.data(), m, st); });
+        e = T(2, [&] { return launch_sat_sino_batch(mode, ss.data(), ms, st); });
         if (e != hipSuccess) return e;
-        e = T(3, [&] { return launch_back_batch(mode, bj.data(), m, st, shape.backFill); });
+        e = T(3, [&] { return launch_back_batch(mode, bj.data(), ms, st, shape.backFill); });
         if (e != hipSuccess) return e;
     }
     std::vector<int*> ctr(n);
