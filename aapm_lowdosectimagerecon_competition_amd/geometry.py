@@ -83,6 +83,19 @@ class DetectorGeometry:
                     detCntIdx=self.detCntIdx, imgXCenter=self.imgXCenter,
                     imgYCenter=self.imgYCenter)
 
+    def matched_backprojector(self) -> "DetectorGeometry":
+        """The same geometry with the backprojector's channels aligned to the
+        projector's.  The reference backprojector reads detector coordinate
+        beta/dbeta + detCntIdx + 1 (MultiSlicesBackGPU.cu:175-176), one channel
+        above the projector's channel of beta (DD2MutiSlices.m:71; the "should
+        minus 1" note of CollectReconCfg.m:56-60).  Only the backprojector reads
+        detCntIdx (the projector's channels come from xds/yds), so lowering it by
+        one aligns the pair; OS-SART with small subsets then converges where the
+        reference convention diverges.  Not the reference's arithmetic."""
+        import dataclasses
+        return dataclasses.replace(self, detCntIdx=float(np.float32(self.detCntIdx - 1.0)),
+                                   _keep=[])
+
 
 def detector_arc(DNU, detCntIdx, detCellWidth, sdd, sid):
     """xds, yds of DD2MutiSlices.m:61-76 (source at (0, sid), arc detector)."""
