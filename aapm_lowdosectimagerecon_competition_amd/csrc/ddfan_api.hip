@@ -10,6 +10,7 @@
 #include <string>
 #include <vector>
 
+#include "ddfan_guard.h"
 #include "ddfan_plan.h"
 
 using namespace ddf;
@@ -139,345 +140,395 @@ const char* ddfan_last_error(void) { return t_err.c_str(); }
 int ddfan_version(void) { return 1; }
 
 int ddfan_device_count(int* count) {
-    CHECK(count, DDFAN_EINVAL, "count is NULL");
-    int n = 0;
-    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
-    *count = n;
-    return DDFAN_OK;
+    return ddf::guarded([&]() -> int {
+        CHECK(count, DDFAN_EINVAL, "count is NULL");
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+        *count = n;
+        return DDFAN_OK;
+    });
 }
 
 int ddfan_set_default_sat_mode(int m) {
-    CHECK(m >= 0 && m <= 2, DDFAN_EINVAL, "sat_mode must be 0, 1 or 2");
-    g_default_sat.store(m);
-    return DDFAN_OK;
+    return ddf::guarded([&]() -> int {
+        CHECK(m >= 0 && m <= 2, DDFAN_EINVAL, "sat_mode must be 0, 1 or 2");
+        g_default_sat.store(m);
+        return DDFAN_OK;
+    });
 }
 
 // ------------------------------------------------------------ tables
 int ddfan_detector_arc(int DNU, float detCntIdx, float detCellWidth, float sdd, float sid,
                        float* xds, float* yds) {
-    CHECK(DNU >= 1 && xds && yds, DDFAN_EINVAL, "bad detector arguments");
-    detector_arc(DNU, detCntIdx, detCellWidth, sdd, sid, xds, yds);
-    return DDFAN_OK;
+    return ddf::guarded([&]() -> int {
+        CHECK(DNU >= 1 && xds && yds, DDFAN_EINVAL, "bad detector arguments");
+        detector_arc(DNU, detCntIdx, detCellWidth, sdd, sid, xds, yds);
+        return DDFAN_OK;
+    });
 }
 
 int ddfan_recon_mask(int XN, int YN, unsigned char* mask) {
-    CHECK(XN >= 1 && YN >= 1 && mask, DDFAN_EINVAL, "bad mask arguments");
-    recon_mask(XN, YN, mask);
-    return DDFAN_OK;
+    return ddf::guarded([&]() -> int {
+        CHECK(XN >= 1 && YN >= 1 && mask, DDFAN_EINVAL, "bad mask arguments");
+        recon_mask(XN, YN, mask);
+        return DDFAN_OK;
+    });
 }
 
 int ddfan_subset_views(int numViews, int numOS, int k, int* out, int* count) {
-    CHECK(numViews >= 0 && numOS >= 1 && k >= 0 && k < numOS && count, DDFAN_EINVAL,
-          "bad subset arguments");
-    *count = subset_views(numViews, numOS, k, out);
-    return DDFAN_OK;
+    return ddf::guarded([&]() -> int {
+        CHECK(numViews >= 0 && numOS >= 1 && k >= 0 && k < numOS && count, DDFAN_EINVAL,
+              "bad subset arguments");
+        *count = subset_views(numViews, numOS, k, out);
+        return DDFAN_OK;
+    });
 }
 
 int ddfan_rebin_starts(const float* zPos, int SLN, float SD, float SO, int DetHeight,
                        float PerDetW, float PerDetH, int DefTimes, float pitch, int* start) {
-    CHECK(zPos && start && SLN >= 0 && DefTimes >= 1 && DetHeight >= 1, DDFAN_EINVAL,
-          "bad rebin arguments");
-    const RebinParams rp = rebin_params(SD, SO, DetHeight, PerDetW, PerDetH, DefTimes, pitch);
-    for (int s = 0; s < SLN; ++s) start[s] = rebin_start(zPos[s], rp.h, rp.deltaZ);
-    return DDFAN_OK;
+    return ddf::guarded([&]() -> int {
+        CHECK(zPos && start && SLN >= 0 && DefTimes >= 1 && DetHeight >= 1, DDFAN_EINVAL,
+              "bad rebin arguments");
+        const RebinParams rp = rebin_params(SD, SO, DetHeight, PerDetW, PerDetH, DefTimes, pitch);
+        for (int s = 0; s < SLN; ++s) start[s] = rebin_start(zPos[s], rp.h, rp.deltaZ);
+        return DDFAN_OK;
+    });
 }
 
 // ------------------------------------------------------------ memory
 int ddfan_dev_alloc(size_t bytes, void** ptr) {
-    CHECK(ptr, DDFAN_EINVAL, "ptr is NULL");
-    int dev;
-    TRY(check_device(&dev));
-    HIPCALL(hipMalloc(ptr, bytes ? bytes : 16));
-    return DDFAN_OK;
+    return ddf::guarded([&]() -> int {
+        CHECK(ptr, DDFAN_EINVAL, "ptr is NULL");
+        int dev;
+        TRY(check_device(&dev));
+        HIPCALL(hipMalloc(ptr, bytes ? bytes : 16));
+        return DDFAN_OK;
+    });
 }
 int ddfan_dev_free(void* ptr) {
-    if (ptr) HIPCALL(hipFree(ptr));
-    return DDFAN_OK;
+    return ddf::guarded([&]() -> int {
+        if (ptr) HIPCALL(hipFree(ptr));
+        return DDFAN_OK;
+    });
 }
 int ddfan_h2d(void* dst, const void* src, size_t bytes) {
-    HIPCALL(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
-    return DDFAN_OK;
+    return ddf::guarded([&]() -> int {
+        HIPCALL(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+        return DDFAN_OK;
+    });
 }
 int ddfan_d2h(void* dst, const void* src, size_t bytes) {
-    HIPCALL(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
-    return DDFAN_OK;
+    return ddf::guarded([&]() -> int {
+        HIPCALL(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+        return DDFAN_OK;
+    });
 }
 int ddfan_d2d(void* dst, const void* src, size_t bytes) {
-    HIPCALL(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
-    return DDFAN_OK;
+    return ddf::guarded([&]() -> int {
+        HIPCALL(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+        return DDFAN_OK;
+    });
 }
 int ddfan_dev_mem_info(size_t* fb, size_t* tb) {
-    int dev;
-    TRY(check_device(&dev));
-    HIPCALL(hipMemGetInfo(fb, tb));
-    return DDFAN_OK;
+    return ddf::guarded([&]() -> int {
+        int dev;
+        TRY(check_device(&dev));
+        HIPCALL(hipMemGetInfo(fb, tb));
+        return DDFAN_OK;
+    });
 }
 
 // ------------------------------------------------------------ plan
 int ddfan_plan_footprint(const ddfan_geom* G, int PN, int sat_mode, int slice_chunk, int numOS,
                          size_t* bytes) {
-    TRY(validate_geom(G));
-    CHECK(bytes && PN >= 1 && sat_mode >= 0 && sat_mode <= 2, DDFAN_EINVAL, "bad arguments");
-    const int ZT = slice_chunk > 0 ? slice_chunk : 16;
-    const int nzc = (G->SLN + ZT - 1) / ZT;
-    size_t b = sizeof(float) * 2 * (size_t)PN * G->SLN + sizeof(float) * 4 * (G->DNU + 1) +
-               (size_t)G->XN * G->YN;
-    b += 2 * sat_vol_bytes(sat_mode, nzc, ZT, G->XN > G->YN ? G->XN : G->YN,
-                           G->XN > G->YN ? G->XN : G->YN);
-    if (numOS > 0) {
-        const int cnt = (PN + numOS - 1) / numOS;
-        const size_t nvox = (size_t)G->XN * G->YN * G->SLN;
-        b += sat_sino_bytes(sat_mode, nzc, ZT, G->DNU, cnt);
-        b += sizeof(float) * ((size_t)cnt * G->DNU * G->SLN + nvox + (size_t)numOS * cnt * G->DNU +
-                              (size_t)numOS * G->XN * G->YN);
-        b += sat_sino_bytes(sat_mode, 1, 1, G->DNU, cnt) + 2 * sat_vol_bytes(sat_mode, 1, 1, G->XN, G->YN);
-    } else {
-        b += sat_sino_bytes(sat_mode, nzc, ZT, G->DNU, PN);
-    }
-    *bytes = b;
-    return DDFAN_OK;
+    return ddf::guarded([&]() -> int {
+        TRY(validate_geom(G));
+        CHECK(bytes && PN >= 1 && sat_mode >= 0 && sat_mode <= 2, DDFAN_EINVAL, "bad arguments");
+        const int ZT = slice_chunk > 0 ? slice_chunk : 16;
+        const int nzc = (G->SLN + ZT - 1) / ZT;
+        size_t b = sizeof(float) * 2 * (size_t)PN * G->SLN + sizeof(float) * 4 * (G->DNU + 1) +
+                   (size_t)G->XN * G->YN;
+        b += 2 * sat_vol_bytes(sat_mode, nzc, ZT, G->XN > G->YN ? G->XN : G->YN,
+                               G->XN > G->YN ? G->XN : G->YN);
+        if (numOS > 0) {
+            const int cnt = (PN + numOS - 1) / numOS;
+            const size_t nvox = (size_t)G->XN * G->YN * G->SLN;
+            b += sat_sino_bytes(sat_mode, nzc, ZT, G->DNU, cnt);
+            b += sizeof(float) * ((size_t)cnt * G->DNU * G->SLN + nvox + (size_t)numOS * cnt * G->DNU +
+                                  (size_t)numOS * G->XN * G->YN);
+            b += sat_sino_bytes(sat_mode, 1, 1, G->DNU, cnt) + 2 * sat_vol_bytes(sat_mode, 1, 1, G->XN, G->YN);
+        } else {
+            b += sat_sino_bytes(sat_mode, nzc, ZT, G->DNU, PN);
+        }
+        *bytes = b;
+        return DDFAN_OK;
+    });
 }
 
 int ddfan_plan_create(const ddfan_geom* G, const float* hangs, int PN, const unsigned char* mask,
                       int sat_mode, int slice_chunk, ddfan_plan** out) {
-    CHECK(out, DDFAN_EINVAL, "out is NULL");
-    *out = nullptr;
-    TRY(validate_geom(G));
-    CHECK(hangs && PN >= 1, DDFAN_EINVAL, "need PN >= 1 angles");
-    // regularizeAngle subtracts 2*pi until the angle is in range; bounded
-    // magnitudes keep that loop short (and finite).
-    for (size_t i = 0; i < (size_t)PN * G->SLN; ++i)
-        CHECK(isfinite(hangs[i]) && fabsf(hangs[i]) < 1.0e6f, DDFAN_EINVAL,
-              "angle %zu is not finite or exceeds 1e6 rad", i);
-    CHECK(sat_mode >= 0 && sat_mode <= 2, DDFAN_EINVAL, "sat_mode must be 0, 1 or 2");
-    CHECK(slice_chunk == 0 || slice_chunk == 1 || slice_chunk == 2 || slice_chunk == 4 ||
-              slice_chunk == 8 || slice_chunk == 16,
-          DDFAN_EINVAL, "slice_chunk must be 0, 1, 2, 4, 8 or 16");
-    int dev;
-    TRY(check_device(&dev));
-    ddfan_plan* P = new (std::nothrow) ddfan_plan();
-    CHECK(P, DDFAN_ENOMEM, "host allocation failed");
-    P->device = dev;
-    int rc = [&]() -> int {
-        HIPCALL(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
-        const int SLN = G->SLN, DNU = G->DNU;
-        fill_geom(P->g, G, SLN);
-        P->PN = PN;
-        P->mode = sat_mode;
-        P->ZTmax = choose_slice_chunk(hangs, PN, SLN, slice_chunk);
-        // Launch-shape defaults measured on MI355X at 512x512x64, 2304 views:
-        // the projector stages its table rows through LDS on chunks of 8
-        // slices with 4 views per wave in one slab pass (narrower chunks read
-        // per lane), the backprojector runs on the widest chunk with its
-        // detector windows staged one view at a time (narrow chunks stage
-        // 4 / chunk views per group, so each wave copies a row).
-        P->ZT = std::min(P->ZTmax, 8);
-        P->projStage = P->ZT >= 4 ? 1 : 0;
-        P->nzc = (SLN + P->ZT - 1) / P->ZT;
-        P->ZTb = P->ZTmax;
-        P->nzcb = (SLN + P->ZTb - 1) / P->ZTb;
-        P->backStage = std::max(1, 4 / P->ZTb);
-        P->viewsPerWave = 4;
-        P->slabTile = 0;
-        P->h_hangs.assign(hangs, hangs + (size_t)PN * SLN);
-        P->h_cs.resize((size_t)PN * SLN * 2);
-        cossin_table(hangs, (size_t)PN * SLN, P->h_cs.data());
-        std::vector<float> hbx(DNU + 1), hby(DNU + 1);
-        dd_boundaries(DNU + 1, G->xds, hbx.data());
-        dd_boundaries(DNU + 1, G->yds, hby.data());
-        std::vector<uint8_t> hm;
-        if (!mask) hm.assign((size_t)G->XN * G->YN, 1);
-        TRY(upload(P->cs, P->h_cs.data(), sizeof(float) * P->h_cs.size(), P->stream));
-        TRY(upload(P->bx, hbx.data(), sizeof(float) * (DNU + 1), P->stream));
-        TRY(upload(P->by, hby.data(), sizeof(float) * (DNU + 1), P->stream));
-        TRY(upload(P->xds, G->xds, sizeof(float) * DNU, P->stream));
-        TRY(upload(P->yds, G->yds, sizeof(float) * DNU, P->stream));
-        TRY(upload(P->mask, mask ? mask : hm.data(), (size_t)G->XN * G->YN, P->stream));
-        HIPCALL(P->satX.reserve(sat_vol_bytes(sat_mode, P->nzc, P->ZT, G->XN, G->YN)));
-        HIPCALL(P->satY.reserve(sat_vol_bytes(sat_mode, P->nzc, P->ZT, G->YN, G->XN)));
-        HIPCALL(hipStreamSynchronize(P->stream));
+    return ddf::guarded([&]() -> int {
+        CHECK(out, DDFAN_EINVAL, "out is NULL");
+        *out = nullptr;
+        TRY(validate_geom(G));
+        CHECK(hangs && PN >= 1, DDFAN_EINVAL, "need PN >= 1 angles");
+        // regularizeAngle subtracts 2*pi until the angle is in range; bounded
+        // magnitudes keep that loop short (and finite).
+        for (size_t i = 0; i < (size_t)PN * G->SLN; ++i)
+            CHECK(isfinite(hangs[i]) && fabsf(hangs[i]) < 1.0e6f, DDFAN_EINVAL,
+                  "angle %zu is not finite or exceeds 1e6 rad", i);
+        CHECK(sat_mode >= 0 && sat_mode <= 2, DDFAN_EINVAL, "sat_mode must be 0, 1 or 2");
+        CHECK(slice_chunk == 0 || slice_chunk == 1 || slice_chunk == 2 || slice_chunk == 4 ||
+                  slice_chunk == 8 || slice_chunk == 16,
+              DDFAN_EINVAL, "slice_chunk must be 0, 1, 2, 4, 8 or 16");
+        int dev;
+        TRY(check_device(&dev));
+        ddfan_plan* P = new (std::nothrow) ddfan_plan();
+        CHECK(P, DDFAN_ENOMEM, "host allocation failed");
+        P->device = dev;
+        int rc = [&]() -> int {
+            HIPCALL(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
+            const int SLN = G->SLN, DNU = G->DNU;
+            fill_geom(P->g, G, SLN);
+            P->PN = PN;
+            P->mode = sat_mode;
+            P->ZTmax = choose_slice_chunk(hangs, PN, SLN, slice_chunk);
+            // Launch-shape defaults measured on MI355X at 512x512x64, 2304 views:
+            // the projector stages its table rows through LDS on chunks of 8
+            // slices with 4 views per wave in one slab pass (narrower chunks read
+            // per lane), the backprojector runs on the widest chunk with its
+            // detector windows staged one view at a time (narrow chunks stage
+            // 4 / chunk views per group, so each wave copies a row).
+            P->ZT = std::min(P->ZTmax, 8);
+            P->projStage = P->ZT >= 4 ? 1 : 0;
+            P->nzc = (SLN + P->ZT - 1) / P->ZT;
+            P->ZTb = P->ZTmax;
+            P->nzcb = (SLN + P->ZTb - 1) / P->ZTb;
+            P->backStage = std::max(1, 4 / P->ZTb);
+            P->viewsPerWave = 4;
+            P->slabTile = 0;
+            P->h_hangs.assign(hangs, hangs + (size_t)PN * SLN);
+            P->h_cs.resize((size_t)PN * SLN * 2);
+            cossin_table(hangs, (size_t)PN * SLN, P->h_cs.data());
+            std::vector<float> hbx(DNU + 1), hby(DNU + 1);
+            dd_boundaries(DNU + 1, G->xds, hbx.data());
+            dd_boundaries(DNU + 1, G->yds, hby.data());
+            std::vector<uint8_t> hm;
+            if (!mask) hm.assign((size_t)G->XN * G->YN, 1);
+            TRY(upload(P->cs, P->h_cs.data(), sizeof(float) * P->h_cs.size(), P->stream));
+            TRY(upload(P->bx, hbx.data(), sizeof(float) * (DNU + 1), P->stream));
+            TRY(upload(P->by, hby.data(), sizeof(float) * (DNU + 1), P->stream));
+            TRY(upload(P->xds, G->xds, sizeof(float) * DNU, P->stream));
+            TRY(upload(P->yds, G->yds, sizeof(float) * DNU, P->stream));
+            TRY(upload(P->mask, mask ? mask : hm.data(), (size_t)G->XN * G->YN, P->stream));
+            HIPCALL(P->satX.reserve(sat_vol_bytes(sat_mode, P->nzc, P->ZT, G->XN, G->YN)));
+            HIPCALL(P->satY.reserve(sat_vol_bytes(sat_mode, P->nzc, P->ZT, G->YN, G->XN)));
+            HIPCALL(hipStreamSynchronize(P->stream));
+            return DDFAN_OK;
+        }();
+        if (rc != DDFAN_OK) {
+            delete P;
+            return rc;
+        }
+        *out = P;
         return DDFAN_OK;
-    }();
-    if (rc != DDFAN_OK) {
-        delete P;
-        return rc;
-    }
-    *out = P;
-    return DDFAN_OK;
+    });
 }
 
 int ddfan_plan_destroy(ddfan_plan* P) {
-    if (!P) return DDFAN_OK;
-    (void)hipSetDevice(P->device);
-    delete P;
-    return DDFAN_OK;
+    return ddf::guarded([&]() -> int {
+        if (!P) return DDFAN_OK;
+        (void)hipSetDevice(P->device);
+        delete P;
+        return DDFAN_OK;
+    });
 }
 
 int ddfan_plan_query(const ddfan_plan* P, int* zt, int* nzc, int* mode) {
-    CHECK(P, DDFAN_EINVAL, "plan is NULL");
-    if (zt) *zt = P->ZTmax;
-    if (nzc) *nzc = (P->g.SLN + P->ZTmax - 1) / P->ZTmax;
-    if (mode) *mode = P->mode;
-    return DDFAN_OK;
+    return ddf::guarded([&]() -> int {
+        CHECK(P, DDFAN_EINVAL, "plan is NULL");
+        if (zt) *zt = P->ZTmax;
+        if (nzc) *nzc = (P->g.SLN + P->ZTmax - 1) / P->ZTmax;
+        if (mode) *mode = P->mode;
+        return DDFAN_OK;
+    });
 }
 
 int ddfan_plan_set_param(ddfan_plan* P, int param, int value) {
-    CHECK(P, DDFAN_EINVAL, "plan is NULL");
-    switch (param) {
-        case DDFAN_PARAM_SLAB_TILE:
-            CHECK(value >= 0, DDFAN_EINVAL, "slab tile must be >= 0");
-            P->slabTile = value;
-            return DDFAN_OK;
-        case DDFAN_PARAM_VIEWS_PER_WAVE:
-            CHECK(value == 1 || value == 2 || value == 4, DDFAN_EINVAL,
-                  "views per wave must be 1, 2 or 4");
-            P->viewsPerWave = value;
-            return DDFAN_OK;
-        case DDFAN_PARAM_PROJ_CHUNK:
-        case DDFAN_PARAM_BACK_CHUNK: {
-            CHECK(value >= 1 && value <= P->ZTmax && (value & (value - 1)) == 0, DDFAN_EINVAL,
-                  "slice chunk must be a power of two <= %d (slices with identical angles)",
-                  P->ZTmax);
-            HIPCALL(hipSetDevice(P->device));
-            HIPCALL(hipStreamSynchronize(P->stream));
-            const int n = (P->g.SLN + value - 1) / value;
-            if (param == DDFAN_PARAM_PROJ_CHUNK) {
-                P->ZT = value;
-                P->nzc = n;
-            } else {
-                P->ZTb = value;
-                P->nzcb = n;
-                while (P->backStage * value > 32) P->backStage /= 2;  // LDS budget
+    return ddf::guarded([&]() -> int {
+        CHECK(P, DDFAN_EINVAL, "plan is NULL");
+        switch (param) {
+            case DDFAN_PARAM_SLAB_TILE:
+                CHECK(value >= 0, DDFAN_EINVAL, "slab tile must be >= 0");
+                P->slabTile = value;
+                return DDFAN_OK;
+            case DDFAN_PARAM_VIEWS_PER_WAVE:
+                CHECK(value == 1 || value == 2 || value == 4, DDFAN_EINVAL,
+                      "views per wave must be 1, 2 or 4");
+                P->viewsPerWave = value;
+                return DDFAN_OK;
+            case DDFAN_PARAM_PROJ_CHUNK:
+            case DDFAN_PARAM_BACK_CHUNK: {
+                CHECK(value >= 1 && value <= P->ZTmax && (value & (value - 1)) == 0, DDFAN_EINVAL,
+                      "slice chunk must be a power of two <= %d (slices with identical angles)",
+                      P->ZTmax);
+                HIPCALL(hipSetDevice(P->device));
+                HIPCALL(hipStreamSynchronize(P->stream));
+                const int n = (P->g.SLN + value - 1) / value;
+                if (param == DDFAN_PARAM_PROJ_CHUNK) {
+                    P->ZT = value;
+                    P->nzc = n;
+                } else {
+                    P->ZTb = value;
+                    P->nzcb = n;
+                    while (P->backStage * value > 32) P->backStage /= 2;  // LDS budget
+                }
+                return DDFAN_OK;
             }
-            return DDFAN_OK;
+            case DDFAN_PARAM_PROJ_STAGE:
+                CHECK(value == 0 || value == 1, DDFAN_EINVAL, "projector stage must be 0 or 1");
+                HIPCALL(hipSetDevice(P->device));
+                HIPCALL(hipStreamSynchronize(P->stream));
+                P->projStage = value;
+                return DDFAN_OK;
+            case DDFAN_PARAM_BACK_STAGE:
+                CHECK((value == 0 || value == 1 || value == 2 || value == 4) && value * P->ZTb <= 32,
+                      DDFAN_EINVAL, "staged views must be 0, 1, 2 or 4 with views x chunk <= 32");
+                HIPCALL(hipSetDevice(P->device));
+                HIPCALL(hipStreamSynchronize(P->stream));
+                P->backStage = value;
+                return DDFAN_OK;
+            default:
+                return fail(DDFAN_EINVAL, "unknown plan parameter %d", param);
         }
-        case DDFAN_PARAM_PROJ_STAGE:
-            CHECK(value == 0 || value == 1, DDFAN_EINVAL, "projector stage must be 0 or 1");
-            HIPCALL(hipSetDevice(P->device));
-            HIPCALL(hipStreamSynchronize(P->stream));
-            P->projStage = value;
-            return DDFAN_OK;
-        case DDFAN_PARAM_BACK_STAGE:
-            CHECK((value == 0 || value == 1 || value == 2 || value == 4) && value * P->ZTb <= 32,
-                  DDFAN_EINVAL, "staged views must be 0, 1, 2 or 4 with views x chunk <= 32");
-            HIPCALL(hipSetDevice(P->device));
-            HIPCALL(hipStreamSynchronize(P->stream));
-            P->backStage = value;
-            return DDFAN_OK;
-        default:
-            return fail(DDFAN_EINVAL, "unknown plan parameter %d", param);
-    }
+    });
 }
 
 int ddfan_plan_get_param(const ddfan_plan* P, int param, int* value) {
-    CHECK(P && value, DDFAN_EINVAL, "NULL argument");
-    switch (param) {
-        case DDFAN_PARAM_SLAB_TILE: *value = P->slabTile; return DDFAN_OK;
-        case DDFAN_PARAM_VIEWS_PER_WAVE: *value = P->viewsPerWave; return DDFAN_OK;
-        case DDFAN_PARAM_PROJ_CHUNK: *value = P->ZT; return DDFAN_OK;
-        case DDFAN_PARAM_BACK_CHUNK: *value = P->ZTb; return DDFAN_OK;
-        case DDFAN_PARAM_BACK_STAGE: *value = P->backStage; return DDFAN_OK;
-        case DDFAN_PARAM_PROJ_STAGE: *value = P->projStage; return DDFAN_OK;
-        default: return fail(DDFAN_EINVAL, "unknown plan parameter %d", param);
-    }
+    return ddf::guarded([&]() -> int {
+        CHECK(P && value, DDFAN_EINVAL, "NULL argument");
+        switch (param) {
+            case DDFAN_PARAM_SLAB_TILE: *value = P->slabTile; return DDFAN_OK;
+            case DDFAN_PARAM_VIEWS_PER_WAVE: *value = P->viewsPerWave; return DDFAN_OK;
+            case DDFAN_PARAM_PROJ_CHUNK: *value = P->ZT; return DDFAN_OK;
+            case DDFAN_PARAM_BACK_CHUNK: *value = P->ZTb; return DDFAN_OK;
+            case DDFAN_PARAM_BACK_STAGE: *value = P->backStage; return DDFAN_OK;
+            case DDFAN_PARAM_PROJ_STAGE: *value = P->projStage; return DDFAN_OK;
+            default: return fail(DDFAN_EINVAL, "unknown plan parameter %d", param);
+        }
+    });
 }
 
 int ddfan_plan_sync(ddfan_plan* P) {
-    CHECK(P, DDFAN_EINVAL, "plan is NULL");
-    HIPCALL(hipSetDevice(P->device));
-    HIPCALL(hipStreamSynchronize(P->stream));
-    return DDFAN_OK;
+    return ddf::guarded([&]() -> int {
+        CHECK(P, DDFAN_EINVAL, "plan is NULL");
+        HIPCALL(hipSetDevice(P->device));
+        HIPCALL(hipStreamSynchronize(P->stream));
+        return DDFAN_OK;
+    });
 }
 
 int ddfan_plan_proj(ddfan_plan* P, const float* d_vol, float* d_prj, int first, int stride,
                     int count) {
-    CHECK(P && d_vol && d_prj, DDFAN_EINVAL, "NULL argument");
-    TRY(check_subset(P->PN, first, stride, count));
-    if (count == 0) return DDFAN_OK;
-    HIPCALL(hipSetDevice(P->device));
-    TRY(ensure_vol_sat(P));
-    const Subset sub{first, stride, count};
-    HIPCALL(P->timed(0, [&] {
-        return launch_sat_vol(P->mode, P->ZT, P->g, P->nzc, d_vol, P->mask.as<uint8_t>(), P->satX.p,
-                              P->satY.p, P->stream);
-    }));
-    HIPCALL(P->timed(1, [&] {
-        return launch_proj(P->mode, P->ZT, PROJ_PLAIN, P->g, P->nzc, P->cs.as<float2>(),
-                           P->bx.as<float>(), P->by.as<float>(), P->xds.as<float>(),
-                           P->yds.as<float>(), P->satX.p, P->satY.p, sub, d_prj, nullptr, nullptr,
-                           P->slabTile, P->viewsPerWave, P->projStage, P->stream);
-    }));
-    return DDFAN_OK;
+    return ddf::guarded([&]() -> int {
+        CHECK(P && d_vol && d_prj, DDFAN_EINVAL, "NULL argument");
+        TRY(check_subset(P->PN, first, stride, count));
+        if (count == 0) return DDFAN_OK;
+        HIPCALL(hipSetDevice(P->device));
+        TRY(ensure_vol_sat(P));
+        const Subset sub{first, stride, count};
+        HIPCALL(P->timed(0, [&] {
+            return launch_sat_vol(P->mode, P->ZT, P->g, P->nzc, d_vol, P->mask.as<uint8_t>(), P->satX.p,
+                                  P->satY.p, P->stream);
+        }));
+        HIPCALL(P->timed(1, [&] {
+            return launch_proj(P->mode, P->ZT, PROJ_PLAIN, P->g, P->nzc, P->cs.as<float2>(),
+                               P->bx.as<float>(), P->by.as<float>(), P->xds.as<float>(),
+                               P->yds.as<float>(), P->satX.p, P->satY.p, sub, d_prj, nullptr, nullptr,
+                               P->slabTile, P->viewsPerWave, P->projStage, P->stream);
+        }));
+        return DDFAN_OK;
+    });
 }
 
 int ddfan_plan_back(ddfan_plan* P, const float* d_prj, float* d_vol, int first, int stride,
                     int count) {
-    CHECK(P && d_vol && d_prj, DDFAN_EINVAL, "NULL argument");
-    TRY(check_subset(P->PN, first, stride, count));
-    HIPCALL(hipSetDevice(P->device));
-    const Subset sub{first, stride, count};
-    const size_t need = sat_sino_bytes(P->mode, P->nzcb, P->ZTb, P->g.DNU, std::max(count, 1));
-    if (need > P->satP.bytes) {
-        HIPCALL(hipStreamSynchronize(P->stream));
-        HIPCALL(P->satP.reserve(need));
-    }
-    HIPCALL(P->timed(2, [&] {
-        return launch_sat_sino(P->mode, P->ZTb, P->g.DNU, P->g.SLN, P->nzcb, count, d_prj, P->satP.p,
-                               P->stream);
-    }));
-    HIPCALL(P->timed(3, [&] {
-        return launch_back(P->mode, P->ZTb, BACK_PLAIN, P->g, P->nzcb, P->cs.as<float2>(),
-                           P->mask.as<uint8_t>(), P->satP.p, sub, d_vol, nullptr, nullptr, nullptr,
-                           nullptr, P->backStage, P->stream);
-    }));
-    return DDFAN_OK;
+    return ddf::guarded([&]() -> int {
+        CHECK(P && d_vol && d_prj, DDFAN_EINVAL, "NULL argument");
+        TRY(check_subset(P->PN, first, stride, count));
+        HIPCALL(hipSetDevice(P->device));
+        const Subset sub{first, stride, count};
+        const size_t need = sat_sino_bytes(P->mode, P->nzcb, P->ZTb, P->g.DNU, std::max(count, 1));
+        if (need > P->satP.bytes) {
+            HIPCALL(hipStreamSynchronize(P->stream));
+            HIPCALL(P->satP.reserve(need));
+        }
+        HIPCALL(P->timed(2, [&] {
+            return launch_sat_sino(P->mode, P->ZTb, P->g.DNU, P->g.SLN, P->nzcb, count, d_prj, P->satP.p,
+                                   P->stream);
+        }));
+        HIPCALL(P->timed(3, [&] {
+            return launch_back(P->mode, P->ZTb, BACK_PLAIN, P->g, P->nzcb, P->cs.as<float2>(),
+                               P->mask.as<uint8_t>(), P->satP.p, sub, d_vol, nullptr, nullptr, nullptr,
+                               nullptr, P->backStage, P->stream);
+        }));
+        return DDFAN_OK;
+    });
 }
 
 int ddfan_plan_timing(ddfan_plan* P, int enable) {
-    CHECK(P, DDFAN_EINVAL, "plan is NULL");
-    HIPCALL(hipSetDevice(P->device));
-    HIPCALL(hipStreamSynchronize(P->stream));
-    P->clear_events();
-    P->timing = enable != 0;
-    return DDFAN_OK;
+    return ddf::guarded([&]() -> int {
+        CHECK(P, DDFAN_EINVAL, "plan is NULL");
+        HIPCALL(hipSetDevice(P->device));
+        HIPCALL(hipStreamSynchronize(P->stream));
+        P->clear_events();
+        P->timing = enable != 0;
+        return DDFAN_OK;
+    });
 }
 
 int ddfan_plan_timing_read(ddfan_plan* P, int kind, double* total_ms, int* launches) {
-    CHECK(P && total_ms && launches, DDFAN_EINVAL, "NULL argument");
-    HIPCALL(hipSetDevice(P->device));
-    HIPCALL(hipStreamSynchronize(P->stream));
-    double t = 0.0;
-    int n = 0;
-    for (auto& e : P->events) {
-        if (e.kind != kind) continue;
-        float ms = 0.0f;
-        HIPCALL(hipEventElapsedTime(&ms, e.a, e.b));
-        t += ms;
-        ++n;
-    }
-    *total_ms = t;
-    *launches = n;
-    return DDFAN_OK;
+    return ddf::guarded([&]() -> int {
+        CHECK(P && total_ms && launches, DDFAN_EINVAL, "NULL argument");
+        HIPCALL(hipSetDevice(P->device));
+        HIPCALL(hipStreamSynchronize(P->stream));
+        double t = 0.0;
+        int n = 0;
+        for (auto& e : P->events) {
+            if (e.kind != kind) continue;
+            float ms = 0.0f;
+            HIPCALL(hipEventElapsedTime(&ms, e.a, e.b));
+            t += ms;
+            ++n;
+        }
+        *total_ms = t;
+        *launches = n;
+        return DDFAN_OK;
+    });
 }
 
 int ddfan_plan_trace_proj(ddfan_plan* P, int first, int stride, int count, int32_t* d_trace) {
-    CHECK(P && d_trace, DDFAN_EINVAL, "NULL argument");
-    TRY(check_subset(P->PN, first, stride, count));
-    HIPCALL(hipSetDevice(P->device));
-    HIPCALL(launch_proj_trace(P->g, P->cs.as<float2>(), P->bx.as<float>(), P->by.as<float>(),
-                              Subset{first, stride, count}, d_trace, P->stream));
-    HIPCALL(hipStreamSynchronize(P->stream));
-    return DDFAN_OK;
+    return ddf::guarded([&]() -> int {
+        CHECK(P && d_trace, DDFAN_EINVAL, "NULL argument");
+        TRY(check_subset(P->PN, first, stride, count));
+        HIPCALL(hipSetDevice(P->device));
+        HIPCALL(launch_proj_trace(P->g, P->cs.as<float2>(), P->bx.as<float>(), P->by.as<float>(),
+                                  Subset{first, stride, count}, d_trace, P->stream));
+        HIPCALL(hipStreamSynchronize(P->stream));
+        return DDFAN_OK;
+    });
 }
 
 int ddfan_plan_trace_back(ddfan_plan* P, int first, int stride, int count, int32_t* d_trace) {
-    CHECK(P && d_trace, DDFAN_EINVAL, "NULL argument");
-    TRY(check_subset(P->PN, first, stride, count));
-    HIPCALL(hipSetDevice(P->device));
-    HIPCALL(launch_back_trace(P->g, P->cs.as<float2>(), Subset{first, stride, count}, d_trace,
-                              P->stream));
-    HIPCALL(hipStreamSynchronize(P->stream));
-    return DDFAN_OK;
+    return ddf::guarded([&]() -> int {
+        CHECK(P && d_trace, DDFAN_EINVAL, "NULL argument");
+        TRY(check_subset(P->PN, first, stride, count));
+        HIPCALL(hipSetDevice(P->device));
+        HIPCALL(launch_back_trace(P->g, P->cs.as<float2>(), Subset{first, stride, count}, d_trace,
+                                  P->stream));
+        HIPCALL(hipStreamSynchronize(P->stream));
+        return DDFAN_OK;
+    });
 }
 
 // ------------------------------------------------------------ drop-in entry points
@@ -485,53 +536,57 @@ int ddfan_proj(float x0, float y0, int DNU, const float* xds, const float* yds, 
                float imgYCenter, const float* hangs, int PN, int XN, int YN, int SLN,
                const float* hvol, float* hprj, float dx, const unsigned char* mask,
                const int* startIdx, int nStart) {
-    CHECK(hvol && hprj && hangs, DDFAN_EINVAL, "NULL buffer");
-    TRY(check_start_idx(startIdx, nStart, SLN));
-    if (PN == 0) return DDFAN_OK;
-    ddfan_geom G{x0, y0, DNU, xds, yds, 0.0f, imgXCenter, imgYCenter, XN, YN, SLN, dx};
-    ddfan_plan* P = nullptr;
-    TRY(ddfan_plan_create(&G, hangs, PN, mask, g_default_sat.load(), 0, &P));
-    const size_t nv = (size_t)XN * YN * SLN, np = (size_t)PN * DNU * SLN;
-    int rc = [&]() -> int {
-        DBuf dv, dp;
-        HIPCALL(dv.reserve(sizeof(float) * nv));
-        HIPCALL(dp.reserve(sizeof(float) * np));
-        HIPCALL(hipMemcpyAsync(dv.p, hvol, sizeof(float) * nv, hipMemcpyHostToDevice, P->stream));
-        TRY(ddfan_plan_proj(P, dv.as<float>(), dp.as<float>(), 0, 1, PN));
-        HIPCALL(hipMemcpyAsync(hprj, dp.p, sizeof(float) * np, hipMemcpyDeviceToHost, P->stream));
-        HIPCALL(hipStreamSynchronize(P->stream));
-        return DDFAN_OK;
-    }();
-    ddfan_plan_destroy(P);
-    return rc;
+    return ddf::guarded([&]() -> int {
+        CHECK(hvol && hprj && hangs, DDFAN_EINVAL, "NULL buffer");
+        TRY(check_start_idx(startIdx, nStart, SLN));
+        if (PN == 0) return DDFAN_OK;
+        ddfan_geom G{x0, y0, DNU, xds, yds, 0.0f, imgXCenter, imgYCenter, XN, YN, SLN, dx};
+        ddfan_plan* P = nullptr;
+        TRY(ddfan_plan_create(&G, hangs, PN, mask, g_default_sat.load(), 0, &P));
+        const size_t nv = (size_t)XN * YN * SLN, np = (size_t)PN * DNU * SLN;
+        int rc = [&]() -> int {
+            DBuf dv, dp;
+            HIPCALL(dv.reserve(sizeof(float) * nv));
+            HIPCALL(dp.reserve(sizeof(float) * np));
+            HIPCALL(hipMemcpyAsync(dv.p, hvol, sizeof(float) * nv, hipMemcpyHostToDevice, P->stream));
+            TRY(ddfan_plan_proj(P, dv.as<float>(), dp.as<float>(), 0, 1, PN));
+            HIPCALL(hipMemcpyAsync(hprj, dp.p, sizeof(float) * np, hipMemcpyDeviceToHost, P->stream));
+            HIPCALL(hipStreamSynchronize(P->stream));
+            return DDFAN_OK;
+        }();
+        ddfan_plan_destroy(P);
+        return rc;
+    });
 }
 
 int ddfan_back(float x0, float y0, int DNU, const float* xds, const float* yds, float detCntIdx,
                float imgXCenter, float imgYCenter, const float* hangs, int PN, int XN, int YN,
                int SLN, float* hvol, const float* hprj, float dx, const unsigned char* mask,
                const int* startIdx, int nStart) {
-    CHECK(hvol && hprj && hangs, DDFAN_EINVAL, "NULL buffer");
-    TRY(check_start_idx(startIdx, nStart, SLN));
-    const size_t nv = (size_t)XN * YN * SLN, np = (size_t)PN * DNU * SLN;
-    if (PN == 0) {
-        memset(hvol, 0, sizeof(float) * nv);
-        return DDFAN_OK;
-    }
-    ddfan_geom G{x0, y0, DNU, xds, yds, detCntIdx, imgXCenter, imgYCenter, XN, YN, SLN, dx};
-    ddfan_plan* P = nullptr;
-    TRY(ddfan_plan_create(&G, hangs, PN, mask, g_default_sat.load(), 0, &P));
-    int rc = [&]() -> int {
-        DBuf dv, dp;
-        HIPCALL(dv.reserve(sizeof(float) * nv));
-        HIPCALL(dp.reserve(sizeof(float) * np));
-        HIPCALL(hipMemcpyAsync(dp.p, hprj, sizeof(float) * np, hipMemcpyHostToDevice, P->stream));
-        TRY(ddfan_plan_back(P, dp.as<float>(), dv.as<float>(), 0, 1, PN));
-        HIPCALL(hipMemcpyAsync(hvol, dv.p, sizeof(float) * nv, hipMemcpyDeviceToHost, P->stream));
-        HIPCALL(hipStreamSynchronize(P->stream));
-        return DDFAN_OK;
-    }();
-    ddfan_plan_destroy(P);
-    return rc;
+    return ddf::guarded([&]() -> int {
+        CHECK(hvol && hprj && hangs, DDFAN_EINVAL, "NULL buffer");
+        TRY(check_start_idx(startIdx, nStart, SLN));
+        const size_t nv = (size_t)XN * YN * SLN, np = (size_t)PN * DNU * SLN;
+        if (PN == 0) {
+            memset(hvol, 0, sizeof(float) * nv);
+            return DDFAN_OK;
+        }
+        ddfan_geom G{x0, y0, DNU, xds, yds, detCntIdx, imgXCenter, imgYCenter, XN, YN, SLN, dx};
+        ddfan_plan* P = nullptr;
+        TRY(ddfan_plan_create(&G, hangs, PN, mask, g_default_sat.load(), 0, &P));
+        int rc = [&]() -> int {
+            DBuf dv, dp;
+            HIPCALL(dv.reserve(sizeof(float) * nv));
+            HIPCALL(dp.reserve(sizeof(float) * np));
+            HIPCALL(hipMemcpyAsync(dp.p, hprj, sizeof(float) * np, hipMemcpyHostToDevice, P->stream));
+            TRY(ddfan_plan_back(P, dp.as<float>(), dv.as<float>(), 0, 1, PN));
+            HIPCALL(hipMemcpyAsync(hvol, dv.p, sizeof(float) * nv, hipMemcpyDeviceToHost, P->stream));
+            HIPCALL(hipStreamSynchronize(P->stream));
+            return DDFAN_OK;
+        }();
+        ddfan_plan_destroy(P);
+        return rc;
+    });
 }
 
 // ------------------------------------------------------------ rebin
@@ -679,9 +734,11 @@ int ddfan_helical_to_fan(float* Proj, float* Views, const float* proj, long long
                          const float* zPos, int SLN, float SD, float SO, float BVAngle,
                          int DetWidth, int DetHeight, float PerDetW, float PerDetH, int DefTimes,
                          float DetCenterW, float SpiralPitchFactor) {
-    return rebin_common(Proj, Views, proj, false, 0, 0, 1.0f, 0, TotalViews, zPos, SLN, SD, SO,
-                        BVAngle, DetWidth, DetHeight, PerDetW, PerDetH, DefTimes, DetCenterW,
-                        SpiralPitchFactor, false, nullptr, nullptr);
+    return ddf::guarded([&]() -> int {
+        return rebin_common(Proj, Views, proj, false, 0, 0, 1.0f, 0, TotalViews, zPos, SLN, SD, SO,
+                            BVAngle, DetWidth, DetHeight, PerDetW, PerDetH, DefTimes, DetCenterW,
+                            SpiralPitchFactor, false, nullptr, nullptr);
+    });
 }
 
 int ddfan_helical_to_fan_dev(float* d_Proj, float* d_Views, const void* d_raw, int raw_kind,
@@ -690,11 +747,13 @@ int ddfan_helical_to_fan_dev(float* d_Proj, float* d_Views, const void* d_raw, i
                              int DetWidth, int DetHeight, float PerDetW, float PerDetH,
                              int DefTimes, float DetCenterW, float SpiralPitchFactor,
                              unsigned char* d_cut, int32_t* d_idv) {
-    CHECK(raw_kind == 0 || count_scale > 0.0f, DDFAN_EINVAL, "count_scale must be positive");
-    return rebin_common(d_Proj, d_Views, d_raw, true, raw_kind, raw_order, count_scale, layout,
-                        TotalViews,
-                        zPos, SLN, SD, SO, BVAngle, DetWidth, DetHeight, PerDetW, PerDetH,
-                        DefTimes, DetCenterW, SpiralPitchFactor, true, d_cut, d_idv);
+    return ddf::guarded([&]() -> int {
+        CHECK(raw_kind == 0 || count_scale > 0.0f, DDFAN_EINVAL, "count_scale must be positive");
+        return rebin_common(d_Proj, d_Views, d_raw, true, raw_kind, raw_order, count_scale, layout,
+                            TotalViews,
+                            zPos, SLN, SD, SO, BVAngle, DetWidth, DetHeight, PerDetW, PerDetH,
+                            DefTimes, DetCenterW, SpiralPitchFactor, true, d_cut, d_idv);
+    });
 }
 
 // ------------------------------------------------------------ plain host OS-SART
@@ -703,72 +762,74 @@ int ddfan_ossart_host_plain(float x0, float y0, int DNU, const float* xds, const
                             const float* hangs, int numViews, int XN, int YN, int SLN,
                             const float* meas, float* x, float dx, const unsigned char* mask,
                             int numIter, int numOS, int useFISTA, int sat_mode) {
-    CHECK(hangs && meas && x, DDFAN_EINVAL, "NULL buffer");
-    CHECK(numOS >= 1 && numOS <= numViews && numIter >= 0, DDFAN_EINVAL, "bad iteration counts");
-    CHECK(sat_mode >= 0 && sat_mode <= 2, DDFAN_EINVAL, "bad sat_mode");
-    const int saved = g_default_sat.exchange(sat_mode);
-    struct Restore {
-        int v;
-        ~Restore() { g_default_sat.store(v); }
-    } restore{saved};
-    const size_t npix = (size_t)XN * YN, nvox = npix * SLN;
-    const int maxc = subset_views(numViews, numOS, 0, nullptr);
-    std::vector<std::vector<int>> idx(numOS);
-    std::vector<std::vector<float>> rowSum(numOS), colSum(numOS);
-    std::vector<float> hsub((size_t)maxc * SLN), prj((size_t)maxc * DNU * SLN), tp(nvox);
-    std::vector<float> ones(std::max(nvox, (size_t)maxc * DNU * SLN), 1.0f), lasImg;
-    // weights: full-slice calls, slice 1 kept and replicated (OSSART_AAPM.m:59-66)
-    for (int k = 0; k < numOS; ++k) {
-        const int n = subset_views(numViews, numOS, k, nullptr);
-        idx[k].resize(n);
-        subset_views(numViews, numOS, k, idx[k].data());
-        for (int j = 0; j < n; ++j)
-            for (int z = 0; z < SLN; ++z) hsub[(size_t)j * SLN + z] = hangs[(size_t)idx[k][j] * SLN + z];
-        TRY(ddfan_proj(x0, y0, DNU, xds, yds, imgXCenter, imgYCenter, hsub.data(), n, XN, YN, SLN,
-                       ones.data(), prj.data(), dx, mask, nullptr, 0));
-        rowSum[k].resize((size_t)n * DNU);
-        for (size_t r = 0; r < (size_t)n * DNU; ++r) rowSum[k][r] = prj[r * SLN];
-        TRY(ddfan_back(x0, y0, DNU, xds, yds, detCntIdx, imgXCenter, imgYCenter, hsub.data(), n, XN,
-                       YN, SLN, tp.data(), ones.data(), dx, mask, nullptr, 0));
-        colSum[k].resize(npix);
-        for (size_t p = 0; p < npix; ++p) colSum[k][p] = tp[p * SLN];
-    }
-    double t0 = 1.0;
-    for (int it = 0; it < numIter; ++it) {
-        if (useFISTA) lasImg.assign(x, x + nvox);
+    return ddf::guarded([&]() -> int {
+        CHECK(hangs && meas && x, DDFAN_EINVAL, "NULL buffer");
+        CHECK(numOS >= 1 && numOS <= numViews && numIter >= 0, DDFAN_EINVAL, "bad iteration counts");
+        CHECK(sat_mode >= 0 && sat_mode <= 2, DDFAN_EINVAL, "bad sat_mode");
+        const int saved = g_default_sat.exchange(sat_mode);
+        struct Restore {
+            int v;
+            ~Restore() { g_default_sat.store(v); }
+        } restore{saved};
+        const size_t npix = (size_t)XN * YN, nvox = npix * SLN;
+        const int maxc = subset_views(numViews, numOS, 0, nullptr);
+        std::vector<std::vector<int>> idx(numOS);
+        std::vector<std::vector<float>> rowSum(numOS), colSum(numOS);
+        std::vector<float> hsub((size_t)maxc * SLN), prj((size_t)maxc * DNU * SLN), tp(nvox);
+        std::vector<float> ones(std::max(nvox, (size_t)maxc * DNU * SLN), 1.0f), lasImg;
+        // weights: full-slice calls, slice 1 kept and replicated (OSSART_AAPM.m:59-66)
         for (int k = 0; k < numOS; ++k) {
-            const int n = (int)idx[k].size();
+            const int n = subset_views(numViews, numOS, k, nullptr);
+            idx[k].resize(n);
+            subset_views(numViews, numOS, k, idx[k].data());
             for (int j = 0; j < n; ++j)
                 for (int z = 0; z < SLN; ++z) hsub[(size_t)j * SLN + z] = hangs[(size_t)idx[k][j] * SLN + z];
-            TRY(ddfan_proj(x0, y0, DNU, xds, yds, imgXCenter, imgYCenter, hsub.data(), n, XN, YN,
-                           SLN, x, prj.data(), dx, mask, nullptr, 0));
-            for (int j = 0; j < n; ++j)
-                for (int u = 0; u < DNU; ++u) {
-                    const float w = rowSum[k][(size_t)j * DNU + u];
-                    if (w == 0.0f) continue;
-                    const float* b = meas + ((size_t)idx[k][j] * DNU + u) * SLN;
-                    float* r = prj.data() + ((size_t)j * DNU + u) * SLN;
-                    for (int z = 0; z < SLN; ++z) r[z] = (b[z] - r[z]) / w;
-                }
-            TRY(ddfan_back(x0, y0, DNU, xds, yds, detCntIdx, imgXCenter, imgYCenter, hsub.data(), n,
-                           XN, YN, SLN, tp.data(), prj.data(), dx, mask, nullptr, 0));
-            for (size_t p = 0; p < npix; ++p) {
-                const float w = colSum[k][p];
-                for (int z = 0; z < SLN; ++z) {
-                    float t = tp[p * SLN + z];
-                    if (w != 0.0f) t = t / w;
-                    x[p * SLN + z] = x[p * SLN + z] + t;
+            TRY(ddfan_proj(x0, y0, DNU, xds, yds, imgXCenter, imgYCenter, hsub.data(), n, XN, YN, SLN,
+                           ones.data(), prj.data(), dx, mask, nullptr, 0));
+            rowSum[k].resize((size_t)n * DNU);
+            for (size_t r = 0; r < (size_t)n * DNU; ++r) rowSum[k][r] = prj[r * SLN];
+            TRY(ddfan_back(x0, y0, DNU, xds, yds, detCntIdx, imgXCenter, imgYCenter, hsub.data(), n, XN,
+                           YN, SLN, tp.data(), ones.data(), dx, mask, nullptr, 0));
+            colSum[k].resize(npix);
+            for (size_t p = 0; p < npix; ++p) colSum[k][p] = tp[p * SLN];
+        }
+        double t0 = 1.0;
+        for (int it = 0; it < numIter; ++it) {
+            if (useFISTA) lasImg.assign(x, x + nvox);
+            for (int k = 0; k < numOS; ++k) {
+                const int n = (int)idx[k].size();
+                for (int j = 0; j < n; ++j)
+                    for (int z = 0; z < SLN; ++z) hsub[(size_t)j * SLN + z] = hangs[(size_t)idx[k][j] * SLN + z];
+                TRY(ddfan_proj(x0, y0, DNU, xds, yds, imgXCenter, imgYCenter, hsub.data(), n, XN, YN,
+                               SLN, x, prj.data(), dx, mask, nullptr, 0));
+                for (int j = 0; j < n; ++j)
+                    for (int u = 0; u < DNU; ++u) {
+                        const float w = rowSum[k][(size_t)j * DNU + u];
+                        if (w == 0.0f) continue;
+                        const float* b = meas + ((size_t)idx[k][j] * DNU + u) * SLN;
+                        float* r = prj.data() + ((size_t)j * DNU + u) * SLN;
+                        for (int z = 0; z < SLN; ++z) r[z] = (b[z] - r[z]) / w;
+                    }
+                TRY(ddfan_back(x0, y0, DNU, xds, yds, detCntIdx, imgXCenter, imgYCenter, hsub.data(), n,
+                               XN, YN, SLN, tp.data(), prj.data(), dx, mask, nullptr, 0));
+                for (size_t p = 0; p < npix; ++p) {
+                    const float w = colSum[k][p];
+                    for (int z = 0; z < SLN; ++z) {
+                        float t = tp[p * SLN + z];
+                        if (w != 0.0f) t = t / w;
+                        x[p * SLN + z] = x[p * SLN + z] + t;
+                    }
                 }
             }
+            if (useFISTA) {
+                const double t1 = (1.0 + sqrt(1.0 + 4.0 * t0 * t0)) / 2.0;
+                const float m = (float)((t0 - 1.0) / t1);
+                for (size_t o = 0; o < nvox; ++o) x[o] = x[o] + m * (x[o] - lasImg[o]);
+                t0 = t1;
+            }
         }
-        if (useFISTA) {
-            const double t1 = (1.0 + sqrt(1.0 + 4.0 * t0 * t0)) / 2.0;
-            const float m = (float)((t0 - 1.0) / t1);
-            for (size_t o = 0; o < nvox; ++o) x[o] = x[o] + m * (x[o] - lasImg[o]);
-            t0 = t1;
-        }
-    }
-    return DDFAN_OK;
+        return DDFAN_OK;
+    });
 }
 
 }  // extern "C"

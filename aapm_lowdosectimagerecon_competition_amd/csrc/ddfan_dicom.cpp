@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "ddfan.h"
+#include "ddfan_guard.h"
 
 namespace ddf {
 int fail(int code, const char* fmt, ...);
@@ -289,162 +290,170 @@ extern "C" {
 
 int ddfan_ctpd_read_header(const char* path, ddfan_ctpd_header* out, float* photon_stats,
                            int capacity) {
-    if (!path || !out || capacity < 0 || (capacity > 0 && !photon_stats))
-        return fail(DDFAN_EINVAL, "ddfan_ctpd_read_header: bad arguments");
-    Dicom d;
-    if (!d.load(path, true)) return io_fail(d);
-    ddfan_ctpd_header h;
-    memset(&h, 0, sizeof h);
-    const double nan = std::nan("");
-    // required geometry (CollectReconCfg.m:22-79)
-    struct F32 {
-        uint32_t t;
-        int idx;
-        float* dst;
-    } req[] = {
-        {tag(0x7029, 0x1002), 0, &h.DetectorElementTransverseSpacing},
-        {tag(0x7029, 0x1006), 0, &h.DetectorElementAxialSpacing},
-        {tag(0x7031, 0x1001), 0, &h.DetectorFocalCenterAngularPosition},
-        {tag(0x7031, 0x1003), 0, &h.DetectorFocalCenterRadialDistance},
-        {tag(0x7031, 0x1033), 0, &h.DetectorCentralElementX},
-        {tag(0x7031, 0x1033), 1, &h.DetectorCentralElementY},
-        {tag(0x7031, 0x1031), 0, &h.ConstantRadialDistance},
-    };
-    for (const F32& r : req)
-        if (!d.f32(r.t, r.idx, r.dst)) return missing(d, r.t);
-    if (!d.us(tag(0x7029, 0x1010), &h.NumberofDetectorRows))
-        return missing(d, tag(0x7029, 0x1010));
-    if (!d.us(tag(0x7029, 0x1011), &h.NumberofDetectorColumns))
-        return missing(d, tag(0x7029, 0x1011));
-    if (!d.us(tag(0x7033, 0x1013), &h.NumberofSourceAngularSteps))
-        return missing(d, tag(0x7033, 0x1013));
-    if (!d.f64(tag(0x0018, 0x9311), 0, &h.SpiralPitchFactor))
-        return missing(d, tag(0x0018, 0x9311));
-    // optional fields: NaN / empty when absent
-    F32 opt[] = {
-        {tag(0x7031, 0x1002), 0, &h.DetectorFocalCenterAxialPosition},
-        {tag(0x7033, 0x100B), 0, &h.SourceAngularPositionShift},
-        {tag(0x7033, 0x100C), 0, &h.SourceAxialPositionShift},
-        {tag(0x7033, 0x100D), 0, &h.SourceRadialDistanceShift},
-    };
-    for (const F32& r : opt)
-        if (!d.f32(r.t, r.idx, r.dst)) *r.dst = (float)nan;
-    if (!d.ds(tag(0x0018, 0x0061), 0, &h.HUCalibrationFactor)) h.HUCalibrationFactor = nan;
-    if (!d.ds(tag(0x0018, 0x0090), 0, &h.DataCollectionDiameter)) h.DataCollectionDiameter = nan;
-    const std::string shape = d.str(tag(0x7029, 0x100B));
-    snprintf(h.DetectorShape, sizeof h.DetectorShape, "%s", shape.c_str());
-    d.us(tag(0x0028, 0x0010), &h.Rows);
-    d.us(tag(0x0028, 0x0011), &h.Columns);
-    h.PhotonStatisticsCount = d.count(tag(0x7033, 0x1065), 4);
-    for (int k = 0; k < std::min(capacity, h.PhotonStatisticsCount); ++k)
-        d.f32(tag(0x7033, 0x1065), k, &photon_stats[k]);
-    *out = h;
-    return DDFAN_OK;
+    return ddf::guarded([&]() -> int {
+        if (!path || !out || capacity < 0 || (capacity > 0 && !photon_stats))
+            return fail(DDFAN_EINVAL, "ddfan_ctpd_read_header: bad arguments");
+        Dicom d;
+        if (!d.load(path, true)) return io_fail(d);
+        ddfan_ctpd_header h;
+        memset(&h, 0, sizeof h);
+        const double nan = std::nan("");
+        // required geometry (CollectReconCfg.m:22-79)
+        struct F32 {
+            uint32_t t;
+            int idx;
+            float* dst;
+        } req[] = {
+            {tag(0x7029, 0x1002), 0, &h.DetectorElementTransverseSpacing},
+            {tag(0x7029, 0x1006), 0, &h.DetectorElementAxialSpacing},
+            {tag(0x7031, 0x1001), 0, &h.DetectorFocalCenterAngularPosition},
+            {tag(0x7031, 0x1003), 0, &h.DetectorFocalCenterRadialDistance},
+            {tag(0x7031, 0x1033), 0, &h.DetectorCentralElementX},
+            {tag(0x7031, 0x1033), 1, &h.DetectorCentralElementY},
+            {tag(0x7031, 0x1031), 0, &h.ConstantRadialDistance},
+        };
+        for (const F32& r : req)
+            if (!d.f32(r.t, r.idx, r.dst)) return missing(d, r.t);
+        if (!d.us(tag(0x7029, 0x1010), &h.NumberofDetectorRows))
+            return missing(d, tag(0x7029, 0x1010));
+        if (!d.us(tag(0x7029, 0x1011), &h.NumberofDetectorColumns))
+            return missing(d, tag(0x7029, 0x1011));
+        if (!d.us(tag(0x7033, 0x1013), &h.NumberofSourceAngularSteps))
+            return missing(d, tag(0x7033, 0x1013));
+        if (!d.f64(tag(0x0018, 0x9311), 0, &h.SpiralPitchFactor))
+            return missing(d, tag(0x0018, 0x9311));
+        // optional fields: NaN / empty when absent
+        F32 opt[] = {
+            {tag(0x7031, 0x1002), 0, &h.DetectorFocalCenterAxialPosition},
+            {tag(0x7033, 0x100B), 0, &h.SourceAngularPositionShift},
+            {tag(0x7033, 0x100C), 0, &h.SourceAxialPositionShift},
+            {tag(0x7033, 0x100D), 0, &h.SourceRadialDistanceShift},
+        };
+        for (const F32& r : opt)
+            if (!d.f32(r.t, r.idx, r.dst)) *r.dst = (float)nan;
+        if (!d.ds(tag(0x0018, 0x0061), 0, &h.HUCalibrationFactor)) h.HUCalibrationFactor = nan;
+        if (!d.ds(tag(0x0018, 0x0090), 0, &h.DataCollectionDiameter)) h.DataCollectionDiameter = nan;
+        const std::string shape = d.str(tag(0x7029, 0x100B));
+        snprintf(h.DetectorShape, sizeof h.DetectorShape, "%s", shape.c_str());
+        d.us(tag(0x0028, 0x0010), &h.Rows);
+        d.us(tag(0x0028, 0x0011), &h.Columns);
+        h.PhotonStatisticsCount = d.count(tag(0x7033, 0x1065), 4);
+        for (int k = 0; k < std::min(capacity, h.PhotonStatisticsCount); ++k)
+            d.f32(tag(0x7033, 0x1065), k, &photon_stats[k]);
+        *out = h;
+        return DDFAN_OK;
+    });
 }
 
 int ddfan_ct_image_read_header(const char* path, ddfan_ct_image_header* out) {
-    if (!path || !out) return fail(DDFAN_EINVAL, "ddfan_ct_image_read_header: bad arguments");
-    Dicom d;
-    if (!d.load(path, true)) return io_fail(d);
-    ddfan_ct_image_header h;
-    memset(&h, 0, sizeof h);
-    const double nan = std::nan("");
-    if (!d.us(tag(0x0028, 0x0011), &h.Width)) return missing(d, tag(0x0028, 0x0011));
-    if (!d.us(tag(0x0028, 0x0010), &h.Height)) return missing(d, tag(0x0028, 0x0010));
-    if (!d.ds(tag(0x0018, 0x1100), 0, &h.ReconstructionDiameter))
-        return missing(d, tag(0x0018, 0x1100));
-    if (!d.us(tag(0x0028, 0x0101), &h.BitDepth)) h.BitDepth = 0;
-    struct DS {
-        uint32_t t;
-        int idx;
-        double* dst;
-    } dss[] = {
-        {tag(0x0018, 0x0050), 0, &h.SliceThickness},
-        {tag(0x0018, 0x0060), 0, &h.KVP},
-        {tag(0x0018, 0x0090), 0, &h.DataCollectionDiameter},
-        {tag(0x0028, 0x0030), 0, &h.PixelSpacing[0]},
-        {tag(0x0028, 0x0030), 1, &h.PixelSpacing[1]},
-        {tag(0x0028, 0x1050), 0, &h.WindowCenter[0]},
-        {tag(0x0028, 0x1050), 1, &h.WindowCenter[1]},
-        {tag(0x0028, 0x1051), 0, &h.WindowWidth[0]},
-        {tag(0x0028, 0x1051), 1, &h.WindowWidth[1]},
-        {tag(0x0028, 0x1052), 0, &h.RescaleIntercept},
-        {tag(0x0028, 0x1053), 0, &h.RescaleSlope},
-    };
-    for (const DS& r : dss)
-        if (!d.ds(r.t, r.idx, r.dst)) *r.dst = nan;
-    struct FD {
-        uint32_t t;
-        int idx;
-        double* dst;
-    } fds[] = {
-        {tag(0x0018, 0x9306), 0, &h.SingleCollimationWidth},
-        {tag(0x0018, 0x9307), 0, &h.TotalCollimationWidth},
-        {tag(0x0018, 0x9309), 0, &h.TableSpeed},
-        {tag(0x0018, 0x9310), 0, &h.TableFeedPerRotation},
-        {tag(0x0018, 0x9311), 0, &h.SpiralPitchFactor},
-        {tag(0x0018, 0x9313), 0, &h.DataCollectionCenterPatient[0]},
-        {tag(0x0018, 0x9313), 1, &h.DataCollectionCenterPatient[1]},
-        {tag(0x0018, 0x9313), 2, &h.DataCollectionCenterPatient[2]},
-        {tag(0x0018, 0x9318), 0, &h.ReconstructionTargetCenterPatient[0]},
-        {tag(0x0018, 0x9318), 1, &h.ReconstructionTargetCenterPatient[1]},
-        {tag(0x0018, 0x9318), 2, &h.ReconstructionTargetCenterPatient[2]},
-    };
-    for (const FD& r : fds)
-        if (!d.f64(r.t, r.idx, r.dst)) *r.dst = nan;
-    // Smallest/LargestImagePixelValue: US or SS after PixelRepresentation
-    int rep = 0, lo = 0, hi = 0;
-    d.us(tag(0x0028, 0x0103), &rep);
-    const bool haveLo = d.us(tag(0x0028, 0x0106), &lo), haveHi = d.us(tag(0x0028, 0x0107), &hi);
-    h.SmallestImagePixelValue = !haveLo ? 0 : rep ? (int)(int16_t)lo : lo;
-    h.LargestImagePixelValue = !haveHi ? 0 : rep ? (int)(int16_t)hi : hi;
-    h.PixelSize = h.ReconstructionDiameter / h.Width;  // CollectImageCfg.m:30
-    *out = h;
-    return DDFAN_OK;
+    return ddf::guarded([&]() -> int {
+        if (!path || !out) return fail(DDFAN_EINVAL, "ddfan_ct_image_read_header: bad arguments");
+        Dicom d;
+        if (!d.load(path, true)) return io_fail(d);
+        ddfan_ct_image_header h;
+        memset(&h, 0, sizeof h);
+        const double nan = std::nan("");
+        if (!d.us(tag(0x0028, 0x0011), &h.Width)) return missing(d, tag(0x0028, 0x0011));
+        if (!d.us(tag(0x0028, 0x0010), &h.Height)) return missing(d, tag(0x0028, 0x0010));
+        if (!d.ds(tag(0x0018, 0x1100), 0, &h.ReconstructionDiameter))
+            return missing(d, tag(0x0018, 0x1100));
+        if (!d.us(tag(0x0028, 0x0101), &h.BitDepth)) h.BitDepth = 0;
+        struct DS {
+            uint32_t t;
+            int idx;
+            double* dst;
+        } dss[] = {
+            {tag(0x0018, 0x0050), 0, &h.SliceThickness},
+            {tag(0x0018, 0x0060), 0, &h.KVP},
+            {tag(0x0018, 0x0090), 0, &h.DataCollectionDiameter},
+            {tag(0x0028, 0x0030), 0, &h.PixelSpacing[0]},
+            {tag(0x0028, 0x0030), 1, &h.PixelSpacing[1]},
+            {tag(0x0028, 0x1050), 0, &h.WindowCenter[0]},
+            {tag(0x0028, 0x1050), 1, &h.WindowCenter[1]},
+            {tag(0x0028, 0x1051), 0, &h.WindowWidth[0]},
+            {tag(0x0028, 0x1051), 1, &h.WindowWidth[1]},
+            {tag(0x0028, 0x1052), 0, &h.RescaleIntercept},
+            {tag(0x0028, 0x1053), 0, &h.RescaleSlope},
+        };
+        for (const DS& r : dss)
+            if (!d.ds(r.t, r.idx, r.dst)) *r.dst = nan;
+        struct FD {
+            uint32_t t;
+            int idx;
+            double* dst;
+        } fds[] = {
+            {tag(0x0018, 0x9306), 0, &h.SingleCollimationWidth},
+            {tag(0x0018, 0x9307), 0, &h.TotalCollimationWidth},
+            {tag(0x0018, 0x9309), 0, &h.TableSpeed},
+            {tag(0x0018, 0x9310), 0, &h.TableFeedPerRotation},
+            {tag(0x0018, 0x9311), 0, &h.SpiralPitchFactor},
+            {tag(0x0018, 0x9313), 0, &h.DataCollectionCenterPatient[0]},
+            {tag(0x0018, 0x9313), 1, &h.DataCollectionCenterPatient[1]},
+            {tag(0x0018, 0x9313), 2, &h.DataCollectionCenterPatient[2]},
+            {tag(0x0018, 0x9318), 0, &h.ReconstructionTargetCenterPatient[0]},
+            {tag(0x0018, 0x9318), 1, &h.ReconstructionTargetCenterPatient[1]},
+            {tag(0x0018, 0x9318), 2, &h.ReconstructionTargetCenterPatient[2]},
+        };
+        for (const FD& r : fds)
+            if (!d.f64(r.t, r.idx, r.dst)) *r.dst = nan;
+        // Smallest/LargestImagePixelValue: US or SS after PixelRepresentation
+        int rep = 0, lo = 0, hi = 0;
+        d.us(tag(0x0028, 0x0103), &rep);
+        const bool haveLo = d.us(tag(0x0028, 0x0106), &lo), haveHi = d.us(tag(0x0028, 0x0107), &hi);
+        h.SmallestImagePixelValue = !haveLo ? 0 : rep ? (int)(int16_t)lo : lo;
+        h.LargestImagePixelValue = !haveHi ? 0 : rep ? (int)(int16_t)hi : hi;
+        h.PixelSize = h.ReconstructionDiameter / h.Width;  // CollectImageCfg.m:30
+        *out = h;
+        return DDFAN_OK;
+    });
 }
 
 int ddfan_dicom_read_u16(const char* path, int rows, int cols, uint16_t* out) {
-    if (!path || !out || rows < 1 || cols < 1)
-        return fail(DDFAN_EINVAL, "ddfan_dicom_read_u16: bad arguments");
-    Dicom d;
-    if (!read_frame(d, path, rows, cols, out)) return io_fail(d);
-    return DDFAN_OK;
+    return ddf::guarded([&]() -> int {
+        if (!path || !out || rows < 1 || cols < 1)
+            return fail(DDFAN_EINVAL, "ddfan_dicom_read_u16: bad arguments");
+        Dicom d;
+        if (!read_frame(d, path, rows, cols, out)) return io_fail(d);
+        return DDFAN_OK;
+    });
 }
 
 int ddfan_ctpd_read_frames(const char* const* paths, int nfiles, int rows, int cols, uint16_t* out,
                            int threads) {
-    if (!paths || !out || nfiles < 0 || rows < 1 || cols < 1)
-        return fail(DDFAN_EINVAL, "ddfan_ctpd_read_frames: bad arguments");
-    if (nfiles == 0) return DDFAN_OK;
-    for (int i = 0; i < nfiles; ++i)
-        if (!paths[i]) return fail(DDFAN_EINVAL, "ddfan_ctpd_read_frames: path %d is NULL", i);
-    int nt = threads > 0 ? threads : (int)std::thread::hardware_concurrency();
-    nt = std::max(1, std::min(nt, std::min(nfiles, 64)));
-    const size_t n = (size_t)rows * cols;
-    std::atomic<int> next{0};
-    std::mutex mu;
-    int firstBad = nfiles;
-    std::string firstErr;
-    auto work = [&]() {
-        Dicom d;
-        for (int i = next++; i < nfiles; i = next++) {
-            if (!read_frame(d, paths[i], rows, cols, out + (size_t)i * n)) {
-                std::lock_guard<std::mutex> lk(mu);
-                if (i < firstBad) {
-                    firstBad = i;
-                    firstErr = d.path + ": " + d.err;
+    return ddf::guarded([&]() -> int {
+        if (!paths || !out || nfiles < 0 || rows < 1 || cols < 1)
+            return fail(DDFAN_EINVAL, "ddfan_ctpd_read_frames: bad arguments");
+        if (nfiles == 0) return DDFAN_OK;
+        for (int i = 0; i < nfiles; ++i)
+            if (!paths[i]) return fail(DDFAN_EINVAL, "ddfan_ctpd_read_frames: path %d is NULL", i);
+        int nt = threads > 0 ? threads : (int)std::thread::hardware_concurrency();
+        nt = std::max(1, std::min(nt, std::min(nfiles, 64)));
+        const size_t n = (size_t)rows * cols;
+        std::atomic<int> next{0};
+        std::mutex mu;
+        int firstBad = nfiles;
+        std::string firstErr;
+        auto work = [&]() {
+            Dicom d;
+            for (int i = next++; i < nfiles; i = next++) {
+                if (!read_frame(d, paths[i], rows, cols, out + (size_t)i * n)) {
+                    std::lock_guard<std::mutex> lk(mu);
+                    if (i < firstBad) {
+                        firstBad = i;
+                        firstErr = d.path + ": " + d.err;
+                    }
+                    next = nfiles;  // stop handing out files
+                    return;
                 }
-                next = nfiles;  // stop handing out files
-                return;
             }
-        }
-    };
-    std::vector<std::thread> pool;
-    for (int t = 1; t < nt; ++t) pool.emplace_back(work);
-    work();
-    for (auto& t : pool) t.join();
-    if (firstBad < nfiles) return fail(DDFAN_EIO, "%s", firstErr.c_str());
-    return DDFAN_OK;
+        };
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nt; ++t) pool.emplace_back(work);
+        work();
+        for (auto& t : pool) t.join();
+        if (firstBad < nfiles) return fail(DDFAN_EIO, "%s", firstErr.c_str());
+        return DDFAN_OK;
+    });
 }
 
 }  // extern "C"

@@ -9,6 +9,7 @@
 #include <new>
 #include <vector>
 
+#include "ddfan_guard.h"
 #include "ddfan_plan.h"
 
 using namespace ddf;
@@ -257,119 +258,133 @@ extern "C" {
 
 int ddfan_ossart(ddfan_plan* P, const float* d_meas, float* d_x, int numIter, int numOS,
                  int useFISTA, int flags) {
-    CHECK(P && d_meas && d_x, DDFAN_EINVAL, "NULL argument");
-    HIPCALL(hipSetDevice(P->device));
-    OsJob J;
-    J.P = P;
-    J.meas = d_meas;
-    J.x = d_x;
-    J.numIter = numIter;
-    J.numOS = numOS;
-    J.fista = useFISTA ? 1 : 0;
-    SyncOnExit sync{P->stream};  // J's staging outlives the uploads
-    TRY(ossart_prepare(J, P->stream));
-    OsJob* jp = &J;
-    const bool graph = (flags & DDFAN_OSSART_GRAPH) != 0;
-    return run_iterations(&jp, 1, P->mode, BatchShape{}, numIter, graph, P->stream,
-                          (P->timing && !graph) ? P : nullptr);
+    return ddf::guarded([&]() -> int {
+        CHECK(P && d_meas && d_x, DDFAN_EINVAL, "NULL argument");
+        HIPCALL(hipSetDevice(P->device));
+        OsJob J;
+        J.P = P;
+        J.meas = d_meas;
+        J.x = d_x;
+        J.numIter = numIter;
+        J.numOS = numOS;
+        J.fista = useFISTA ? 1 : 0;
+        SyncOnExit sync{P->stream};  // J's staging outlives the uploads
+        TRY(ossart_prepare(J, P->stream));
+        OsJob* jp = &J;
+        const bool graph = (flags & DDFAN_OSSART_GRAPH) != 0;
+        return run_iterations(&jp, 1, P->mode, BatchShape{}, numIter, graph, P->stream,
+                              (P->timing && !graph) ? P : nullptr);
+    });
 }
 
 int ddfan_batch_create(int sat_mode, ddfan_batch** out) {
-    CHECK(out, DDFAN_EINVAL, "out is NULL");
-    *out = nullptr;
-    CHECK(sat_mode >= 0 && sat_mode <= 2, DDFAN_EINVAL, "sat_mode must be 0, 1 or 2");
-    int dev;
-    TRY(check_device(&dev));
-    ddfan_batch* B = new (std::nothrow) ddfan_batch();
-    CHECK(B, DDFAN_ENOMEM, "host allocation failed");
-    B->device = dev;
-    B->mode = sat_mode;
-    const hipError_t e = hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking);
-    if (e != hipSuccess) {
-        delete B;
-        HIPCALL(e);
-    }
-    *out = B;
-    return DDFAN_OK;
+    return ddf::guarded([&]() -> int {
+        CHECK(out, DDFAN_EINVAL, "out is NULL");
+        *out = nullptr;
+        CHECK(sat_mode >= 0 && sat_mode <= 2, DDFAN_EINVAL, "sat_mode must be 0, 1 or 2");
+        int dev;
+        TRY(check_device(&dev));
+        ddfan_batch* B = new (std::nothrow) ddfan_batch();
+        CHECK(B, DDFAN_ENOMEM, "host allocation failed");
+        B->device = dev;
+        B->mode = sat_mode;
+        const hipError_t e = hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete B;
+            HIPCALL(e);
+        }
+        *out = B;
+        return DDFAN_OK;
+    });
 }
 
 int ddfan_batch_add(ddfan_batch* B, ddfan_plan* P, const float* d_meas, float* d_x, int numIter,
                     int numOS, int useFISTA, int* index) {
-    CHECK(B && P && d_meas && d_x, DDFAN_EINVAL, "NULL argument");
-    CHECK(P->device == B->device, DDFAN_EINVAL, "plan and batch are on different devices");
-    CHECK(P->mode == B->mode, DDFAN_EINVAL, "plan SAT mode %d differs from the batch's %d",
-          P->mode, B->mode);
-    CHECK(numIter >= 0, DDFAN_EINVAL, "numIter must be >= 0");
-    CHECK(numOS >= 1 && numOS <= P->PN, DDFAN_EINVAL, "numOS must be in [1, %d]", P->PN);
-    for (const OsJob& J : B->jobs)
-        CHECK(J.P != P, DDFAN_EINVAL, "a plan holds the state of one reconstruction: add it once");
-    OsJob J;
-    J.P = P;
-    J.meas = d_meas;
-    J.x = d_x;
-    J.numIter = numIter;
-    J.numOS = numOS;
-    J.fista = useFISTA ? 1 : 0;
-    B->jobs.push_back(std::move(J));
-    if (index) *index = (int)B->jobs.size() - 1;
-    return DDFAN_OK;
+    return ddf::guarded([&]() -> int {
+        CHECK(B && P && d_meas && d_x, DDFAN_EINVAL, "NULL argument");
+        CHECK(P->device == B->device, DDFAN_EINVAL, "plan and batch are on different devices");
+        CHECK(P->mode == B->mode, DDFAN_EINVAL, "plan SAT mode %d differs from the batch's %d",
+              P->mode, B->mode);
+        CHECK(numIter >= 0, DDFAN_EINVAL, "numIter must be >= 0");
+        CHECK(numOS >= 1 && numOS <= P->PN, DDFAN_EINVAL, "numOS must be in [1, %d]", P->PN);
+        for (const OsJob& J : B->jobs)
+            CHECK(J.P != P, DDFAN_EINVAL, "a plan holds the state of one reconstruction: add it once");
+        OsJob J;
+        J.P = P;
+        J.meas = d_meas;
+        J.x = d_x;
+        J.numIter = numIter;
+        J.numOS = numOS;
+        J.fista = useFISTA ? 1 : 0;
+        B->jobs.push_back(std::move(J));
+        if (index) *index = (int)B->jobs.size() - 1;
+        return DDFAN_OK;
+    });
 }
 
 int ddfan_batch_set_param(ddfan_batch* B, int param, long long value) {
-    CHECK(B, DDFAN_EINVAL, "batch is NULL");
-    switch (param) {
-        case DDFAN_BATCH_PROJ_FILL:
-            CHECK(value >= 1, DDFAN_EINVAL, "projector launch limit must be >= 1");
-            B->shape.projFill = value;
-            return DDFAN_OK;
-        case DDFAN_BATCH_BACK_FILL:
-            CHECK(value >= 1, DDFAN_EINVAL, "backprojector launch limit must be >= 1");
-            B->shape.backFill = value;
-            return DDFAN_OK;
-        default:
-            return fail(DDFAN_EINVAL, "unknown batch parameter %d", param);
-    }
+    return ddf::guarded([&]() -> int {
+        CHECK(B, DDFAN_EINVAL, "batch is NULL");
+        switch (param) {
+            case DDFAN_BATCH_PROJ_FILL:
+                CHECK(value >= 1, DDFAN_EINVAL, "projector launch limit must be >= 1");
+                B->shape.projFill = value;
+                return DDFAN_OK;
+            case DDFAN_BATCH_BACK_FILL:
+                CHECK(value >= 1, DDFAN_EINVAL, "backprojector launch limit must be >= 1");
+                B->shape.backFill = value;
+                return DDFAN_OK;
+            default:
+                return fail(DDFAN_EINVAL, "unknown batch parameter %d", param);
+        }
+    });
 }
 
 int ddfan_batch_size(const ddfan_batch* B, int* n) {
-    CHECK(B && n, DDFAN_EINVAL, "NULL argument");
-    *n = (int)B->jobs.size();
-    return DDFAN_OK;
+    return ddf::guarded([&]() -> int {
+        CHECK(B && n, DDFAN_EINVAL, "NULL argument");
+        *n = (int)B->jobs.size();
+        return DDFAN_OK;
+    });
 }
 
 int ddfan_batch_run(ddfan_batch* B, int flags) {
-    CHECK(B, DDFAN_EINVAL, "batch is NULL");
-    HIPCALL(hipSetDevice(B->device));
-    const int n = (int)B->jobs.size();
-    if (n == 0) return DDFAN_OK;
-    SyncOnExit sync{B->stream};
-    for (OsJob& J : B->jobs) TRY(ossart_prepare(J, B->stream));
-    // Segments of constant membership: a job leaves after its last iteration.
-    std::vector<int> left(n);
-    for (int i = 0; i < n; ++i) left[i] = B->jobs[i].numIter;
-    const bool graph = (flags & DDFAN_OSSART_GRAPH) != 0;
-    for (;;) {
-        std::vector<OsJob*> act;
-        int seg = 0;
-        for (int i = 0; i < n; ++i)
-            if (left[i] > 0) {
-                act.push_back(&B->jobs[i]);
-                seg = seg == 0 ? left[i] : std::min(seg, left[i]);
-            }
-        if (act.empty()) break;
-        TRY(run_iterations(act.data(), (int)act.size(), B->mode, B->shape, seg, graph, B->stream,
-                           nullptr));
-        for (int i = 0; i < n; ++i)
-            if (left[i] > 0) left[i] -= seg;
-    }
-    return DDFAN_OK;
+    return ddf::guarded([&]() -> int {
+        CHECK(B, DDFAN_EINVAL, "batch is NULL");
+        HIPCALL(hipSetDevice(B->device));
+        const int n = (int)B->jobs.size();
+        if (n == 0) return DDFAN_OK;
+        SyncOnExit sync{B->stream};
+        for (OsJob& J : B->jobs) TRY(ossart_prepare(J, B->stream));
+        // Segments of constant membership: a job leaves after its last iteration.
+        std::vector<int> left(n);
+        for (int i = 0; i < n; ++i) left[i] = B->jobs[i].numIter;
+        const bool graph = (flags & DDFAN_OSSART_GRAPH) != 0;
+        for (;;) {
+            std::vector<OsJob*> act;
+            int seg = 0;
+            for (int i = 0; i < n; ++i)
+                if (left[i] > 0) {
+                    act.push_back(&B->jobs[i]);
+                    seg = seg == 0 ? left[i] : std::min(seg, left[i]);
+                }
+            if (act.empty()) break;
+            TRY(run_iterations(act.data(), (int)act.size(), B->mode, B->shape, seg, graph, B->stream,
+                               nullptr));
+            for (int i = 0; i < n; ++i)
+                if (left[i] > 0) left[i] -= seg;
+        }
+        return DDFAN_OK;
+    });
 }
 
 int ddfan_batch_destroy(ddfan_batch* B) {
-    if (!B) return DDFAN_OK;
-    (void)hipSetDevice(B->device);
-    delete B;
-    return DDFAN_OK;
+    return ddf::guarded([&]() -> int {
+        if (!B) return DDFAN_OK;
+        (void)hipSetDevice(B->device);
+        delete B;
+        return DDFAN_OK;
+    });
 }
 
 }  // extern "C"

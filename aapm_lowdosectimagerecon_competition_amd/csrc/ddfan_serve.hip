@@ -26,6 +26,7 @@
 #include <thread>
 #include <vector>
 
+#include "ddfan_guard.h"
 #include "ddfan_plan.h"
 #include "ddfan_sched.h"
 
@@ -42,6 +43,7 @@ double ms_between(Clock::time_point a, Clock::time_point b) {
 // memcpy split over up to 8 threads for large buffers.  A fresh destination
 // is faulted in by the copying threads, so the first touch of its pages runs
 // in parallel too (a 434 MB sinogram: 87 ms single-threaded into a new vector).
+// A thread that cannot be started leaves its share to the calling thread.
 void parallel_copy(void* dst, const void* src, size_t bytes) {
     constexpr size_t kMinPerThread = 16u << 20;
     const size_t hw = std::max(1u, std::thread::hardware_concurrency());
@@ -57,7 +59,12 @@ void parallel_copy(void* dst, const void* src, size_t bytes) {
         const size_t a = i * step;
         if (a >= bytes) break;
         const size_t n = std::min(step, bytes - a);
-        th.emplace_back([=] { memcpy((char*)dst + a, (const char*)src + a, n); });
+        auto part = [=] { memcpy((char*)dst + a, (const char*)src + a, n); };
+        try {
+            th.emplace_back(part);
+        } catch (...) {
+            part();
+        }
     }
     for (auto& t : th) t.join();
 }
@@ -70,13 +77,15 @@ struct HostFloats {
     size_t size() const { return n; }
     float* data() { return p.get(); }
     const float* data() const { return p.get(); }
-    void alloc(size_t count) {
-        p.reset(count ? new float[count] : nullptr);
-        n = count;
+    bool alloc(size_t count) {
+        p.reset(count ? new (std::nothrow) float[count] : nullptr);
+        n = p ? count : 0;
+        return count == 0 || p != nullptr;
     }
-    void assign_copy(const float* src, size_t count) {
-        alloc(count);
+    bool assign_copy(const float* src, size_t count) {
+        if (!alloc(count)) return false;
         parallel_copy(p.get(), src, count * sizeof(float));
+        return true;
     }
 };
 
@@ -223,12 +232,14 @@ void ddfan_server::loop() {
                 continue;
             }
             const size_t nv = (size_t)R->G.XN * R->G.YN * R->G.SLN;
-            R->result.alloc(nv);
             int st = DDFAN_OK;
             std::string err;
-            if (hipMemcpyAsync(R->result.data(), R->d_x.p, nv * sizeof(float),
-                               hipMemcpyDeviceToHost, copyStream) != hipSuccess ||
-                hipStreamSynchronize(copyStream) != hipSuccess) {
+            if (!R->result.alloc(nv)) {
+                st = DDFAN_ENOMEM;
+                err = "host allocation of the result failed";
+            } else if (hipMemcpyAsync(R->result.data(), R->d_x.p, nv * sizeof(float),
+                                      hipMemcpyDeviceToHost, copyStream) != hipSuccess ||
+                       hipStreamSynchronize(copyStream) != hipSuccess) {
                 st = DDFAN_EDEVICE;
                 err = "result copy failed";
             }
@@ -347,136 +358,149 @@ void ddfan_server::loop() {
 extern "C" {
 
 int ddfan_server_create(size_t hbm_budget, int max_slots, int sat_mode, ddfan_server** out) {
-    CHECK(out, DDFAN_EINVAL, "out is NULL");
-    *out = nullptr;
-    CHECK(max_slots >= 1, DDFAN_EINVAL, "max_slots must be >= 1");
-    CHECK(sat_mode >= 0 && sat_mode <= 2, DDFAN_EINVAL, "sat_mode must be 0, 1 or 2");
-    int dev;
-    TRY(check_device(&dev));
-    if (hbm_budget == 0) {
-        size_t fb = 0, tb = 0;
-        HIPCALL(hipMemGetInfo(&fb, &tb));
-        hbm_budget = fb / 10 * 8;  // 80% of what is free now
-    }
-    ddfan_server* S = new (std::nothrow) ddfan_server(hbm_budget, max_slots);
-    CHECK(S, DDFAN_ENOMEM, "host allocation failed");
-    S->device = dev;
-    S->mode = sat_mode;
-    hipError_t e = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&S->copyStream, hipStreamNonBlocking);
-    if (e != hipSuccess) {
-        if (S->stream) (void)hipStreamDestroy(S->stream);
-        delete S;
-        HIPCALL(e);
-    }
-    try {
-        S->worker = std::thread([S] { S->loop(); });
-    } catch (...) {
-        (void)hipStreamDestroy(S->stream);
-        (void)hipStreamDestroy(S->copyStream);
-        delete S;
-        return fail(DDFAN_ENOMEM, "cannot start the worker thread");
-    }
-    *out = S;
-    return DDFAN_OK;
+    return ddf::guarded([&]() -> int {
+        CHECK(out, DDFAN_EINVAL, "out is NULL");
+        *out = nullptr;
+        CHECK(max_slots >= 1, DDFAN_EINVAL, "max_slots must be >= 1");
+        CHECK(sat_mode >= 0 && sat_mode <= 2, DDFAN_EINVAL, "sat_mode must be 0, 1 or 2");
+        int dev;
+        TRY(check_device(&dev));
+        if (hbm_budget == 0) {
+            size_t fb = 0, tb = 0;
+            HIPCALL(hipMemGetInfo(&fb, &tb));
+            hbm_budget = fb / 10 * 8;  // 80% of what is free now
+        }
+        ddfan_server* S = new (std::nothrow) ddfan_server(hbm_budget, max_slots);
+        CHECK(S, DDFAN_ENOMEM, "host allocation failed");
+        S->device = dev;
+        S->mode = sat_mode;
+        hipError_t e = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&S->copyStream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            if (S->stream) (void)hipStreamDestroy(S->stream);
+            delete S;
+            HIPCALL(e);
+        }
+        try {
+            S->worker = std::thread([S] { S->loop(); });
+        } catch (...) {
+            (void)hipStreamDestroy(S->stream);
+            (void)hipStreamDestroy(S->copyStream);
+            delete S;
+            return fail(DDFAN_ENOMEM, "cannot start the worker thread");
+        }
+        *out = S;
+        return DDFAN_OK;
+    });
 }
 
 int ddfan_server_submit(ddfan_server* S, const ddfan_geom* G, const float* hangs, int PN,
                         const unsigned char* mask, const float* meas, const float* x0,
                         int numIter, int numOS, int useFISTA, long long* id) {
-    CHECK(S && G && hangs && meas && id, DDFAN_EINVAL, "NULL argument");
-    CHECK(G->XN >= 1 && G->YN >= 1 && G->SLN >= 1 && G->DNU >= 2 && G->xds && G->yds,
-          DDFAN_EINVAL, "bad geometry");
-    CHECK(PN >= 1 && numIter >= 0 && numOS >= 1 && numOS <= PN, DDFAN_EINVAL,
-          "bad view or iteration counts (PN %d numIter %d numOS %d)", PN, numIter, numOS);
-    size_t bytes = 0;
-    TRY(request_bytes(G, PN, S->mode, numOS, &bytes));
-    auto R = std::make_unique<Request>();
-    R->G = *G;
-    R->xds.assign(G->xds, G->xds + G->DNU);
-    R->yds.assign(G->yds, G->yds + G->DNU);
-    R->hangs.assign(hangs, hangs + (size_t)PN * G->SLN);
-    R->meas.assign_copy(meas, (size_t)PN * G->DNU * G->SLN);
-    const size_t nv = (size_t)G->XN * G->YN * G->SLN;
-    if (x0) R->x0.assign_copy(x0, nv);
-    if (mask) R->mask.assign(mask, mask + (size_t)G->XN * G->YN);
-    R->PN = PN;
-    R->numIter = numIter;
-    R->numOS = numOS;
-    R->fista = useFISTA ? 1 : 0;
-    R->bytes = bytes;
-    R->tSubmit = Clock::now();
-    std::lock_guard<std::mutex> lk(S->mu);
-    CHECK(!S->stop, DDFAN_EINVAL, "server is shutting down");
-    const long long rid = S->nextId;
-    CHECK(S->adm.submit(rid, bytes), DDFAN_EBUSY,
-          "request needs %zu device bytes, more than the whole budget of %zu", bytes,
-          S->adm.budget());
-    ++S->nextId;
-    R->id = rid;
-    S->reqs[rid] = std::move(R);
-    *id = rid;
-    S->cvWork.notify_all();
-    return DDFAN_OK;
+    return ddf::guarded([&]() -> int {
+        CHECK(S && G && hangs && meas && id, DDFAN_EINVAL, "NULL argument");
+        CHECK(G->XN >= 1 && G->YN >= 1 && G->SLN >= 1 && G->DNU >= 2 && G->xds && G->yds,
+              DDFAN_EINVAL, "bad geometry");
+        CHECK(PN >= 1 && numIter >= 0 && numOS >= 1 && numOS <= PN, DDFAN_EINVAL,
+              "bad view or iteration counts (PN %d numIter %d numOS %d)", PN, numIter, numOS);
+        size_t bytes = 0;
+        TRY(request_bytes(G, PN, S->mode, numOS, &bytes));
+        auto R = std::make_unique<Request>();
+        R->G = *G;
+        R->xds.assign(G->xds, G->xds + G->DNU);
+        R->yds.assign(G->yds, G->yds + G->DNU);
+        R->hangs.assign(hangs, hangs + (size_t)PN * G->SLN);
+        const size_t nv = (size_t)G->XN * G->YN * G->SLN;
+        CHECK(R->meas.assign_copy(meas, (size_t)PN * G->DNU * G->SLN) &&
+                  (!x0 || R->x0.assign_copy(x0, nv)),
+              DDFAN_ENOMEM, "host copy of the request's inputs failed");
+        if (mask) R->mask.assign(mask, mask + (size_t)G->XN * G->YN);
+        R->PN = PN;
+        R->numIter = numIter;
+        R->numOS = numOS;
+        R->fista = useFISTA ? 1 : 0;
+        R->bytes = bytes;
+        R->tSubmit = Clock::now();
+        std::lock_guard<std::mutex> lk(S->mu);
+        CHECK(!S->stop, DDFAN_EINVAL, "server is shutting down");
+        const long long rid = S->nextId;
+        CHECK(S->adm.submit(rid, bytes), DDFAN_EBUSY,
+              "request needs %zu device bytes, more than the whole budget of %zu", bytes,
+              S->adm.budget());
+        ++S->nextId;
+        R->id = rid;
+        S->reqs[rid] = std::move(R);
+        *id = rid;
+        S->cvWork.notify_all();
+        return DDFAN_OK;
+    });
 }
 
 int ddfan_server_poll(ddfan_server* S, long long id, int* done) {
-    CHECK(S && done, DDFAN_EINVAL, "NULL argument");
-    std::lock_guard<std::mutex> lk(S->mu);
-    auto it = S->reqs.find(id);
-    CHECK(it != S->reqs.end(), DDFAN_EINVAL, "unknown request %lld", id);
-    *done = it->second->done ? 1 : 0;
-    return DDFAN_OK;
+    return ddf::guarded([&]() -> int {
+        CHECK(S && done, DDFAN_EINVAL, "NULL argument");
+        std::lock_guard<std::mutex> lk(S->mu);
+        auto it = S->reqs.find(id);
+        CHECK(it != S->reqs.end(), DDFAN_EINVAL, "unknown request %lld", id);
+        *done = it->second->done ? 1 : 0;
+        return DDFAN_OK;
+    });
 }
 
 int ddfan_server_wait(ddfan_server* S, long long id, float* x, double* queue_ms,
                       double* latency_ms) {
-    CHECK(S, DDFAN_EINVAL, "server is NULL");
-    std::unique_ptr<Request> R;
-    {
-        std::unique_lock<std::mutex> lk(S->mu);
-        auto it = S->reqs.find(id);
-        CHECK(it != S->reqs.end(), DDFAN_EINVAL, "unknown request %lld", id);
-        Request* r = it->second.get();
-        S->cvDone.wait(lk, [&] { return r->done; });
-        R = std::move(it->second);
-        S->reqs.erase(it);
-    }
-    if (queue_ms) *queue_ms = R->status == DDFAN_OK ? ms_between(R->tSubmit, R->tAdmit) : 0.0;
-    if (latency_ms) *latency_ms = ms_between(R->tSubmit, R->tDone);
-    if (R->status != DDFAN_OK)
-        return fail(R->status, "request %lld failed: %s", id, R->err.c_str());
-    if (x) parallel_copy(x, R->result.data(), R->result.size() * sizeof(float));
-    return DDFAN_OK;
+    return ddf::guarded([&]() -> int {
+        CHECK(S, DDFAN_EINVAL, "server is NULL");
+        std::unique_ptr<Request> R;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            auto it = S->reqs.find(id);
+            CHECK(it != S->reqs.end(), DDFAN_EINVAL, "unknown request %lld", id);
+            Request* r = it->second.get();
+            S->cvDone.wait(lk, [&] { return r->done; });
+            R = std::move(it->second);
+            S->reqs.erase(it);
+        }
+        if (queue_ms) *queue_ms = R->status == DDFAN_OK ? ms_between(R->tSubmit, R->tAdmit) : 0.0;
+        if (latency_ms) *latency_ms = ms_between(R->tSubmit, R->tDone);
+        if (R->status != DDFAN_OK)
+            return fail(R->status, "request %lld failed: %s", id, R->err.c_str());
+        if (x) parallel_copy(x, R->result.data(), R->result.size() * sizeof(float));
+        return DDFAN_OK;
+    });
 }
 
 int ddfan_server_stats(ddfan_server* S, long long* completed, long long* failed, int* running,
                        long long* waiting, size_t* used_bytes, double* voxel_views) {
-    CHECK(S, DDFAN_EINVAL, "server is NULL");
-    std::lock_guard<std::mutex> lk(S->mu);
-    if (completed) *completed = S->completed;
-    if (failed) *failed = S->failed;
-    if (running) *running = S->adm.active();
-    if (waiting) *waiting = (long long)S->adm.waiting();
-    if (used_bytes) *used_bytes = S->adm.used();
-    if (voxel_views) *voxel_views = S->busyVoxelViews;
-    return DDFAN_OK;
+    return ddf::guarded([&]() -> int {
+        CHECK(S, DDFAN_EINVAL, "server is NULL");
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (completed) *completed = S->completed;
+        if (failed) *failed = S->failed;
+        if (running) *running = S->adm.active();
+        if (waiting) *waiting = (long long)S->adm.waiting();
+        if (used_bytes) *used_bytes = S->adm.used();
+        if (voxel_views) *voxel_views = S->busyVoxelViews;
+        return DDFAN_OK;
+    });
 }
 
 int ddfan_server_destroy(ddfan_server* S) {
-    if (!S) return DDFAN_OK;
-    {
-        std::lock_guard<std::mutex> lk(S->mu);
-        S->stop = true;
-        S->cvWork.notify_all();
-    }
-    if (S->worker.joinable()) S->worker.join();
-    (void)hipSetDevice(S->device);
-    S->reqs.clear();
-    (void)hipStreamDestroy(S->stream);
-    (void)hipStreamDestroy(S->copyStream);
-    delete S;
-    return DDFAN_OK;
+    return ddf::guarded([&]() -> int {
+        if (!S) return DDFAN_OK;
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->stop = true;
+            S->cvWork.notify_all();
+        }
+        if (S->worker.joinable()) S->worker.join();
+        (void)hipSetDevice(S->device);
+        S->reqs.clear();
+        (void)hipStreamDestroy(S->stream);
+        (void)hipStreamDestroy(S->copyStream);
+        delete S;
+        return DDFAN_OK;
+    });
 }
 
 }  // extern "C"

@@ -21,6 +21,7 @@
 #include <tuple>
 #include <vector>
 
+#include "ddfan_guard.h"
 #include "ddfan_plan.h"
 
 using namespace ddf;
@@ -96,210 +97,220 @@ extern "C" {
 
 int ddfan_plan_autotune(ddfan_plan* P, int views, int reps, int flags, int* out_params,
                         double* out_ms) {
-    CHECK(P, DDFAN_EINVAL, "plan is NULL");
-    if (views <= 0) views = P->PN;
-    CHECK(views <= P->PN, DDFAN_EINVAL, "views %d exceeds the plan's %d", views, P->PN);
-    reps = std::max(1, std::min(reps, 20));
-    const TuneKey key = key_of(P, views);
-    TuneVal best{};
-    bool have = false;
-    if (flags & DDFAN_TUNE_USE_CACHE) {
-        std::lock_guard<std::mutex> lk(g_tuneMu);
-        auto it = g_tune.find(key);
-        if (it != g_tune.end()) {
-            best = it->second;
-            have = true;
+    return ddf::guarded([&]() -> int {
+        CHECK(P, DDFAN_EINVAL, "plan is NULL");
+        if (views <= 0) views = P->PN;
+        CHECK(views <= P->PN, DDFAN_EINVAL, "views %d exceeds the plan's %d", views, P->PN);
+        reps = std::max(1, std::min(reps, 20));
+        const TuneKey key = key_of(P, views);
+        TuneVal best{};
+        bool have = false;
+        if (flags & DDFAN_TUNE_USE_CACHE) {
+            std::lock_guard<std::mutex> lk(g_tuneMu);
+            auto it = g_tune.find(key);
+            if (it != g_tune.end()) {
+                best = it->second;
+                have = true;
+            }
         }
-    }
-    if (!have) {
-        HIPCALL(hipSetDevice(P->device));
-        const GeomDev& g = P->g;
-        const size_t nvox = (size_t)g.XN * g.YN * g.SLN;
-        const size_t nprj = (size_t)views * g.DNU * g.SLN;
-        DBuf vol, prj;
-        HIPCALL(vol.reserve(nvox * sizeof(float)));
-        HIPCALL(prj.reserve(nprj * sizeof(float)));
-        HIPCALL(launch_fill(vol.as<float>(), 1.0f, nvox, P->stream));
-        HIPCALL(launch_fill(prj.as<float>(), 1.0f, nprj, P->stream));
-        // the plan's own settings are restored if tuning fails part-way
-        const TuneVal saved{P->ZT,        P->viewsPerWave, P->slabTile, P->ZTb,
-                            P->backStage, P->projStage,    0.0,         0.0};
-        int rc = [&]() -> int {
-            std::vector<int> chunks;
-            for (int z = std::min(P->ZTmax, 16); z >= 1; z >>= 1) chunks.push_back(z);
-            // launches of `views` views spaced PN / views apart: an OS-SART
-            // subset (consecutive views when views = PN); the spacing decides
-            // how much of a table row neighbouring views of a launch share
-            const int stride = std::max(1, P->PN / views);
-            auto proj = [&]() {
-                return ddfan_plan_proj(P, vol.as<float>(), prj.as<float>(), 0, stride, views);
-            };
-            auto back = [&]() {
-                return ddfan_plan_back(P, prj.as<float>(), vol.as<float>(), 0, stride, views);
-            };
-            double bestP = 1e30;
-            TRY(ddfan_plan_set_param(P, DDFAN_PARAM_SLAB_TILE, 0));
-            for (int z : chunks) {
-                TRY(ddfan_plan_set_param(P, DDFAN_PARAM_PROJ_CHUNK, z));
-                for (int v : {4, 2, 1}) {
-                    TRY(ddfan_plan_set_param(P, DDFAN_PARAM_VIEWS_PER_WAVE, v));
-                    for (int ps : {1, 0}) {
-                        TRY(ddfan_plan_set_param(P, DDFAN_PARAM_PROJ_STAGE, ps));
-                        double ms;
-                        TRY(time_ms(P->stream, reps, proj, &ms));
-                        if (ms < bestP) {
-                            bestP = ms;
-                            best.ZT = z;
-                            best.V = v;
-                            best.tile = 0;
-                            best.pstage = ps;
+        if (!have) {
+            HIPCALL(hipSetDevice(P->device));
+            const GeomDev& g = P->g;
+            const size_t nvox = (size_t)g.XN * g.YN * g.SLN;
+            const size_t nprj = (size_t)views * g.DNU * g.SLN;
+            DBuf vol, prj;
+            HIPCALL(vol.reserve(nvox * sizeof(float)));
+            HIPCALL(prj.reserve(nprj * sizeof(float)));
+            HIPCALL(launch_fill(vol.as<float>(), 1.0f, nvox, P->stream));
+            HIPCALL(launch_fill(prj.as<float>(), 1.0f, nprj, P->stream));
+            // the plan's own settings are restored if tuning fails part-way
+            const TuneVal saved{P->ZT,        P->viewsPerWave, P->slabTile, P->ZTb,
+                                P->backStage, P->projStage,    0.0,         0.0};
+            int rc = [&]() -> int {
+                std::vector<int> chunks;
+                for (int z = std::min(P->ZTmax, 16); z >= 1; z >>= 1) chunks.push_back(z);
+                // launches of `views` views spaced PN / views apart: an OS-SART
+                // subset (consecutive views when views = PN); the spacing decides
+                // how much of a table row neighbouring views of a launch share
+                const int stride = std::max(1, P->PN / views);
+                auto proj = [&]() {
+                    return ddfan_plan_proj(P, vol.as<float>(), prj.as<float>(), 0, stride, views);
+                };
+                auto back = [&]() {
+                    return ddfan_plan_back(P, prj.as<float>(), vol.as<float>(), 0, stride, views);
+                };
+                double bestP = 1e30;
+                TRY(ddfan_plan_set_param(P, DDFAN_PARAM_SLAB_TILE, 0));
+                for (int z : chunks) {
+                    TRY(ddfan_plan_set_param(P, DDFAN_PARAM_PROJ_CHUNK, z));
+                    for (int v : {4, 2, 1}) {
+                        TRY(ddfan_plan_set_param(P, DDFAN_PARAM_VIEWS_PER_WAVE, v));
+                        for (int ps : {1, 0}) {
+                            TRY(ddfan_plan_set_param(P, DDFAN_PARAM_PROJ_STAGE, ps));
+                            double ms;
+                            TRY(time_ms(P->stream, reps, proj, &ms));
+                            if (ms < bestP) {
+                                bestP = ms;
+                                best.ZT = z;
+                                best.V = v;
+                                best.tile = 0;
+                                best.pstage = ps;
+                            }
                         }
                     }
                 }
-            }
-            TRY(ddfan_plan_set_param(P, DDFAN_PARAM_PROJ_CHUNK, best.ZT));
-            TRY(ddfan_plan_set_param(P, DDFAN_PARAM_VIEWS_PER_WAVE, best.V));
-            TRY(ddfan_plan_set_param(P, DDFAN_PARAM_PROJ_STAGE, best.pstage));
-            const int nmax = std::max(g.XN, g.YN);
-            for (int t : {256, 128, 64}) {
-                if (t >= nmax) continue;
-                TRY(ddfan_plan_set_param(P, DDFAN_PARAM_SLAB_TILE, t));
-                double ms;
-                TRY(time_ms(P->stream, reps, proj, &ms));
-                if (ms < bestP) {
-                    bestP = ms;
-                    best.tile = t;
-                }
-            }
-            best.projMs = bestP;
-            double bestB = 1e30;
-            for (int z : chunks) {
-                TRY(ddfan_plan_set_param(P, DDFAN_PARAM_BACK_STAGE, 0));
-                TRY(ddfan_plan_set_param(P, DDFAN_PARAM_BACK_CHUNK, z));
-                for (int st : {1, 2, 4, 0}) {
-                    if (!stage_fits(st, z)) continue;
-                    TRY(ddfan_plan_set_param(P, DDFAN_PARAM_BACK_STAGE, st));
+                TRY(ddfan_plan_set_param(P, DDFAN_PARAM_PROJ_CHUNK, best.ZT));
+                TRY(ddfan_plan_set_param(P, DDFAN_PARAM_VIEWS_PER_WAVE, best.V));
+                TRY(ddfan_plan_set_param(P, DDFAN_PARAM_PROJ_STAGE, best.pstage));
+                const int nmax = std::max(g.XN, g.YN);
+                for (int t : {256, 128, 64}) {
+                    if (t >= nmax) continue;
+                    TRY(ddfan_plan_set_param(P, DDFAN_PARAM_SLAB_TILE, t));
                     double ms;
-                    TRY(time_ms(P->stream, reps, back, &ms));
-                    if (ms < bestB) {
-                        bestB = ms;
-                        best.ZTb = z;
-                        best.stage = st;
+                    TRY(time_ms(P->stream, reps, proj, &ms));
+                    if (ms < bestP) {
+                        bestP = ms;
+                        best.tile = t;
                     }
                 }
+                best.projMs = bestP;
+                double bestB = 1e30;
+                for (int z : chunks) {
+                    TRY(ddfan_plan_set_param(P, DDFAN_PARAM_BACK_STAGE, 0));
+                    TRY(ddfan_plan_set_param(P, DDFAN_PARAM_BACK_CHUNK, z));
+                    for (int st : {1, 2, 4, 0}) {
+                        if (!stage_fits(st, z)) continue;
+                        TRY(ddfan_plan_set_param(P, DDFAN_PARAM_BACK_STAGE, st));
+                        double ms;
+                        TRY(time_ms(P->stream, reps, back, &ms));
+                        if (ms < bestB) {
+                            bestB = ms;
+                            best.ZTb = z;
+                            best.stage = st;
+                        }
+                    }
+                }
+                best.backMs = bestB;
+                return DDFAN_OK;
+            }();
+            HIPCALL(hipStreamSynchronize(P->stream));
+            if (rc != DDFAN_OK) {
+                (void)apply(P, saved);
+                return rc;
             }
-            best.backMs = bestB;
-            return DDFAN_OK;
-        }();
-        HIPCALL(hipStreamSynchronize(P->stream));
-        if (rc != DDFAN_OK) {
-            (void)apply(P, saved);
-            return rc;
+            std::lock_guard<std::mutex> lk(g_tuneMu);
+            g_tune[key] = best;
         }
-        std::lock_guard<std::mutex> lk(g_tuneMu);
-        g_tune[key] = best;
-    }
-    TRY(apply(P, best));
-    if (out_params) {
-        out_params[0] = best.ZT;
-        out_params[1] = best.V;
-        out_params[2] = best.tile;
-        out_params[3] = best.ZTb;
-        out_params[4] = best.stage;
-        out_params[5] = best.pstage;
-    }
-    if (out_ms) {
-        out_ms[0] = best.projMs;
-        out_ms[1] = best.backMs;
-    }
-    return DDFAN_OK;
+        TRY(apply(P, best));
+        if (out_params) {
+            out_params[0] = best.ZT;
+            out_params[1] = best.V;
+            out_params[2] = best.tile;
+            out_params[3] = best.ZTb;
+            out_params[4] = best.stage;
+            out_params[5] = best.pstage;
+        }
+        if (out_ms) {
+            out_ms[0] = best.projMs;
+            out_ms[1] = best.backMs;
+        }
+        return DDFAN_OK;
+    });
 }
 
 int ddfan_plan_apply_tuned(ddfan_plan* P, int views, int* found) {
-    CHECK(P && found, DDFAN_EINVAL, "NULL argument");
-    if (views <= 0) views = P->PN;
-    TuneVal v{};
-    {
-        std::lock_guard<std::mutex> lk(g_tuneMu);
-        auto it = g_tune.find(key_of(P, views));
-        *found = it != g_tune.end();
-        if (!*found) return DDFAN_OK;
-        v = it->second;
-    }
-    return apply(P, v);
+    return ddf::guarded([&]() -> int {
+        CHECK(P && found, DDFAN_EINVAL, "NULL argument");
+        if (views <= 0) views = P->PN;
+        TuneVal v{};
+        {
+            std::lock_guard<std::mutex> lk(g_tuneMu);
+            auto it = g_tune.find(key_of(P, views));
+            *found = it != g_tune.end();
+            if (!*found) return DDFAN_OK;
+            v = it->second;
+        }
+        return apply(P, v);
+    });
 }
 
 // One line per entry (bstage: backprojector staged views, pstage: projector
 // staging):
 // mode XN YN SLN DNU ZTmax views  ZT V tile ZTb bstage pstage  proj_ms back_ms
 int ddfan_tune_save(const char* path) {
-    CHECK(path, DDFAN_EINVAL, "path is NULL");
-    FILE* f = fopen(path, "w");
-    CHECK(f, DDFAN_EINVAL, "cannot write %s", path);
-    fprintf(f, "# ddfan launch-shape table: mode XN YN SLN DNU ZTmax views  ZT V tile ZTb bstage "
-               "pstage  proj_ms back_ms\n");
-    std::lock_guard<std::mutex> lk(g_tuneMu);
-    for (const auto& kv : g_tune) {
-        const TuneKey& k = kv.first;
-        const TuneVal& v = kv.second;
-        fprintf(f, "%d %d %d %d %d %d %d  %d %d %d %d %d %d  %.4f %.4f\n", k.mode, k.XN, k.YN,
-                k.SLN, k.DNU, k.ZTmax, k.views, v.ZT, v.V, v.tile, v.ZTb, v.stage, v.pstage,
-                v.projMs, v.backMs);
-    }
-    const int bad = ferror(f);
-    fclose(f);
-    CHECK(!bad, DDFAN_EINVAL, "write to %s failed", path);
-    return DDFAN_OK;
+    return ddf::guarded([&]() -> int {
+        CHECK(path, DDFAN_EINVAL, "path is NULL");
+        FILE* f = fopen(path, "w");
+        CHECK(f, DDFAN_EINVAL, "cannot write %s", path);
+        fprintf(f, "# ddfan launch-shape table: mode XN YN SLN DNU ZTmax views  ZT V tile ZTb bstage "
+                   "pstage  proj_ms back_ms\n");
+        std::lock_guard<std::mutex> lk(g_tuneMu);
+        for (const auto& kv : g_tune) {
+            const TuneKey& k = kv.first;
+            const TuneVal& v = kv.second;
+            fprintf(f, "%d %d %d %d %d %d %d  %d %d %d %d %d %d  %.4f %.4f\n", k.mode, k.XN, k.YN,
+                    k.SLN, k.DNU, k.ZTmax, k.views, v.ZT, v.V, v.tile, v.ZTb, v.stage, v.pstage,
+                    v.projMs, v.backMs);
+        }
+        const int bad = ferror(f);
+        fclose(f);
+        CHECK(!bad, DDFAN_EINVAL, "write to %s failed", path);
+        return DDFAN_OK;
+    });
 }
 
 int ddfan_tune_load(const char* path, int* entries) {
-    CHECK(path, DDFAN_EINVAL, "path is NULL");
-    FILE* f = fopen(path, "r");
-    CHECK(f, DDFAN_EINVAL, "cannot read %s", path);
-    char line[512];
-    int n = 0, lineNo = 0, bad = 0;
-    std::vector<std::pair<TuneKey, TuneVal>> rows;
-    while (fgets(line, sizeof line, f)) {
-        ++lineNo;
-        if (line[0] == '#' || line[0] == '\n') continue;
-        TuneKey k{};
-        TuneVal v{};
-        int tokens = 0;
-        for (const char* c = line; *c;) {
-            while (*c == ' ' || *c == '\t' || *c == '\n' || *c == '\r') ++c;
-            if (!*c) break;
-            ++tokens;
-            while (*c && *c != ' ' && *c != '\t' && *c != '\n' && *c != '\r') ++c;
+    return ddf::guarded([&]() -> int {
+        CHECK(path, DDFAN_EINVAL, "path is NULL");
+        FILE* f = fopen(path, "r");
+        CHECK(f, DDFAN_EINVAL, "cannot read %s", path);
+        char line[512];
+        int n = 0, lineNo = 0, bad = 0;
+        std::vector<std::pair<TuneKey, TuneVal>> rows;
+        while (fgets(line, sizeof line, f)) {
+            ++lineNo;
+            if (line[0] == '#' || line[0] == '\n') continue;
+            TuneKey k{};
+            TuneVal v{};
+            int tokens = 0;
+            for (const char* c = line; *c;) {
+                while (*c == ' ' || *c == '\t' || *c == '\n' || *c == '\r') ++c;
+                if (!*c) break;
+                ++tokens;
+                while (*c && *c != ' ' && *c != '\t' && *c != '\n' && *c != '\r') ++c;
+            }
+            if (tokens != 15 ||
+                sscanf(line, "%d %d %d %d %d %d %d %d %d %d %d %d %d %lf %lf", &k.mode, &k.XN, &k.YN,
+                       &k.SLN, &k.DNU, &k.ZTmax, &k.views, &v.ZT, &v.V, &v.tile, &v.ZTb, &v.stage,
+                       &v.pstage, &v.projMs, &v.backMs) != 15) {
+                bad = lineNo;
+                break;
+            }
+            const bool pow2 = v.ZT >= 1 && v.ZT <= 16 && (v.ZT & (v.ZT - 1)) == 0 && v.ZTb >= 1 &&
+                              v.ZTb <= 16 && (v.ZTb & (v.ZTb - 1)) == 0;
+            if (!pow2 || v.ZT > k.ZTmax || v.ZTb > k.ZTmax || !(v.V == 1 || v.V == 2 || v.V == 4) ||
+                v.tile < 0 || !stage_fits(v.stage, v.ZTb) || (v.pstage != 0 && v.pstage != 1)) {
+                bad = lineNo;
+                break;
+            }
+            rows.push_back({k, v});
+            ++n;
         }
-        if (tokens != 15 ||
-            sscanf(line, "%d %d %d %d %d %d %d %d %d %d %d %d %d %lf %lf", &k.mode, &k.XN, &k.YN,
-                   &k.SLN, &k.DNU, &k.ZTmax, &k.views, &v.ZT, &v.V, &v.tile, &v.ZTb, &v.stage,
-                   &v.pstage, &v.projMs, &v.backMs) != 15) {
-            bad = lineNo;
-            break;
-        }
-        const bool pow2 = v.ZT >= 1 && v.ZT <= 16 && (v.ZT & (v.ZT - 1)) == 0 && v.ZTb >= 1 &&
-                          v.ZTb <= 16 && (v.ZTb & (v.ZTb - 1)) == 0;
-        if (!pow2 || v.ZT > k.ZTmax || v.ZTb > k.ZTmax || !(v.V == 1 || v.V == 2 || v.V == 4) ||
-            v.tile < 0 || !stage_fits(v.stage, v.ZTb) || (v.pstage != 0 && v.pstage != 1)) {
-            bad = lineNo;
-            break;
-        }
-        rows.push_back({k, v});
-        ++n;
-    }
-    fclose(f);
-    CHECK(!bad, DDFAN_EINVAL, "%s:%d: malformed launch-shape entry", path, bad);
-    std::lock_guard<std::mutex> lk(g_tuneMu);
-    for (const auto& r : rows) g_tune[r.first] = r.second;
-    if (entries) *entries = n;
-    return DDFAN_OK;
+        fclose(f);
+        CHECK(!bad, DDFAN_EINVAL, "%s:%d: malformed launch-shape entry", path, bad);
+        std::lock_guard<std::mutex> lk(g_tuneMu);
+        for (const auto& r : rows) g_tune[r.first] = r.second;
+        if (entries) *entries = n;
+        return DDFAN_OK;
+    });
 }
 
 int ddfan_tune_clear(void) {
-    std::lock_guard<std::mutex> lk(g_tuneMu);
-    g_tune.clear();
-    return DDFAN_OK;
+    return ddf::guarded([&]() -> int {
+        std::lock_guard<std::mutex> lk(g_tuneMu);
+        g_tune.clear();
+        return DDFAN_OK;
+    });
 }
 
 }  // extern "C"
