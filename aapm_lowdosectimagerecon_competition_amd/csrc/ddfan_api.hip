@@ -538,9 +538,11 @@ int ddfan_proj(float x0, float y0, int DNU, const float* xds, const float* yds, 
                const int* startIdx, int nStart) {
     return ddf::guarded([&]() -> int {
         CHECK(hvol && hprj && hangs, DDFAN_EINVAL, "NULL buffer");
+        ddfan_geom G{x0, y0, DNU, xds, yds, 0.0f, imgXCenter, imgYCenter, XN, YN, SLN, dx};
+        TRY(validate_geom(&G));
+        CHECK(PN >= 0, DDFAN_EINVAL, "PN must be >= 0 (got %d)", PN);
         TRY(check_start_idx(startIdx, nStart, SLN));
         if (PN == 0) return DDFAN_OK;
-        ddfan_geom G{x0, y0, DNU, xds, yds, 0.0f, imgXCenter, imgYCenter, XN, YN, SLN, dx};
         ddfan_plan* P = nullptr;
         TRY(ddfan_plan_create(&G, hangs, PN, mask, g_default_sat.load(), 0, &P));
         const size_t nv = (size_t)XN * YN * SLN, np = (size_t)PN * DNU * SLN;
@@ -565,13 +567,15 @@ int ddfan_back(float x0, float y0, int DNU, const float* xds, const float* yds, 
                const int* startIdx, int nStart) {
     return ddf::guarded([&]() -> int {
         CHECK(hvol && hprj && hangs, DDFAN_EINVAL, "NULL buffer");
+        ddfan_geom G{x0, y0, DNU, xds, yds, detCntIdx, imgXCenter, imgYCenter, XN, YN, SLN, dx};
+        TRY(validate_geom(&G));
+        CHECK(PN >= 0, DDFAN_EINVAL, "PN must be >= 0 (got %d)", PN);
         TRY(check_start_idx(startIdx, nStart, SLN));
         const size_t nv = (size_t)XN * YN * SLN, np = (size_t)PN * DNU * SLN;
         if (PN == 0) {
             memset(hvol, 0, sizeof(float) * nv);
             return DDFAN_OK;
         }
-        ddfan_geom G{x0, y0, DNU, xds, yds, detCntIdx, imgXCenter, imgYCenter, XN, YN, SLN, dx};
         ddfan_plan* P = nullptr;
         TRY(ddfan_plan_create(&G, hangs, PN, mask, g_default_sat.load(), 0, &P));
         int rc = [&]() -> int {

@@ -91,6 +91,13 @@ def test_bad_arguments_return_status_not_exit():
                  ctypes.byref(hp)) == N.DDFAN_EINVAL
     assert _call("ddfan_plan_create", ctypes.byref(gs), p(h), 4, None, 0, 3,
                  ctypes.byref(hp)) == N.DDFAN_EINVAL
+    # the drop-ins validate the geometry before anything else, also without views
+    # (a zero-view backprojection zero-fills the volume: never with bad sizes)
+    for XN, SLN, PN in ((-4, 2, 0), (4, -2, 0), (4, 2, -1)):
+        assert _call("ddfan_back", 0.0, 595.0, 8, p(xds), p(xds), 4.5, 0.0, 0.0, p(h), PN, XN, 4,
+                     SLN, p(vol), p(out), 1.0, None, None, 0) == N.DDFAN_EINVAL
+        assert _call("ddfan_proj", 0.0, 595.0, 8, p(xds), p(xds), 0.0, 0.0, p(h), PN, XN, 4, SLN,
+                     p(vol), p(out), 1.0, None, None, 0) == N.DDFAN_EINVAL
     # subset queries
     cnt = ctypes.c_int()
     assert _call("ddfan_subset_views", 10, 0, 0, None, ctypes.byref(cnt)) == N.DDFAN_EINVAL
