@@ -1158,10 +1158,13 @@ __global__ __launch_bounds__(256) void k_rebin_to_sino(const float* __restrict__
         default: return hipErrorInvalidValue;                             \
     }
 
-// Slices per SAT block: the widest G whose LDS tiles fit in 64 KiB.
+// Slices per SAT block: the widest G <= 4 whose LDS tiles fit in 64 KiB.  (Eight
+// slices per block give 512-entry two-float rows 51 KiB of LDS, three blocks per
+// CU: the two volume tables of 512x512x64 took 0.272 ms, against 0.245 ms at
+// four slices per block.)
 static int sat_group(int mode, int K) {
     const size_t lsz = mode == SAT_F32 ? 4 : 8;
-    for (int G : {16, 8, 4, 2, 1})
+    for (int G : {4, 2, 1})
         if ((size_t)G * K * (lsz + 4) + (size_t)256 * lsz <= 65536) return G;
     return 0;
 }
@@ -1192,8 +1195,6 @@ static hipError_t launch_sat_group(int mode, int G, SatDev* t, int n, hipStream_
 #define DDF_SATV(GG) \
     DDF_MODE_SWITCH(mode, k_sat_rows<MC, GG><<<grid, 256, lds, st>>>(L))
     switch (G) {
-        case 16: DDF_SATV(16); break;
-        case 8: DDF_SATV(8); break;
         case 4: DDF_SATV(4); break;
         case 2: DDF_SATV(2); break;
         default: DDF_SATV(1); break;
@@ -1207,7 +1208,7 @@ static hipError_t launch_sat_group(int mode, int G, SatDev* t, int n, hipStream_
 // own row length, never from the other tables of the launch, so a table's
 // entries are the same whatever it is batched with.
 static hipError_t launch_sat_tables(int mode, SatDev* t, int n, hipStream_t st) {
-    for (int G : {16, 8, 4, 2, 1}) {
+    for (int G : {4, 2, 1}) {
         SatDev grp[2 * kBatch];
         int m = 0;
         for (int i = 0; i < n; ++i)
