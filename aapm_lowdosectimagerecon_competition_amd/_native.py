@@ -46,6 +46,7 @@ PARAM_PROJ_STAGE = 6
 TUNE_USE_CACHE = 1
 BATCH_PROJ_FILL = 1
 BATCH_BACK_FILL = 2
+BATCH_SOLO_BYTES = 3
 
 
 class DDFanError(RuntimeError):

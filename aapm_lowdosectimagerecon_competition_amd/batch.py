@@ -28,9 +28,13 @@ class Batch:
         self._keep.append((plan, d_meas, d_x))
         return idx.value
 
-    def set_fill(self, proj_waves=None, back_blocks=None):
-        """Launch-group limits: reconstructions share a launch while it holds at
-        most proj_waves projector waves / back_blocks backprojector blocks."""
+    def set_fill(self, proj_waves=None, back_blocks=None, solo_bytes=None):
+        """Launch grouping: a reconstruction whose volume tables take at least
+        solo_bytes runs its kernels back to back; the others share launches
+        while one holds at most proj_waves projector waves / back_blocks
+        backprojector blocks."""
+        if solo_bytes is not None:
+            N.call("ddfan_batch_set_param", self.handle, N.BATCH_SOLO_BYTES, int(solo_bytes))
         if proj_waves is not None:
             N.call("ddfan_batch_set_param", self.handle, N.BATCH_PROJ_FILL, int(proj_waves))
         if back_blocks is not None:

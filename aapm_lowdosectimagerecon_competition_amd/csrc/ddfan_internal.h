@@ -94,14 +94,13 @@ struct BackJob {
 
 hipError_t launch_sat_vol_batch(int mode, const SatVolJob* jobs, int n, hipStream_t st);
 hipError_t launch_sat_sino_batch(int mode, const SatSinoJob* jobs, int n, hipStream_t st);
-// Default launch-group limits of the batched launches (see launch_proj_batch),
-// measured on MI355X over a 16-reconstruction mix of 256^2-768^2 images.
-constexpr long long kProjFillWaves = 2048;
-constexpr long long kBackFillBlocks = 512;
+// Launch-group limit of the batched launches (see launch_proj_batch): no limit
+// beyond kBatch jobs per launch unless a caller passes one.
+constexpr long long kNoFillLimit = 1LL << 40;
 hipError_t launch_proj_batch(int mode, const ProjJob* jobs, int n, hipStream_t st,
-                             long long fillWaves = kProjFillWaves);
+                             long long fillWaves = kNoFillLimit);
 hipError_t launch_back_batch(int mode, const BackJob* jobs, int n, hipStream_t st,
-                             long long fillBlocks = kBackFillBlocks);
+                             long long fillBlocks = kNoFillLimit);
 hipError_t launch_counter_inc_batch(int* const* counters, int n, hipStream_t st);
 
 // Single-job forms.

@@ -97,6 +97,8 @@ hipError_t enqueue_batch_iteration(OsJob* const* jobs, int n, int mode, const Ba
     std::vector<ProjJob> pj;
     std::vector<SatSinoJob> ss;
     std::vector<BackJob> bj;
+    std::vector<size_t> tb;  // volume-table bytes of each job of the round
+    tb.reserve(n);
     sv.reserve(n);
     pj.reserve(n);
     ss.reserve(n);
@@ -107,6 +109,7 @@ hipError_t enqueue_batch_iteration(OsJob* const* jobs, int n, int mode, const Ba
         pj.clear();
         ss.clear();
         bj.clear();
+        tb.clear();
         for (int i = 0; i < n; ++i) {
             const OsJob& J = *jobs[i];
             const long long nO = J.numOS;
@@ -119,6 +122,7 @@ hipError_t enqueue_batch_iteration(OsJob* const* jobs, int n, int mode, const Ba
             const Subset sub{(int)k, (int)nO, J.cnt[k]};
             const int epi = (J.fista && k == nO - 1) ? BACK_UPDATE_FISTA : BACK_UPDATE;
             sv.push_back(SatVolJob{g, P->nzc, P->ZT, J.x, P->mask.as<uint8_t>(), P->satX.p, P->satY.p});
+            tb.push_back(P->satX.bytes + P->satY.bytes);
             pj.push_back(ProjJob{g, sub, P->nzc, P->ZT, P->slabTile, PROJ_RATIO, P->viewsPerWave,
                                  P->cs.as<float2>(),
                                  P->bx.as<float>(), P->by.as<float>(), P->xds.as<float>(),
@@ -134,22 +138,18 @@ hipError_t enqueue_batch_iteration(OsJob* const* jobs, int n, int mode, const Ba
         }
         const int m = (int)sv.size();
         if (m == 0) continue;
-        // A job whose projector fills a launch group by itself gets launches of
-        // its own anyway: its four kernels run back to back, so the tables it
-        // has just built are read before other jobs' tables evict them.  The
-        // rest share batched launches per kernel kind.
+        // A job with large tables runs its four kernels back to back, so the
+        // tables it has just built are read before other jobs' tables evict
+        // them from the caches; the rest share batched launches per kernel kind.
         int ms = 0;  // shared jobs compacted to the front
         for (int i = 0; i < m; ++i) {
-            const ProjJob& J = pj[i];
-            const int V = J.V == 2 || J.V == 4 ? J.V : 1;
-            const long long waves =
-                (long long)((J.g.DNU + 62) / 63) * ((J.sub.count + V - 1) / V) * J.nzc;
-            if (waves < shape.projFill) {
+            if ((long long)tb[i] < shape.soloBytes) {
                 if (ms != i) {
                     std::swap(sv[ms], sv[i]);
                     std::swap(pj[ms], pj[i]);
                     std::swap(ss[ms], ss[i]);
                     std::swap(bj[ms], bj[i]);
+                    std::swap(tb[ms], tb[i]);
                 }
                 ++ms;
             }
@@ -333,6 +333,10 @@ int ddfan_batch_set_param(ddfan_batch* B, int param, long long value) {
             case DDFAN_BATCH_BACK_FILL:
                 CHECK(value >= 1, DDFAN_EINVAL, "backprojector launch limit must be >= 1");
                 B->shape.backFill = value;
+                return DDFAN_OK;
+            case DDFAN_BATCH_SOLO_BYTES:
+                CHECK(value >= 0, DDFAN_EINVAL, "table-size threshold must be >= 0");
+                B->shape.soloBytes = value;
                 return DDFAN_OK;
             default:
                 return fail(DDFAN_EINVAL, "unknown batch parameter %d", param);

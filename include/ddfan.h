@@ -225,11 +225,15 @@ int ddfan_batch_create(int sat_mode, ddfan_batch** out);
 int ddfan_batch_add(ddfan_batch* batch, ddfan_plan* plan, const float* d_meas, float* d_x,
                     int numIter, int numOS, int useFISTA, int* index);
 int ddfan_batch_size(const ddfan_batch* batch, int* n);
-/* Launch grouping: reconstructions share a projector (backprojector) launch
- * while it holds at most this many 64-lane waves (256-thread blocks); larger
- * ones get launches of their own.  Grouping never changes results. */
+/* Launch grouping.  A reconstruction whose two volume tables take at least
+ * DDFAN_BATCH_SOLO_BYTES bytes (default 64 MiB) runs its kernels back to back;
+ * the others share launches per kernel kind, up to 8 per launch, while a
+ * projector (backprojector) launch holds at most DDFAN_BATCH_PROJ_FILL 64-lane
+ * waves (DDFAN_BATCH_BACK_FILL 256-thread blocks; default: no limit).  Grouping
+ * never changes results. */
 #define DDFAN_BATCH_PROJ_FILL 1
 #define DDFAN_BATCH_BACK_FILL 2
+#define DDFAN_BATCH_SOLO_BYTES 3
 int ddfan_batch_set_param(ddfan_batch* batch, int param, long long value);
 /* Runs every reconstruction to completion (flags: DDFAN_OSSART_GRAPH). */
 int ddfan_batch_run(ddfan_batch* batch, int flags);
