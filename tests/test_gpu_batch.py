@@ -45,9 +45,10 @@ def alone(dd, J, sat="twofloat"):
         return plan.ossart_host(J["meas"], J["x0"], J["it"], J["nos"], J["fista"], graph=True)
 
 
-def run_batch(dd, jobs, sat="twofloat", graph=True):
+def run_batch(dd, jobs, sat="twofloat", graph=True, solo_bytes=None, proj_waves=None):
     plans, bufs = [], []
     with dd.Batch(sat) as B:
+        B.set_fill(proj_waves=proj_waves, solo_bytes=solo_bytes)
         for J in jobs:
             p = dd.Plan(J["g"], J["hangs"], J["mask"], sat_mode=sat)
             db = dd.DeviceBuffer.from_array(J["meas"])
@@ -64,6 +65,22 @@ def run_batch(dd, jobs, sat="twofloat", graph=True):
         db.free()
         dx.free()
     return outs
+
+
+@pytest.mark.parametrize("solo_bytes,proj_waves", [(0, None), (1 << 20, None), (None, 512)],
+                         ids=["all-back-to-back", "large-back-to-back", "wave-limit"])
+def test_batch_grouping_never_changes_results(solo_bytes, proj_waves):
+    """Every job back to back, only the one with large tables (the 512^2 job,
+    ~8 MB) back to back with the rest sharing launches, and shared launches
+    under a wave limit: each result equals the job run alone."""
+    import aapm_lowdosectimagerecon_competition_amd as dd
+    jobs = [make_job(s, 40 + i) for i, s in enumerate(JOBS)]
+    ref = [alone(dd, J) for J in jobs]
+    got = run_batch(dd, jobs, graph=True, solo_bytes=solo_bytes, proj_waves=proj_waves)
+    nd = [int(np.count_nonzero(a != b)) for a, b in zip(got, ref)]
+    print(f"\n[batch-group] solo_bytes {solo_bytes} proj_waves {proj_waves}: voxels differing "
+          f"per job {nd}", flush=True)
+    assert not any(nd)
 
 
 def test_batch_results_equal_each_alone(oracle):
