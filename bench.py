@@ -382,6 +382,12 @@ def main():
                     help="one projector + backprojector call, then one 256 MiB device-to-device "
                          "copy whose counter readings calibrate FETCH_SIZE / WRITE_SIZE; "
                          "prints no JSON (for rocprofv3 runs)")
+    ap.add_argument("--ossart-pass", action="store_true",
+                    help="only the device-resident OS-SART (--ossart-iters iterations x 24 "
+                         "subsets, hipGraph per iteration, after one warm-up iteration) on the "
+                         "projection of the phantom; prints no JSON (for rocprofv3 runs)")
+    ap.add_argument("--ossart-eager", action="store_true",
+                    help="with --ossart-pass: launch the iterations eagerly, without hipGraphs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the path shards by slice with no exchange); "
@@ -435,6 +441,19 @@ def main():
         with dd.DeviceBuffer(COPY_BYTES) as a, dd.DeviceBuffer(COPY_BYTES) as b:
             b.copy_from(a)  # hipMemcpy device-to-device: returns when the copy is done
         log("[bench] profile pass done")
+        return
+    if args.ossart_pass:
+        nos, it = 24, args.ossart_iters
+        plan.proj(d_vol, d_prj)
+        shape_for(PN // nos)
+        with dd.DeviceBuffer(4 * XN * YN * SLN) as d_x:
+            graph = not args.ossart_eager
+            plan.ossart(d_prj, d_x, 1, nos, 1, graph=graph)  # warm-up iteration
+            d_x.upload(np.zeros((YN, XN, SLN), np.float32))
+            ta = time.perf_counter()
+            plan.ossart(d_prj, d_x, it, nos, 1, graph=graph)
+            tb = time.perf_counter()
+        log(f"[bench] ossart pass done: {it} it x {nos} subsets {tb - ta:.4f} s")
         return
     log(f"[bench] warmup {args.warmup}")
     for _ in range(args.warmup):
