@@ -87,6 +87,10 @@ struct HostFloats {
         parallel_copy(p.get(), src, count * sizeof(float));
         return true;
     }
+    void release() {
+        p.reset();
+        n = 0;
+    }
 };
 
 struct Request {
@@ -103,16 +107,22 @@ struct Request {
     DBuf d_meas, d_x;
     OsJob job;
     int left = 0;                    // iterations not yet enqueued
+    hipEvent_t uploaded = nullptr;   // recorded on the copy stream after each input upload
     hipEvent_t lastIter = nullptr;   // recorded after its last iteration
     // outcome
     int status = DDFAN_OK;
     std::string err;
     bool done = false;
+    bool claimed = false;        // a wait has taken the request (it is collected once)
+    float* dst = nullptr;        // the waiting caller's output, registered before completion
+    bool inDst = false;          // the result was copied straight into dst
     HostFloats result;
     Clock::time_point tSubmit, tAdmit, tDone;
 
     ~Request() { release_device(); }
     void release_device() {
+        if (uploaded) (void)hipEventDestroy(uploaded);
+        uploaded = nullptr;
         if (lastIter) (void)hipEventDestroy(lastIter);
         lastIter = nullptr;
         d_meas.release();
@@ -156,9 +166,11 @@ int request_bytes(const ddfan_geom* G, int PN, int mode, int numOS, size_t* byte
     return DDFAN_OK;
 }
 
-// Creates the plan, uploads the sinogram and initial iterate on `st` and
-// enqueues the OS-SART set-up.  Host inputs stay alive in the request.
-int admit_request(Request& R, int mode, hipStream_t st) {
+// Creates the plan, uploads the initial iterate on `cst`, enqueues the OS-SART
+// set-up on `st` after it, then uploads the sinogram on `cst` while the set-up
+// runs; `st` waits for that upload before anything enqueued after this call.
+// The host inputs stay alive until `uploaded` has completed.
+int admit_request(Request& R, int mode, hipStream_t st, hipStream_t cst) {
     ddfan_geom G = R.G;
     G.xds = R.xds.data();
     G.yds = R.yds.data();
@@ -169,19 +181,25 @@ int admit_request(Request& R, int mode, hipStream_t st) {
     const size_t nv = (size_t)G.XN * G.YN * G.SLN;
     HIPCALL(R.d_meas.reserve(R.meas.size() * sizeof(float)));
     HIPCALL(R.d_x.reserve(nv * sizeof(float)));
-    HIPCALL(hipMemcpyAsync(R.d_meas.p, R.meas.data(), R.meas.size() * sizeof(float),
-                           hipMemcpyHostToDevice, st));
-    if (R.x0.empty())
+    HIPCALL(hipEventCreateWithFlags(&R.uploaded, hipEventDisableTiming));
+    if (R.x0.empty()) {
         HIPCALL(launch_fill(R.d_x.as<float>(), 0.0f, nv, st));
-    else
-        HIPCALL(hipMemcpyAsync(R.d_x.p, R.x0.data(), nv * sizeof(float), hipMemcpyHostToDevice, st));
+    } else {
+        HIPCALL(hipMemcpyAsync(R.d_x.p, R.x0.data(), nv * sizeof(float), hipMemcpyHostToDevice, cst));
+        HIPCALL(hipEventRecord(R.uploaded, cst));
+        HIPCALL(hipStreamWaitEvent(st, R.uploaded, 0));  // the FISTA copy reads it
+    }
     R.job.P = R.plan;
     R.job.meas = R.d_meas.as<float>();
     R.job.x = R.d_x.as<float>();
     R.job.numIter = R.numIter;
     R.job.numOS = R.numOS;
     R.job.fista = R.fista;
-    TRY(ossart_prepare(R.job, st));
+    TRY(ossart_prepare(R.job, st));  // weights only: the sinogram is not read yet
+    HIPCALL(hipMemcpyAsync(R.d_meas.p, R.meas.data(), R.meas.size() * sizeof(float),
+                           hipMemcpyHostToDevice, cst));
+    HIPCALL(hipEventRecord(R.uploaded, cst));
+    HIPCALL(hipStreamWaitEvent(st, R.uploaded, 0));
     HIPCALL(hipEventCreateWithFlags(&R.lastIter, hipEventDisableTiming));
     R.left = R.numIter;
     return DDFAN_OK;
@@ -223,7 +241,9 @@ void ddfan_server::loop() {
         running.clear();
         finishing.clear();
     };
-    // Copies back and completes the requests whose last iteration has run.
+    // Copies back and completes the requests whose last iteration has run:
+    // into the output of a caller already waiting for it, else into a host
+    // buffer of the request.
     auto collect = [&]() {
         for (size_t i = 0; i < finishing.size();) {
             Request* R = finishing[i];
@@ -234,7 +254,21 @@ void ddfan_server::loop() {
             const size_t nv = (size_t)R->G.XN * R->G.YN * R->G.SLN;
             int st = DDFAN_OK;
             std::string err;
-            if (!R->result.alloc(nv)) {
+            float* dst = nullptr;
+            {
+                // A registered output stays valid: its caller blocks until done.
+                std::lock_guard<std::mutex> lk(mu);
+                dst = R->dst;
+                R->inDst = dst != nullptr;
+            }
+            if (dst) {
+                if (hipMemcpyAsync(dst, R->d_x.p, nv * sizeof(float), hipMemcpyDeviceToHost,
+                                   copyStream) != hipSuccess ||
+                    hipStreamSynchronize(copyStream) != hipSuccess) {
+                    st = DDFAN_EDEVICE;
+                    err = "result copy failed";
+                }
+            } else if (!R->result.alloc(nv)) {
                 st = DDFAN_ENOMEM;
                 err = "host allocation of the result failed";
             } else if (hipMemcpyAsync(R->result.data(), R->d_x.p, nv * sizeof(float),
@@ -270,10 +304,11 @@ void ddfan_server::loop() {
         }
         bool changed = false;
         for (Request* R : admitted) {
-            const int rc = admit_request(*R, mode, stream);
+            const int rc = admit_request(*R, mode, stream, copyStream);
             if (rc != DDFAN_OK) {
-                // the request's set-up may still be in flight on the stream
+                // the request's set-up and upload may still be in flight
                 (void)hipStreamSynchronize(stream);
+                (void)hipStreamSynchronize(copyStream);
                 finish(R, rc, ddfan_last_error());
                 continue;
             }
@@ -339,6 +374,13 @@ void ddfan_server::loop() {
                 ++i;
             }
         }
+        // host copies of inputs already on the device are no longer needed
+        for (auto* list : {&running, &finishing})
+            for (Request* R : *list)
+                if (!R->meas.empty() && hipEventQuery(R->uploaded) == hipSuccess) {
+                    R->meas.release();
+                    R->x0.release();
+                }
         // wait for the previous iteration, then complete finished requests
         const int prev = slot ^ 1;
         if (inFlight[prev]) {
@@ -457,6 +499,9 @@ int ddfan_server_wait(ddfan_server* S, long long id, float* x, double* queue_ms,
             auto it = S->reqs.find(id);
             CHECK(it != S->reqs.end(), DDFAN_EINVAL, "unknown request %lld", id);
             Request* r = it->second.get();
+            CHECK(!r->claimed, DDFAN_EINVAL, "request %lld is already being waited for", id);
+            r->claimed = true;
+            if (!r->done) r->dst = x;  // the loop copies the result straight into x
             S->cvDone.wait(lk, [&] { return r->done; });
             R = std::move(it->second);
             S->reqs.erase(it);
@@ -465,7 +510,7 @@ int ddfan_server_wait(ddfan_server* S, long long id, float* x, double* queue_ms,
         if (latency_ms) *latency_ms = ms_between(R->tSubmit, R->tDone);
         if (R->status != DDFAN_OK)
             return fail(R->status, "request %lld failed: %s", id, R->err.c_str());
-        if (x) parallel_copy(x, R->result.data(), R->result.size() * sizeof(float));
+        if (x && !R->inDst) parallel_copy(x, R->result.data(), R->result.size() * sizeof(float));
         return DDFAN_OK;
     });
 }

@@ -259,7 +259,10 @@ int ddfan_server_submit(ddfan_server* server, const ddfan_geom* geom, const floa
 int ddfan_server_poll(ddfan_server* server, long long id, int* done);
 /* Blocks until the request is done, copies its reconstruction to x (may be
  * NULL) and returns the milliseconds from submission to admission and to
- * completion.  A request is collected once; a failed one returns its status. */
+ * completion.  Called before the request completes, the loop copies the result
+ * from the device straight into x.  A request is collected once: a second wait
+ * on it, concurrent or later, returns DDFAN_EINVAL; a failed request returns
+ * its status. */
 int ddfan_server_wait(ddfan_server* server, long long id, float* x, double* queue_ms,
                       double* latency_ms);
 /* Counts so far; voxel_views sums 2*XN*YN*SLN*PN over every request iteration run. */
