@@ -39,6 +39,17 @@ public:
         return out;
     }
 
+    // Admits a new request at once, without queueing it, when no request is
+    // waiting (so nothing is overtaken), a slot is free and it fits; false
+    // leaves the state unchanged.
+    bool admit_now(size_t bytes) {
+        if (!queue_.empty() || active_ >= slots_ || bytes > budget_ || used_ + bytes > budget_)
+            return false;
+        used_ += bytes;
+        ++active_;
+        return true;
+    }
+
     // An admitted request finished (or failed) and gave back its bytes.
     void release(size_t bytes) {
         used_ = bytes > used_ ? 0 : used_ - bytes;

@@ -1,9 +1,12 @@
 // ddfan_serve.hip -- reconstruction request loop with continuous batching.
 //
-// Submissions are copied to host memory and queued.  A worker thread owns the
-// device: it admits waiting requests in arrival order while a slot is free and
-// their device footprint fits in the HBM budget (ddf::Admission), creates
-// their plans and uploads their sinograms, and runs every admitted
+// A submission that finds no request waiting, a free slot and room in the HBM
+// budget is admitted at once and its inputs are copied by the submitting
+// thread straight to the device; any other is copied to host memory and
+// queued.  A worker thread owns the device: it admits waiting requests in
+// arrival order while a slot is free and their device footprint fits in the
+// HBM budget (ddf::Admission), creates their plans and uploads the queued
+// requests' inputs, and runs every admitted
 // reconstruction together, one batched OS-SART iteration (a captured hipGraph)
 // at a time.  Requests join and leave at iteration boundaries, so a new one
 // starts while others are mid-run, and each request's arithmetic is the same
@@ -100,6 +103,8 @@ struct Request {
     std::vector<float> xds, yds, hangs;
     HostFloats meas, x0;
     std::vector<uint8_t> mask;
+    bool preloaded = false;  // meas (and x0, if given) already copied to d_meas / d_x
+    bool x0Given = false;
     int PN = 0, numIter = 0, numOS = 1, fista = 0;
     size_t bytes = 0;
     // device state while admitted
@@ -138,10 +143,12 @@ struct ddfan_server {
     int device = 0, mode = 0;
     BatchShape shape;
     hipStream_t stream = nullptr, copyStream = nullptr;
+    hipStream_t upStream = nullptr;  // inputs of requests admitted at submission
     std::mutex mu;
     std::condition_variable cvWork, cvDone;
     ddf::Admission adm;
     std::map<long long, std::unique_ptr<Request>> reqs;  // every request not yet collected
+    std::vector<Request*> ready;  // admitted at submission, inputs on the device
     long long nextId = 1;
     bool stop = false;
     std::thread worker;
@@ -179,12 +186,14 @@ int admit_request(Request& R, int mode, hipStream_t st, hipStream_t cst) {
     int tuned = 0;  // recorded launch shapes for this size of subset, if any
     TRY(ddfan_plan_apply_tuned(R.plan, subset_views(R.PN, R.numOS, 0, nullptr), &tuned));
     const size_t nv = (size_t)G.XN * G.YN * G.SLN;
-    HIPCALL(R.d_meas.reserve(R.meas.size() * sizeof(float)));
-    HIPCALL(R.d_x.reserve(nv * sizeof(float)));
+    if (!R.preloaded) {
+        HIPCALL(R.d_meas.reserve(R.meas.size() * sizeof(float)));
+        HIPCALL(R.d_x.reserve(nv * sizeof(float)));
+    }
     HIPCALL(hipEventCreateWithFlags(&R.uploaded, hipEventDisableTiming));
-    if (R.x0.empty()) {
+    if (!R.x0Given) {
         HIPCALL(launch_fill(R.d_x.as<float>(), 0.0f, nv, st));
-    } else {
+    } else if (!R.preloaded) {
         HIPCALL(hipMemcpyAsync(R.d_x.p, R.x0.data(), nv * sizeof(float), hipMemcpyHostToDevice, cst));
         HIPCALL(hipEventRecord(R.uploaded, cst));
         HIPCALL(hipStreamWaitEvent(st, R.uploaded, 0));  // the FISTA copy reads it
@@ -196,12 +205,35 @@ int admit_request(Request& R, int mode, hipStream_t st, hipStream_t cst) {
     R.job.numOS = R.numOS;
     R.job.fista = R.fista;
     TRY(ossart_prepare(R.job, st));  // weights only: the sinogram is not read yet
-    HIPCALL(hipMemcpyAsync(R.d_meas.p, R.meas.data(), R.meas.size() * sizeof(float),
-                           hipMemcpyHostToDevice, cst));
-    HIPCALL(hipEventRecord(R.uploaded, cst));
-    HIPCALL(hipStreamWaitEvent(st, R.uploaded, 0));
+    if (!R.preloaded) {
+        HIPCALL(hipMemcpyAsync(R.d_meas.p, R.meas.data(), R.meas.size() * sizeof(float),
+                               hipMemcpyHostToDevice, cst));
+        HIPCALL(hipEventRecord(R.uploaded, cst));
+        HIPCALL(hipStreamWaitEvent(st, R.uploaded, 0));
+    }
     HIPCALL(hipEventCreateWithFlags(&R.lastIter, hipEventDisableTiming));
     R.left = R.numIter;
+    return DDFAN_OK;
+}
+
+// Copies a request admitted at submission from the caller's buffers to its
+// device buffers on `up` (the submitting thread, with the server's device
+// current) and waits for the copies.
+int upload_inputs(Request& R, const float* meas, size_t nm, const float* x0, size_t nv,
+                  int device, hipStream_t up) {
+    int prev = -1;
+    HIPCALL(hipGetDevice(&prev));
+    struct Restore {
+        int d;
+        ~Restore() { (void)hipSetDevice(d); }
+    } restore{prev};
+    HIPCALL(hipSetDevice(device));
+    HIPCALL(R.d_meas.reserve(nm * sizeof(float)));
+    HIPCALL(R.d_x.reserve(nv * sizeof(float)));
+    HIPCALL(hipMemcpyAsync(R.d_meas.p, meas, nm * sizeof(float), hipMemcpyHostToDevice, up));
+    if (x0) HIPCALL(hipMemcpyAsync(R.d_x.p, x0, nv * sizeof(float), hipMemcpyHostToDevice, up));
+    HIPCALL(hipStreamSynchronize(up));
+    R.preloaded = true;
     return DDFAN_OK;
 }
 
@@ -288,6 +320,8 @@ void ddfan_server::loop() {
         {
             std::unique_lock<std::mutex> lk(mu);
             for (;;) {
+                for (Request* R : ready) admitted.push_back(R);
+                ready.clear();
                 for (long long id : adm.admit()) {
                     Request* R = reqs[id].get();
                     R->tAdmit = Clock::now();
@@ -418,8 +452,10 @@ int ddfan_server_create(size_t hbm_budget, int max_slots, int sat_mode, ddfan_se
         S->mode = sat_mode;
         hipError_t e = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&S->copyStream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&S->upStream, hipStreamNonBlocking);
         if (e != hipSuccess) {
             if (S->stream) (void)hipStreamDestroy(S->stream);
+            if (S->copyStream) (void)hipStreamDestroy(S->copyStream);
             delete S;
             HIPCALL(e);
         }
@@ -428,6 +464,7 @@ int ddfan_server_create(size_t hbm_budget, int max_slots, int sat_mode, ddfan_se
         } catch (...) {
             (void)hipStreamDestroy(S->stream);
             (void)hipStreamDestroy(S->copyStream);
+            (void)hipStreamDestroy(S->upStream);
             delete S;
             return fail(DDFAN_ENOMEM, "cannot start the worker thread");
         }
@@ -440,6 +477,7 @@ int ddfan_server_submit(ddfan_server* S, const ddfan_geom* G, const float* hangs
                         const unsigned char* mask, const float* meas, const float* x0,
                         int numIter, int numOS, int useFISTA, long long* id) {
     return ddf::guarded([&]() -> int {
+        const Clock::time_point arrival = Clock::now();
         CHECK(S && G && hangs && meas && id, DDFAN_EINVAL, "NULL argument");
         CHECK(G->XN >= 1 && G->YN >= 1 && G->SLN >= 1 && G->DNU >= 2 && G->xds && G->yds,
               DDFAN_EINVAL, "bad geometry");
@@ -452,27 +490,51 @@ int ddfan_server_submit(ddfan_server* S, const ddfan_geom* G, const float* hangs
         R->xds.assign(G->xds, G->xds + G->DNU);
         R->yds.assign(G->yds, G->yds + G->DNU);
         R->hangs.assign(hangs, hangs + (size_t)PN * G->SLN);
-        const size_t nv = (size_t)G->XN * G->YN * G->SLN;
-        CHECK(R->meas.assign_copy(meas, (size_t)PN * G->DNU * G->SLN) &&
-                  (!x0 || R->x0.assign_copy(x0, nv)),
-              DDFAN_ENOMEM, "host copy of the request's inputs failed");
         if (mask) R->mask.assign(mask, mask + (size_t)G->XN * G->YN);
         R->PN = PN;
         R->numIter = numIter;
         R->numOS = numOS;
         R->fista = useFISTA ? 1 : 0;
+        R->x0Given = x0 != nullptr;
         R->bytes = bytes;
-        R->tSubmit = Clock::now();
+        R->tSubmit = arrival;
+        const size_t nm = (size_t)PN * G->DNU * G->SLN, nv = (size_t)G->XN * G->YN * G->SLN;
+        bool now = false;
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            CHECK(!S->stop, DDFAN_EINVAL, "server is shutting down");
+            CHECK(bytes <= S->adm.budget(), DDFAN_EBUSY,
+                  "request needs %zu device bytes, more than the whole budget of %zu", bytes,
+                  S->adm.budget());
+            R->id = S->nextId++;
+            now = S->adm.admit_now(bytes);
+        }
+        if (now) {
+            // Admitted: copy the inputs from the caller's buffers to the device.
+            R->tAdmit = Clock::now();
+            const int rc = upload_inputs(*R, meas, nm, x0, nv, S->device, S->upStream);
+            std::lock_guard<std::mutex> lk(S->mu);
+            if (rc != DDFAN_OK) {
+                S->adm.release(bytes);
+                S->cvWork.notify_all();
+                return rc;
+            }
+            const long long rid = R->id;
+            *id = rid;
+            S->ready.push_back(R.get());
+            S->reqs[rid] = std::move(R);
+            S->cvWork.notify_all();
+            return DDFAN_OK;
+        }
+        CHECK(R->meas.assign_copy(meas, nm) && (!x0 || R->x0.assign_copy(x0, nv)), DDFAN_ENOMEM,
+              "host copy of the request's inputs failed");
         std::lock_guard<std::mutex> lk(S->mu);
         CHECK(!S->stop, DDFAN_EINVAL, "server is shutting down");
-        const long long rid = S->nextId;
-        CHECK(S->adm.submit(rid, bytes), DDFAN_EBUSY,
-              "request needs %zu device bytes, more than the whole budget of %zu", bytes,
-              S->adm.budget());
-        ++S->nextId;
-        R->id = rid;
-        S->reqs[rid] = std::move(R);
+        const bool queued = S->adm.submit(R->id, bytes);  // bytes <= budget: always true
+        CHECK(queued, DDFAN_EBUSY, "request rejected by the HBM budget");
+        const long long rid = R->id;
         *id = rid;
+        S->reqs[rid] = std::move(R);
         S->cvWork.notify_all();
         return DDFAN_OK;
     });
@@ -543,6 +605,7 @@ int ddfan_server_destroy(ddfan_server* S) {
         S->reqs.clear();
         (void)hipStreamDestroy(S->stream);
         (void)hipStreamDestroy(S->copyStream);
+        (void)hipStreamDestroy(S->upStream);
         delete S;
         return DDFAN_OK;
     });

@@ -35,7 +35,8 @@ class Server:
 
     def submit(self, geom: DetectorGeometry, hangs, meas, num_iter, num_os, use_fista=1,
                mask=None, x0=None) -> int:
-        """Queues one reconstruction; the inputs are copied before return.
+        """Queues one reconstruction; the inputs are copied (to the device when
+        it is admitted at once, else to host memory) before return.
         hangs (PN, SLN), meas (PN, DNU, SLN), x0 (YN, XN, SLN) or None."""
         h = np.ascontiguousarray(hangs, np.float32)
         PN = h.shape[0]

@@ -252,7 +252,10 @@ int ddfan_batch_destroy(ddfan_batch* batch);
 typedef struct ddfan_server ddfan_server;
 int ddfan_server_create(size_t hbm_budget, int max_slots, int sat_mode, ddfan_server** out);
 /* Copies the request's host inputs (meas [PN][DNU][SLN], x0 [YN][XN][SLN] or
- * NULL for zeros, mask or NULL) and queues it; *id names it. */
+ * NULL for zeros, mask or NULL) and queues it; *id names it.  With no request
+ * waiting, a free slot and room in the budget it is admitted at once and the
+ * inputs go straight to the device; else they are copied to host memory.
+ * Either way the caller's buffers are not read after the call returns. */
 int ddfan_server_submit(ddfan_server* server, const ddfan_geom* geom, const float* hangs, int PN,
                         const unsigned char* mask, const float* meas, const float* x0,
                         int numIter, int numOS, int useFISTA, long long* id);

@@ -32,6 +32,20 @@ int main() {
         v = a.admit();
         assert(v.size() == 1 && v[0] == 2);
     }
+    {
+        Admission a(100, 2);  // immediate admission only with nobody waiting
+        assert(!a.admit_now(101));
+        assert(a.admit_now(60) && a.used() == 60 && a.active() == 1);
+        assert(!a.admit_now(50) && a.used() == 60);  // does not fit
+        assert(a.submit(7, 50));                      // queued instead
+        assert(!a.admit_now(10));                     // 7 is waiting: no overtaking
+        assert(a.admit().empty());
+        a.release(60);
+        auto v = a.admit();
+        assert(v.size() == 1 && v[0] == 7);
+        assert(a.admit_now(40) && a.active() == 2);
+        assert(!a.admit_now(1));                      // both slots taken
+    }
     std::printf("sched ok\n");
     return 0;
 }
