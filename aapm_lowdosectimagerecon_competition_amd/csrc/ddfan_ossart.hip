@@ -60,17 +60,28 @@ int ossart_prepare(OsJob& J, hipStream_t st) {
     HIPCALL(launch_fill(P->onesP.as<float>(), 1.0f, (size_t)maxc * DNU, st));
     HIPCALL(launch_sat_vol(mode, 1, g0, 1, P->onesV.as<float>(), P->mask.as<uint8_t>(), P->satX0.p,
                            P->satY0.p, st));
+    // One table of the ones sinogram serves every subset: a subset of c views
+    // reads its first c view rows, which do not depend on the view count.
+    HIPCALL(launch_sat_sino(mode, 1, DNU, 1, 1, maxc, P->onesP.as<float>(), P->satP0.p, st));
+    // The subsets' weights as batched launches (up to kBatch subsets each):
+    // every subset's arithmetic is that of its own launch.
+    std::vector<ProjJob> pj;
+    std::vector<BackJob> bj;
+    pj.reserve(numOS);
+    bj.reserve(numOS);
     for (int k = 0; k < numOS; ++k) {
         const Subset sub{k, numOS, J.cnt[k]};
-        HIPCALL(launch_proj(mode, 1, PROJ_PLAIN, g0, 1, P->cs0.as<float2>(), P->bx.as<float>(),
-                            P->by.as<float>(), P->xds.as<float>(), P->yds.as<float>(), P->satX0.p,
-                            P->satY0.p, sub, P->rowSum.as<float>() + (size_t)k * maxc * DNU,
-                            nullptr, nullptr, 0, 1, 0, st));
-        HIPCALL(launch_sat_sino(mode, 1, DNU, 1, 1, J.cnt[k], P->onesP.as<float>(), P->satP0.p, st));
-        HIPCALL(launch_back(mode, 1, BACK_PLAIN, g0, 1, P->cs0.as<float2>(), P->mask.as<uint8_t>(),
-                            P->satP0.p, sub, P->colSum.as<float>() + (size_t)k * npix, nullptr,
-                            nullptr, nullptr, nullptr, 0, st));
+        pj.push_back(ProjJob{g0, sub, 1, 1, 0, PROJ_PLAIN, 1, P->cs0.as<float2>(),
+                             P->bx.as<float>(), P->by.as<float>(), P->xds.as<float>(),
+                             P->yds.as<float>(), P->satX0.p, P->satY0.p,
+                             P->rowSum.as<float>() + (size_t)k * maxc * DNU, nullptr, nullptr, 0});
+        bj.push_back(BackJob{g0, sub, 1, 1, BACK_PLAIN, P->cs0.as<float2>(),
+                             P->mask.as<uint8_t>(), P->satP0.p,
+                             P->colSum.as<float>() + (size_t)k * npix, nullptr, nullptr, nullptr,
+                             nullptr, 0});
     }
+    HIPCALL(launch_proj_batch(mode, pj.data(), numOS, st));
+    HIPCALL(launch_back_batch(mode, bj.data(), numOS, st));
     // FISTA momenta (OSSART_AAPM.m:111-115): MATLAB double scalars, applied to
     // single arrays as single((t0-1)/t1).
     J.h_mom.assign(std::max(J.numIter, 1), 0.0f);
