@@ -143,9 +143,10 @@ struct Scan<SAT_F64> {
 // in[(row*rs + k*ks)*SLN + z], optionally times mask[row*rs + k*ks] (its byte
 // value, MultiSlicesProjGPU.cu:202-209).  One block scans one row for G
 // consecutive slices: 256/G threads per slice own contiguous k segments
-// (loads coalesce over the slices, contiguous in memory), segment totals are
-// combined through LDS, and finished rows leave through LDS as contiguous k
-// runs of paired entries.  A launch holds several tables (both axes of a
+// (loads coalesce over the slices, contiguous in memory), the segment totals
+// of a slice are scanned across lanes (lanes G apart) and across the block's
+// four waves, and finished rows leave through LDS as contiguous k runs of
+// paired entries.  A launch holds several tables (both axes of a
 // volume, or the tables of several reconstructions), block ranges one after
 // another.
 struct SatDev {
@@ -158,6 +159,13 @@ struct SatLaunch {
     int njobs, nwork;
     SatDev j[2 * kBatch];
 };
+
+// Value of lane - d of the same wave (callers use it for lane >= d only).
+__device__ __forceinline__ float lane_up(float v, int d) { return __shfl_up(v, d); }
+__device__ __forceinline__ double lane_up(double v, int d) { return __shfl_up(v, d); }
+__device__ __forceinline__ float2 lane_up(float2 v, int d) {
+    return make_float2(__shfl_up(v.x, d), __shfl_up(v.y, d));
+}
 
 // Longest scan segment a thread holds in registers: the rows sat_group admits
 // (G * K * (prefix + sample bytes) + 256 prefixes within 64 KiB) split over
@@ -187,8 +195,8 @@ __global__ __launch_bounds__(256) void k_sat_rows(const SatLaunch L) {
     const int z = zg * G + tz;
     const bool real = z < jb.SLN;
     Lt* pre = reinterpret_cast<Lt*>(smem);  // [G][K] exclusive prefixes
-    Lt* tot = pre + (size_t)G * K;          // [G][NSEG] segment totals
-    float* val = reinterpret_cast<float*>(tot + (size_t)G * NSEG);  // [G][K] samples
+    Lt* wtot = pre + (size_t)G * K;         // [4 waves][G] wave totals (room for 256)
+    float* val = reinterpret_cast<float*>(wtot + (size_t)G * NSEG);  // [G][K] samples
     const int Lseg = (jb.Nk + NSEG - 1) / NSEG;  // <= kSatSeg (launch_sat_group)
     const int k0 = min(seg * Lseg, jb.Nk), k1 = min(k0 + Lseg, jb.Nk);
     // the segment's samples are loaded first, all in flight together, then scanned
@@ -213,23 +221,39 @@ __global__ __launch_bounds__(256) void k_sat_rows(const SatLaunch L) {
             acc.add(sv[q]);
         }
     }
-    tot[tz * NSEG + seg] = acc.get();
     if (seg == NSEG - 1) val[(size_t)tz * K + jb.Nk] = 0.0f;
+    // Exclusive prefix of the segment totals of slice tz: a scan over the
+    // wave's lanes of that slice (lane = G * (segment within the wave) + tz),
+    // then the totals of the waves before this one.  Every lane takes part.
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    Lt incl = acc.get();
+#pragma unroll
+    for (int d = G; d < 64; d *= 2) {
+        const Lt o = lane_up(incl, d);
+        if (lane >= d) incl = S::plus(o, incl);
+    }
+    Lt excl = lane_up(incl, G);
+    if (lane < G) excl = S::zero();
+    if (lane >= 64 - G) wtot[wave * G + tz] = incl;
     __syncthreads();
     Lt off = S::zero();
-    for (int q = 0; q < seg; ++q) off = S::plus(off, tot[tz * NSEG + q]);
+    for (int w = 0; w < wave; ++w) off = S::plus(off, wtot[w * G + tz]);
+    off = S::plus(off, excl);
     for (int k = k0; k < k1; ++k) pre[(size_t)tz * K + k] = S::plus(pre[(size_t)tz * K + k], off);
     if (seg == NSEG - 1) pre[(size_t)tz * K + jb.Nk] = S::plus(off, acc.get());
     __syncthreads();
+    // Rows out, one slice at a time: consecutive threads write consecutive k.
     const int SLNp = jb.nzc * jb.ZT;
     E* base = reinterpret_cast<E*>(jb.out);
-    for (int idx = threadIdx.x; idx < G * K; idx += 256) {
-        const int q = idx / K, k = idx - q * K;
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
         const int zz = zg * G + q;
-        if (zz < SLNp) {
-            const int zc = zz / jb.ZT, zt = zz - zc * jb.ZT;
-            base[(((size_t)zc * jb.nrows + row) * jb.ZT + zt) * K + k] = S::entry(pre[idx], val[idx]);
-        }
+        if (zz >= SLNp) break;
+        const int zc = zz / jb.ZT, zt = zz - zc * jb.ZT;
+        E* dst = base + (((size_t)zc * jb.nrows + row) * jb.ZT + zt) * K;
+        const Lt* pq = pre + (size_t)q * K;
+        const float* vq = val + (size_t)q * K;
+        for (int k = threadIdx.x; k < K; k += 256) dst[k] = S::entry(pq[k], vq[k]);
     }
 }
 
