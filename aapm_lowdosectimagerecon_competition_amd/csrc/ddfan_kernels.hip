@@ -425,8 +425,62 @@ void k_proj(const ProjLaunch L) {
 #pragma unroll
         for (int q = 0; q < ZT; ++q) acc[v][q] = 0.0f;
     int iiEnd = 0;
+    bool sameEnd = true;  // every view of the wave has the same slab count
 #pragma unroll
-    for (int v = 0; v < V; ++v) iiEnd = max(iiEnd, min(J.ii1, ((xs >> v) & 1u) ? g.XN : g.YN));
+    for (int v = 0; v < V; ++v) {
+        const int n = ((xs >> v) & 1u) ? g.XN : g.YN;
+        sameEnd = sameEnd && n == (((xs & 1u) ? g.XN : g.YN));
+        iiEnd = max(iiEnd, min(J.ii1, n));
+    }
+    // Few loads per slab (ZT x V <= 4) leave a wave waiting for each slab's
+    // gathers: there the entries of slab ii + 1 are loaded before the sums of
+    // slab ii (the same arithmetic in the same order, so the same results).
+    if constexpr (ZT * V <= 4) {
+        if (sameEnd && J.ii0 < iiEnd) {
+            E e[V][ZT];
+            float tw[V], il[V];
+            auto fetch = [&](int ii, E (&ee)[V][ZT], float (&t)[V], float (&inv)[V])
+                __attribute__((always_inline)) {
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    const bool xv = (xs >> v) & 1u;
+                    const int Nk = xv ? g.YN : g.XN;
+                    const float ocS = xv ? g.ocxP : g.ocyP, ocK = xv ? g.ocyP : g.ocxP;
+                    const int K = Nk + 1;
+                    const float r = slab_r(slab_obj(ii, ocS, g.dx), pa[v], pf[v], pbb[v]);
+                    const Fetch f = fetch_pos<E>(slab_coord(r, g.invdx, ocK), Nk);
+                    inv[v] = __builtin_amdgcn_rcpf(r - lane_prev(r));
+                    t[v] = f.t;
+                    const E* row = (xv ? baseY : baseX) + (size_t)ii * ZT * K;
+#pragma unroll
+                    for (int q = 0; q < ZT; ++q) ee[v][q] = entry_at(row + (size_t)q * K, f.off);
+                }
+            };
+            fetch(J.ii0, e, tw, il);
+            for (int ii = J.ii0; ii < iiEnd; ++ii) {
+                E en[V][ZT];
+                float tn[V], iln[V];
+                fetch(min(ii + 1, iiEnd - 1), en, tn, iln);  // the last slab loads itself again
+#pragma unroll
+                for (int v = 0; v < V; ++v)
+#pragma unroll
+                    for (int q = 0; q < ZT; ++q) {
+                        const Samp<MODE> T(e[v][q], tw[v]);
+                        acc[v][q] = fmaf(T.minus_prev_lane(), il[v], acc[v][q]);
+                    }
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    tw[v] = tn[v];
+                    il[v] = iln[v];
+#pragma unroll
+                    for (int q = 0; q < ZT; ++q) e[v][q] = en[v][q];
+                }
+            }
+            if (!act) return;
+            proj_store<ZT, V>(J, g, sub, jg, u, z0, acc, cval);
+            return;
+        }
+    }
     for (int ii = J.ii0; ii < iiEnd; ++ii) {
 #pragma unroll
         for (int v = 0; v < V; ++v) {
