@@ -386,6 +386,10 @@ def main():
                     help="only the device-resident OS-SART (--ossart-iters iterations x 24 "
                          "subsets, hipGraph per iteration, after one warm-up iteration) on the "
                          "projection of the phantom; prints no JSON (for rocprofv3 runs)")
+    ap.add_argument("--helical-pass", action="store_true",
+                    help="only the helical workload (device rebin, then --helical-iters OS-SART "
+                         "iterations with the reference's and with the aligned backprojector); "
+                         "prints no JSON (for rocprofv3 runs)")
     ap.add_argument("--ossart-eager", action="store_true",
                     help="with --ossart-pass: launch the iterations eagerly, without hipGraphs")
     args = ap.parse_args()
@@ -404,6 +408,10 @@ def main():
     if args.rebin_pass:
         bench_helical(dd, args.sat, 0)
         log("[bench] rebin pass done")
+        return
+    if args.helical_pass:
+        h = bench_helical(dd, args.sat, args.helical_iters)
+        log(f"[bench] helical pass done: OS-SART {h.get('ossart_s', 0.0):.3f} s")
         return
     conf = dd.Conf.make(XN=XN, YN=YN, SLN=SLN, dx=DX, sid=SID, sdd=SDD, DNU=DNU,
                         detCellWidth=float(np.float32(2 * SDD * np.tan(STEP / 2))),
