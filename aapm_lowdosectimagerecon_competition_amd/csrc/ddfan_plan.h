@@ -49,6 +49,13 @@ struct DBuf {
         p = nullptr;
         bytes = 0;
     }
+    // Gives up ownership: the caller frees the returned pointer.
+    void* detach() {
+        void* q = p;
+        p = nullptr;
+        bytes = 0;
+        return q;
+    }
     hipError_t reserve(size_t b) {
         if (p && b <= bytes) return hipSuccess;
         release();

@@ -11,9 +11,11 @@
 // at a time.  Requests join and leave at iteration boundaries, so a new one
 // starts while others are mid-run, and each request's arithmetic is the same
 // launch sequence it would get alone.  The worker keeps at most two batch
-// iterations in flight, finishes requests whose last iteration completed
-// (result copied back on a second stream while the next iteration runs) and
-// gives their bytes back to the budget.
+// iterations in flight and finishes requests whose last iteration completed
+// (result copied back on a second stream while the next iteration runs).  A
+// third thread frees finished requests' device memory (freeing waits for the
+// GPU work in flight, which the worker must not) and gives their bytes back to
+// the budget.
 #include <hip/hip_runtime.h>
 #include <string.h>
 
@@ -96,6 +98,15 @@ struct HostFloats {
     }
 };
 
+// Device state of a finished request, freed by the server's reaper thread.
+struct Dead {
+    ddfan_plan* plan = nullptr;
+    void* meas = nullptr;
+    void* x = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    size_t bytes = 0;
+};
+
 struct Request {
     long long id = 0;
     // inputs (host copies)
@@ -125,6 +136,19 @@ struct Request {
     Clock::time_point tSubmit, tAdmit, tDone;
 
     ~Request() { release_device(); }
+    // Moves the device state out (the request keeps its host side).
+    Dead take_device() {
+        Dead d;
+        d.plan = plan;
+        plan = nullptr;
+        d.meas = d_meas.detach();
+        d.x = d_x.detach();
+        d.ev[0] = uploaded;
+        d.ev[1] = lastIter;
+        uploaded = lastIter = nullptr;
+        d.bytes = bytes;
+        return d;
+    }
     void release_device() {
         if (uploaded) (void)hipEventDestroy(uploaded);
         uploaded = nullptr;
@@ -149,15 +173,18 @@ struct ddfan_server {
     ddf::Admission adm;
     std::map<long long, std::unique_ptr<Request>> reqs;  // every request not yet collected
     std::vector<Request*> ready;  // admitted at submission, inputs on the device
+    std::deque<Dead> dead;        // finished, device memory not yet freed
+    std::condition_variable cvDead;
     long long nextId = 1;
-    bool stop = false;
-    std::thread worker;
+    bool stop = false, reaperStop = false;
+    std::thread worker, reaper;
     // statistics
     long long completed = 0, failed = 0;
     double busyVoxelViews = 0.0;
 
     ddfan_server(size_t budget, int slots) : adm(budget, slots) {}
     void loop();
+    void reap();
 };
 
 namespace {
@@ -252,20 +279,22 @@ void ddfan_server::loop() {
     long long iterNo = 0;
     bool inFlight[2] = {false, false};
 
+    // Completes a request and hands its device memory to the reaper; its
+    // waiter may destroy the request as soon as it is marked done.
     auto finish = [&](Request* R, int status, const std::string& err) {
-        R->release_device();
+        Dead d = R->take_device();
         std::lock_guard<std::mutex> lk(mu);
         R->status = status;
         R->err = err;
         R->done = true;
         R->tDone = Clock::now();
-        adm.release(R->bytes);
         if (status == DDFAN_OK)
             ++completed;
         else
             ++failed;
+        dead.push_back(d);
         cvDone.notify_all();
-        cvWork.notify_all();
+        cvDead.notify_one();
     };
     auto fail_all_running = [&](int status, const std::string& err) {
         for (Request* R : running) finish(R, status, err);
@@ -431,6 +460,28 @@ void ddfan_server::loop() {
     (void)hipEventDestroy(iterEv[1]);
 }
 
+// Frees finished requests' device memory, then returns their bytes to the
+// budget (the worker admits waiting requests against it).
+void ddfan_server::reap() {
+    (void)hipSetDevice(device);
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+        cvDead.wait(lk, [&] { return !dead.empty() || reaperStop; });
+        if (dead.empty()) break;  // stopping, nothing left
+        Dead d = dead.front();
+        dead.pop_front();
+        lk.unlock();
+        if (d.plan) (void)ddfan_plan_destroy(d.plan);
+        if (d.meas) (void)hipFree(d.meas);
+        if (d.x) (void)hipFree(d.x);
+        for (hipEvent_t e : d.ev)
+            if (e) (void)hipEventDestroy(e);
+        lk.lock();
+        adm.release(d.bytes);
+        cvWork.notify_all();
+    }
+}
+
 extern "C" {
 
 int ddfan_server_create(size_t hbm_budget, int max_slots, int sat_mode, ddfan_server** out) {
@@ -460,13 +511,22 @@ int ddfan_server_create(size_t hbm_budget, int max_slots, int sat_mode, ddfan_se
             HIPCALL(e);
         }
         try {
+            S->reaper = std::thread([S] { S->reap(); });
             S->worker = std::thread([S] { S->loop(); });
         } catch (...) {
+            if (S->reaper.joinable()) {
+                {
+                    std::lock_guard<std::mutex> lk(S->mu);
+                    S->reaperStop = true;
+                    S->cvDead.notify_all();
+                }
+                S->reaper.join();
+            }
             (void)hipStreamDestroy(S->stream);
             (void)hipStreamDestroy(S->copyStream);
             (void)hipStreamDestroy(S->upStream);
             delete S;
-            return fail(DDFAN_ENOMEM, "cannot start the worker thread");
+            return fail(DDFAN_ENOMEM, "cannot start the worker threads");
         }
         *out = S;
         return DDFAN_OK;
@@ -601,6 +661,12 @@ int ddfan_server_destroy(ddfan_server* S) {
             S->cvWork.notify_all();
         }
         if (S->worker.joinable()) S->worker.join();
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->reaperStop = true;
+            S->cvDead.notify_all();
+        }
+        if (S->reaper.joinable()) S->reaper.join();  // frees everything finished
         (void)hipSetDevice(S->device);
         S->reqs.clear();
         (void)hipStreamDestroy(S->stream);

@@ -268,7 +268,9 @@ int ddfan_server_poll(ddfan_server* server, long long id, int* done);
  * its status. */
 int ddfan_server_wait(ddfan_server* server, long long id, float* x, double* queue_ms,
                       double* latency_ms);
-/* Counts so far; voxel_views sums 2*XN*YN*SLN*PN over every request iteration run. */
+/* Counts so far; voxel_views sums 2*XN*YN*SLN*PN over every request iteration run.
+ * used_bytes counts admitted requests until their device memory has been freed,
+ * which happens on a thread of the server shortly after each completes. */
 int ddfan_server_stats(ddfan_server* server, long long* completed, long long* failed,
                        int* running, long long* waiting, size_t* used_bytes,
                        double* voxel_views);

@@ -60,6 +60,10 @@ def test_budget_admission_and_rejection():
         ids = [S.submit(J["g"], J["hangs"], J["meas"], J["it"], J["nos"], J["fista"],
                         mask=J["mask"]) for J in jobs]
         res = [S.wait(i) for i in ids]
+        # device memory goes back to the budget once the reaper has freed it
+        deadline = time.time() + 30
+        while S.stats()["used_bytes"] and time.time() < deadline:
+            time.sleep(0.01)
         assert S.stats()["used_bytes"] == 0
     assert np.array_equal(res[0].x, ref)
     # admitted one after another: each waits for the previous one to finish
