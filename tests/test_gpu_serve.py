@@ -160,3 +160,30 @@ def test_request_is_collected_once():
             N.call("ddfan_server_wait", S.handle, ctypes.c_longlong(rid), None, None, None)
     print(f"[serve] first wait: latency {got['r'].latency_ms:.1f} ms", flush=True)
     assert np.array_equal(got["r"].x, ref)
+
+
+def test_destroy_finishes_queued_requests_and_frees_them():
+    """Closing a server with requests still queued or running finishes them
+    and frees every device byte they held (the worker and the thread that frees
+    finished requests both drain before destroy returns)."""
+    import ctypes
+
+    import aapm_lowdosectimagerecon_competition_amd as dd
+    from aapm_lowdosectimagerecon_competition_amd import _native as N
+
+    def free_bytes():
+        f, t = ctypes.c_size_t(), ctypes.c_size_t()
+        N.call("ddfan_dev_mem_info", ctypes.byref(f), ctypes.byref(t))
+        return f.value
+
+    jobs = [make_job(JOBS[i % 3], 160 + i) for i in range(5)]
+    free0 = free_bytes()
+    print(f"\n[serve] close with 5 requests outstanding, free {free0 / 2**20:.0f} MiB", flush=True)
+    S = dd.Server(max_slots=2)
+    for J in jobs:
+        S.submit(J["g"], J["hangs"], J["meas"], J["it"], J["nos"], J["fista"], mask=J["mask"])
+    t0 = time.time()
+    S.close()
+    free1 = free_bytes()
+    print(f"[serve] closed in {time.time() - t0:.2f} s, free {free1 / 2**20:.0f} MiB", flush=True)
+    assert free1 >= free0 - (64 << 20)  # allocator slack only
