@@ -272,143 +272,195 @@ typedef struct {
     int32_t *trace; /* optional [v][u][z][ii][2] texel indices (f64 run only) */
 } orc_proj_ctx;
 
+/* Work item = one (view, slice) pair: the branch is fixed for the item, and
+ * every slab's table row is read for all DNU detectors before the next slab,
+ * so a row stays in the core's cache.  Each detector still sums its slabs in
+ * ascending order, the same operations in the same order as one ray at a
+ * time, so the results do not depend on this loop order. */
+typedef struct {
+    float dLx, dLy, dRx, dRy, fL, fR;
+    double dLx64, dLy64, dRx64, dRy64, fL64, fR64, adir; /* |normalised ray component| */
+    double summ;
+} orc_ray64;
+
 static void *orc_proj_worker64(void *arg) {
     orc_task *tk = (orc_task *)arg;
     orc_proj_ctx *C = (orc_proj_ctx *)tk->ctx;
     const int SLN = C->SLN, DNU = C->DNU, XN = C->XN, YN = C->YN;
     const float x0 = C->x0, y0 = C->y0, dx = C->dx, ocx = C->ocx, ocy = C->ocy, invdx = C->invdx;
-    for (long vu = tk->tid; vu < (long)C->PN * DNU; vu += tk->nthreads) {
-        {
-            const int v = (int)(vu / DNU), u = (int)(vu % DNU);
-            for (int z = 0; z < SLN; ++z) {
-                const float c = C->cs[2 * ((size_t)v * SLN + z)];
-                const float s = C->cs[2 * ((size_t)v * SLN + z) + 1];
-                /* FP32 index path (MultiSlicesProjGPU.cu:122-137) */
-                const float srcx = x0 * c - y0 * s, srcy = x0 * s + y0 * c;
-                const float dLx = C->bx[u] * c - C->by[u] * s, dLy = C->bx[u] * s + C->by[u] * c;
-                const float dRx = C->bx[u + 1] * c - C->by[u + 1] * s;
-                const float dRy = C->bx[u + 1] * s + C->by[u + 1] * c;
-                /* FP64 continuous path */
-                const double c64 = c, s64 = s;
-                const double srcx64 = (double)x0 * c64 - (double)y0 * s64;
-                const double srcy64 = (double)x0 * s64 + (double)y0 * c64;
-                const double dLx64 = (double)C->bx[u] * c64 - (double)C->by[u] * s64;
-                const double dLy64 = (double)C->bx[u] * s64 + (double)C->by[u] * c64;
-                const double dRx64 = (double)C->bx[u + 1] * c64 - (double)C->by[u + 1] * s64;
-                const double dRy64 = (double)C->bx[u + 1] * s64 + (double)C->by[u + 1] * c64;
-                double ddx = ((double)C->xds[u] * c64 - (double)C->yds[u] * s64) - srcx64;
-                double ddy = ((double)C->xds[u] * s64 + (double)C->yds[u] * c64) - srcy64;
-                const double dl = sqrt(ddx * ddx + ddy * ddy);
-                ddx /= dl;
-                ddy /= dl;
-                double summ = 0.0;
-                int32_t *tr = C->trace ? C->trace + ((((size_t)v * DNU + u) * SLN + z) * (size_t)(XN > YN ? XN : YN)) * 2 : NULL;
-                if (fabsf(c) <= fabsf(s)) { /* x-slabs, SAT along y (MultiSlicesProjGPU.cu:152-173) */
-                    const float fL = (dLy - srcy) / (dLx - srcx);
-                    const float fR = (dRy - srcy) / (dRx - srcx);
-                    const double fL64 = (dLy64 - srcy64) / (dLx64 - srcx64);
-                    const double fR64 = (dRy64 - srcy64) / (dRx64 - srcx64);
-                    const double *S = C->SY64 + (size_t)z * XN * (YN + 1);
-                    for (int ii = 0; ii < XN; ++ii) {
-                        const float obj = ((float)ii - ocx) * dx;
-                        const float rL = (obj - dLx) * fL + dLy;
-                        const float rR = (obj - dRx) * fR + dRy;
-                        const int iL = orc_texel(rL * invdx + ocy + 1.0f, YN);
-                        const int iR = orc_texel(rR * invdx + ocy + 1.0f, YN);
-                        const double obj64 = ((double)ii - (double)ocx) * (double)dx;
-                        const double rL64 = (obj64 - dLx64) * fL64 + dLy64;
-                        const double rR64 = (obj64 - dRx64) * fR64 + dRy64;
-                        const double* row = S + (size_t)ii * (YN + 1);
-                        const double TL = orc_fetch64(row, YN, iL, rL64 / (double)dx + (double)ocy + 1.0);
-                        const double TR = orc_fetch64(row, YN, iR, rR64 / (double)dx + (double)ocy + 1.0);
-                        summ += (TR - TL) / (rR64 - rL64);
-                        if (tr) { tr[2 * ii] = iL; tr[2 * ii + 1] = iR; }
+    const int Smax = XN > YN ? XN : YN;
+    orc_ray64 *R = (orc_ray64 *)malloc(sizeof(orc_ray64) * (size_t)DNU);
+    if (!R) return NULL;
+    for (long vz = tk->tid; vz < (long)C->PN * SLN; vz += tk->nthreads) {
+        const int v = (int)(vz / SLN), z = (int)(vz % SLN);
+        const float c = C->cs[2 * ((size_t)v * SLN + z)];
+        const float s = C->cs[2 * ((size_t)v * SLN + z) + 1];
+        /* FP32 index path (MultiSlicesProjGPU.cu:122-137) */
+        const float srcx = x0 * c - y0 * s, srcy = x0 * s + y0 * c;
+        /* FP64 continuous path */
+        const double c64 = c, s64 = s;
+        const double srcx64 = (double)x0 * c64 - (double)y0 * s64;
+        const double srcy64 = (double)x0 * s64 + (double)y0 * c64;
+        const int xslab = fabsf(c) <= fabsf(s); /* MultiSlicesProjGPU.cu:152 */
+        for (int u = 0; u < DNU; ++u) {
+            orc_ray64 *r = &R[u];
+            const float dLx = C->bx[u] * c - C->by[u] * s, dLy = C->bx[u] * s + C->by[u] * c;
+            const float dRx = C->bx[u + 1] * c - C->by[u + 1] * s;
+            const float dRy = C->bx[u + 1] * s + C->by[u + 1] * c;
+            const double dLx64 = (double)C->bx[u] * c64 - (double)C->by[u] * s64;
+            const double dLy64 = (double)C->bx[u] * s64 + (double)C->by[u] * c64;
+            const double dRx64 = (double)C->bx[u + 1] * c64 - (double)C->by[u + 1] * s64;
+            const double dRy64 = (double)C->bx[u + 1] * s64 + (double)C->by[u + 1] * c64;
+            double ddx = ((double)C->xds[u] * c64 - (double)C->yds[u] * s64) - srcx64;
+            double ddy = ((double)C->xds[u] * s64 + (double)C->yds[u] * c64) - srcy64;
+            const double dl = sqrt(ddx * ddx + ddy * ddy);
+            ddx /= dl;
+            ddy /= dl;
+            r->dLx = dLx; r->dLy = dLy; r->dRx = dRx; r->dRy = dRy;
+            r->dLx64 = dLx64; r->dLy64 = dLy64; r->dRx64 = dRx64; r->dRy64 = dRy64;
+            r->summ = 0.0;
+            if (xslab) { /* x-slabs, SAT along y (MultiSlicesProjGPU.cu:152-173) */
+                r->fL = (dLy - srcy) / (dLx - srcx);
+                r->fR = (dRy - srcy) / (dRx - srcx);
+                r->fL64 = (dLy64 - srcy64) / (dLx64 - srcx64);
+                r->fR64 = (dRy64 - srcy64) / (dRx64 - srcx64);
+                r->adir = fabs(ddx);
+            } else { /* y-slabs, SAT along x (MultiSlicesProjGPU.cu:176-196) */
+                r->fL = (dLx - srcx) / (dLy - srcy);
+                r->fR = (dRx - srcx) / (dRy - srcy);
+                r->fL64 = (dLx64 - srcx64) / (dLy64 - srcy64);
+                r->fR64 = (dRx64 - srcx64) / (dRy64 - srcy64);
+                r->adir = fabs(ddy);
+            }
+        }
+        if (xslab) {
+            const double *S = C->SY64 + (size_t)z * XN * (YN + 1);
+            for (int ii = 0; ii < XN; ++ii) {
+                const float obj = ((float)ii - ocx) * dx;
+                const double obj64 = ((double)ii - (double)ocx) * (double)dx;
+                const double *row = S + (size_t)ii * (YN + 1);
+                for (int u = 0; u < DNU; ++u) {
+                    orc_ray64 *r = &R[u];
+                    const float rL = (obj - r->dLx) * r->fL + r->dLy;
+                    const float rR = (obj - r->dRx) * r->fR + r->dRy;
+                    const int iL = orc_texel(rL * invdx + ocy + 1.0f, YN);
+                    const int iR = orc_texel(rR * invdx + ocy + 1.0f, YN);
+                    const double rL64 = (obj64 - r->dLx64) * r->fL64 + r->dLy64;
+                    const double rR64 = (obj64 - r->dRx64) * r->fR64 + r->dRy64;
+                    const double TL = orc_fetch64(row, YN, iL, rL64 / (double)dx + (double)ocy + 1.0);
+                    const double TR = orc_fetch64(row, YN, iR, rR64 / (double)dx + (double)ocy + 1.0);
+                    r->summ += (TR - TL) / (rR64 - rL64);
+                    if (C->trace) {
+                        int32_t *tr = C->trace + ((((size_t)v * DNU + u) * SLN + z) * (size_t)Smax) * 2;
+                        tr[2 * ii] = iL;
+                        tr[2 * ii + 1] = iR;
                     }
-                    C->prj64[((size_t)v * DNU + u) * SLN + z] = summ * (double)dx / fabs(ddx);
-                } else { /* y-slabs, SAT along x (MultiSlicesProjGPU.cu:176-196) */
-                    const float fL = (dLx - srcx) / (dLy - srcy);
-                    const float fR = (dRx - srcx) / (dRy - srcy);
-                    const double fL64 = (dLx64 - srcx64) / (dLy64 - srcy64);
-                    const double fR64 = (dRx64 - srcx64) / (dRy64 - srcy64);
-                    const double *S = C->SX64 + (size_t)z * YN * (XN + 1);
-                    for (int ii = 0; ii < YN; ++ii) {
-                        const float obj = ((float)ii - ocy) * dx;
-                        const float rL = (obj - dLy) * fL + dLx;
-                        const float rR = (obj - dRy) * fR + dRx;
-                        const int iL = orc_texel(rL * invdx + ocx + 1.0f, XN);
-                        const int iR = orc_texel(rR * invdx + ocx + 1.0f, XN);
-                        const double obj64 = ((double)ii - (double)ocy) * (double)dx;
-                        const double rL64 = (obj64 - dLy64) * fL64 + dLx64;
-                        const double rR64 = (obj64 - dRy64) * fR64 + dRx64;
-                        const double* row = S + (size_t)ii * (XN + 1);
-                        const double TL = orc_fetch64(row, XN, iL, rL64 / (double)dx + (double)ocx + 1.0);
-                        const double TR = orc_fetch64(row, XN, iR, rR64 / (double)dx + (double)ocx + 1.0);
-                        summ += (TR - TL) / (rR64 - rL64);
-                        if (tr) { tr[2 * ii] = iL; tr[2 * ii + 1] = iR; }
+                }
+            }
+        } else {
+            const double *S = C->SX64 + (size_t)z * YN * (XN + 1);
+            for (int ii = 0; ii < YN; ++ii) {
+                const float obj = ((float)ii - ocy) * dx;
+                const double obj64 = ((double)ii - (double)ocy) * (double)dx;
+                const double *row = S + (size_t)ii * (XN + 1);
+                for (int u = 0; u < DNU; ++u) {
+                    orc_ray64 *r = &R[u];
+                    const float rL = (obj - r->dLy) * r->fL + r->dLx;
+                    const float rR = (obj - r->dRy) * r->fR + r->dRx;
+                    const int iL = orc_texel(rL * invdx + ocx + 1.0f, XN);
+                    const int iR = orc_texel(rR * invdx + ocx + 1.0f, XN);
+                    const double rL64 = (obj64 - r->dLy64) * r->fL64 + r->dLx64;
+                    const double rR64 = (obj64 - r->dRy64) * r->fR64 + r->dRx64;
+                    const double TL = orc_fetch64(row, XN, iL, rL64 / (double)dx + (double)ocx + 1.0);
+                    const double TR = orc_fetch64(row, XN, iR, rR64 / (double)dx + (double)ocx + 1.0);
+                    r->summ += (TR - TL) / (rR64 - rL64);
+                    if (C->trace) {
+                        int32_t *tr = C->trace + ((((size_t)v * DNU + u) * SLN + z) * (size_t)Smax) * 2;
+                        tr[2 * ii] = iL;
+                        tr[2 * ii + 1] = iR;
                     }
-                    C->prj64[((size_t)v * DNU + u) * SLN + z] = summ * (double)dx / fabs(ddy);
                 }
             }
         }
+        for (int u = 0; u < DNU; ++u)
+            C->prj64[((size_t)v * DNU + u) * SLN + z] = R[u].summ * (double)dx / R[u].adir;
     }
+    free(R);
     return NULL;
 }
+
+typedef struct {
+    float dLx, dLy, dRx, dRy, fL, fR, cv, summ;
+} orc_ray32;
 
 static void *orc_proj_worker32(void *arg) {
     orc_task *tk = (orc_task *)arg;
     orc_proj_ctx *C = (orc_proj_ctx *)tk->ctx;
     const int SLN = C->SLN, DNU = C->DNU, XN = C->XN, YN = C->YN;
     const float x0 = C->x0, y0 = C->y0, dx = C->dx, ocx = C->ocx, ocy = C->ocy, invdx = C->invdx;
-    for (long vu = tk->tid; vu < (long)C->PN * DNU; vu += tk->nthreads) {
-        {
-            const int v = (int)(vu / DNU), u = (int)(vu % DNU);
-            for (int z = 0; z < SLN; ++z) {
-                const float c = C->cs[2 * ((size_t)v * SLN + z)];
-                const float s = C->cs[2 * ((size_t)v * SLN + z) + 1];
-                const float srcx = x0 * c - y0 * s, srcy = x0 * s + y0 * c;
-                const float dLx = C->bx[u] * c - C->by[u] * s, dLy = C->bx[u] * s + C->by[u] * c;
-                const float dRx = C->bx[u + 1] * c - C->by[u + 1] * s;
-                const float dRy = C->bx[u + 1] * s + C->by[u + 1] * c;
-                float ddx = (C->xds[u] * c - C->yds[u] * s) - srcx;
-                float ddy = (C->xds[u] * s + C->yds[u] * c) - srcy;
-                const float dl = sqrtf(ddx * ddx + ddy * ddy);
-                ddx = ddx / dl;
-                ddy = ddy / dl;
-                float summ = 0.0f;
-                if (fabsf(c) <= fabsf(s)) {
-                    const float fL = (dLy - srcy) / (dLx - srcx);
-                    const float fR = (dRy - srcy) / (dRx - srcx);
-                    const float cv = dx / fabsf(ddx);
-                    const float *S = C->SY32 + (size_t)z * XN * (YN + 1);
-                    for (int ii = 0; ii < XN; ++ii) {
-                        const float obj = ((float)ii - ocx) * dx;
-                        const float rL = (obj - dLx) * fL + dLy;
-                        const float rR = (obj - dRx) * fR + dRy;
-                        const float len = rR - rL;
-                        const float *row = S + (size_t)ii * (YN + 1);
-                        summ += (orc_fetch32(row, YN, rR * invdx + ocy + 1.0f) -
-                                 orc_fetch32(row, YN, rL * invdx + ocy + 1.0f)) / len;
-                    }
-                    C->prj32[((size_t)v * DNU + u) * SLN + z] = summ * cv;
-                } else {
-                    const float fL = (dLx - srcx) / (dLy - srcy);
-                    const float fR = (dRx - srcx) / (dRy - srcy);
-                    const float cv = dx / fabsf(ddy);
-                    const float *S = C->SX32 + (size_t)z * YN * (XN + 1);
-                    for (int ii = 0; ii < YN; ++ii) {
-                        const float obj = ((float)ii - ocy) * dx;
-                        const float rL = (obj - dLy) * fL + dLx;
-                        const float rR = (obj - dRy) * fR + dRx;
-                        const float len = rR - rL;
-                        const float *row = S + (size_t)ii * (XN + 1);
-                        summ += (orc_fetch32(row, XN, rR * invdx + ocx + 1.0f) -
-                                 orc_fetch32(row, XN, rL * invdx + ocx + 1.0f)) / len;
-                    }
-                    C->prj32[((size_t)v * DNU + u) * SLN + z] = summ * cv;
+    orc_ray32 *R = (orc_ray32 *)malloc(sizeof(orc_ray32) * (size_t)DNU);
+    if (!R) return NULL;
+    for (long vz = tk->tid; vz < (long)C->PN * SLN; vz += tk->nthreads) {
+        const int v = (int)(vz / SLN), z = (int)(vz % SLN);
+        const float c = C->cs[2 * ((size_t)v * SLN + z)];
+        const float s = C->cs[2 * ((size_t)v * SLN + z) + 1];
+        const float srcx = x0 * c - y0 * s, srcy = x0 * s + y0 * c;
+        const int xslab = fabsf(c) <= fabsf(s);
+        for (int u = 0; u < DNU; ++u) {
+            orc_ray32 *r = &R[u];
+            const float dLx = C->bx[u] * c - C->by[u] * s, dLy = C->bx[u] * s + C->by[u] * c;
+            const float dRx = C->bx[u + 1] * c - C->by[u + 1] * s;
+            const float dRy = C->bx[u + 1] * s + C->by[u + 1] * c;
+            float ddx = (C->xds[u] * c - C->yds[u] * s) - srcx;
+            float ddy = (C->xds[u] * s + C->yds[u] * c) - srcy;
+            const float dl = sqrtf(ddx * ddx + ddy * ddy);
+            ddx = ddx / dl;
+            ddy = ddy / dl;
+            r->dLx = dLx; r->dLy = dLy; r->dRx = dRx; r->dRy = dRy;
+            r->summ = 0.0f;
+            if (xslab) {
+                r->fL = (dLy - srcy) / (dLx - srcx);
+                r->fR = (dRy - srcy) / (dRx - srcx);
+                r->cv = dx / fabsf(ddx);
+            } else {
+                r->fL = (dLx - srcx) / (dLy - srcy);
+                r->fR = (dRx - srcx) / (dRy - srcy);
+                r->cv = dx / fabsf(ddy);
+            }
+        }
+        if (xslab) {
+            const float *S = C->SY32 + (size_t)z * XN * (YN + 1);
+            for (int ii = 0; ii < XN; ++ii) {
+                const float obj = ((float)ii - ocx) * dx;
+                const float *row = S + (size_t)ii * (YN + 1);
+                for (int u = 0; u < DNU; ++u) {
+                    orc_ray32 *r = &R[u];
+                    const float rL = (obj - r->dLx) * r->fL + r->dLy;
+                    const float rR = (obj - r->dRx) * r->fR + r->dRy;
+                    const float len = rR - rL;
+                    r->summ += (orc_fetch32(row, YN, rR * invdx + ocy + 1.0f) -
+                                orc_fetch32(row, YN, rL * invdx + ocy + 1.0f)) / len;
+                }
+            }
+        } else {
+            const float *S = C->SX32 + (size_t)z * YN * (XN + 1);
+            for (int ii = 0; ii < YN; ++ii) {
+                const float obj = ((float)ii - ocy) * dx;
+                const float *row = S + (size_t)ii * (XN + 1);
+                for (int u = 0; u < DNU; ++u) {
+                    orc_ray32 *r = &R[u];
+                    const float rL = (obj - r->dLy) * r->fL + r->dLx;
+                    const float rR = (obj - r->dRy) * r->fR + r->dRx;
+                    const float len = rR - rL;
+                    r->summ += (orc_fetch32(row, XN, rR * invdx + ocx + 1.0f) -
+                                orc_fetch32(row, XN, rL * invdx + ocx + 1.0f)) / len;
                 }
             }
         }
+        for (int u = 0; u < DNU; ++u)
+            C->prj32[((size_t)v * DNU + u) * SLN + z] = R[u].summ * R[u].cv;
     }
+    free(R);
     return NULL;
 }
 
@@ -509,6 +561,10 @@ typedef struct {
     int32_t *trace; /* optional [y][x][z][v][2] */
 } orc_back_ctx;
 
+/* Work item = one image row y; per slice the views run in the outer loop and
+ * the row's voxels in the inner one, so each view's table row is read for
+ * the whole image row while it is in cache.  Every voxel still sums its
+ * views in ascending order with the same operations. */
 static void *orc_back_worker64(void *arg) {
     orc_task *tk = (orc_task *)arg;
     orc_back_ctx *C = (orc_back_ctx *)tk->ctx;
@@ -519,57 +575,59 @@ static void *orc_back_worker64(void *arg) {
     const float invdb = (float)(1.0 / (double)C->dbeta);
     const double inv_sid64 = 1.0 / sqrt((double)x0 * x0 + (double)y0 * y0);
     const double invdb64 = 1.0 / (double)C->dbeta;
+    double *acc = (double *)malloc(sizeof(double) * (size_t)XN);
+    if (!acc) return NULL;
     for (int y = tk->tid; y < C->YN; y += tk->nthreads) {
-        for (int x = 0; x < XN; ++x) {
-            const int m = C->mask[y * XN + x];
-            for (int z = 0; z < SLN; ++z) {
-                const size_t o = ((size_t)y * XN + x) * SLN + z;
-                if (m == 0) {
-                    C->vol64[o] = 0.0;
-                    continue;
-                }
-                const float vx = ((float)x - C->ocx) * dx, vy = ((float)y - C->ocy) * dx;
-                const double vx64 = ((double)x - C->ocx64) * dx, vy64 = ((double)y - C->ocy64) * dx;
-                double summ = 0.0;
-                for (int v = 0; v < PN; ++v) {
-                    const float c = C->cs[2 * ((size_t)v * SLN + z)];
-                    const float s = C->cs[2 * ((size_t)v * SLN + z) + 1];
-                    /* FP32 index path (MultiSlicesBackGPU.cu:155-176) */
-                    const float srcx = x0 * c - y0 * s, srcy = x0 * s + y0 * c;
+        const float vy = ((float)y - C->ocy) * dx;
+        const double vy64 = ((double)y - C->ocy64) * dx;
+        for (int z = 0; z < SLN; ++z) {
+            for (int x = 0; x < XN; ++x) acc[x] = 0.0;
+            for (int v = 0; v < PN; ++v) {
+                const float c = C->cs[2 * ((size_t)v * SLN + z)];
+                const float s = C->cs[2 * ((size_t)v * SLN + z) + 1];
+                /* FP32 index path (MultiSlicesBackGPU.cu:155-176) */
+                const float srcx = x0 * c - y0 * s, srcy = x0 * s + y0 * c;
+                const int bxsel = fabsf(srcx) > fabsf(srcy);
+                /* FP64 continuous path */
+                const double c64 = c, s64 = s;
+                const double srcx64 = (double)x0 * c64 - (double)y0 * s64;
+                const double srcy64 = (double)x0 * s64 + (double)y0 * c64;
+                const double *row = C->P64 + ((size_t)z * PN + v) * (DNU + 1);
+                for (int x = 0; x < XN; ++x) {
+                    if (C->mask[y * XN + x] == 0) continue;
+                    const float vx = ((float)x - C->ocx) * dx;
+                    const double vx64 = ((double)x - C->ocx64) * dx;
                     const float ddx = vx - srcx, ddy = vy - srcy;
                     const float l2 = ddx * ddx + ddy * ddy;
                     const float l = 1.0f / sqrtf(l2);
                     const float alpha = orc_asinf32(((srcy * ddx - srcx * ddy) * inv_sid) * l);
-                    const int bxsel = fabsf(srcx) > fabsf(srcy);
                     const float ddv = bxsel ? ddx : ddy;
                     const float dA = ((ddv / l2) * dx) * 0.5f;
                     const float idxL = ((alpha - dA) * invdb + C->detCntIdx) + 1.0f;
                     const float idxR = ((alpha + dA) * invdb + C->detCntIdx) + 1.0f;
                     const int iL = orc_texel(idxL, DNU), iR = orc_texel(idxR, DNU);
-                    /* FP64 continuous path */
-                    const double c64 = c, s64 = s;
-                    const double srcx64 = (double)x0 * c64 - (double)y0 * s64;
-                    const double srcy64 = (double)x0 * s64 + (double)y0 * c64;
                     const double ex = vx64 - srcx64, ey = vy64 - srcy64;
                     const double l264 = ex * ex + ey * ey;
                     const double al64 = asin((srcy64 * ex - srcx64 * ey) * inv_sid64 / sqrt(l264));
                     const double ddv64 = bxsel ? ex : ey;
                     const double dA64 = ddv64 / l264 * (double)dx * 0.5;
                     const double cosVal = (double)dx / ddv64 * sqrt(l264);
-                    const double *row = C->P64 + ((size_t)z * PN + v) * (DNU + 1);
                     const double TL = orc_fetch64(row, DNU, iL, (al64 - dA64) * invdb64 + C->detCntIdx + 1.0);
                     const double TR = orc_fetch64(row, DNU, iR, (al64 + dA64) * invdb64 + C->detCntIdx + 1.0);
-                    summ += (TR - TL) * cosVal;
+                    acc[x] += (TR - TL) * cosVal;
                     if (C->trace) {
+                        const size_t o = ((size_t)y * XN + x) * SLN + z;
                         int32_t *tr = C->trace + (o * PN + v) * 2;
                         tr[0] = iL;
                         tr[1] = iR;
                     }
                 }
-                C->vol64[o] = summ;
             }
+            for (int x = 0; x < XN; ++x)
+                C->vol64[((size_t)y * XN + x) * SLN + z] = C->mask[y * XN + x] == 0 ? 0.0 : acc[x];
         }
     }
+    free(acc);
     return NULL;
 }
 
@@ -580,37 +638,38 @@ static void *orc_back_worker32(void *arg) {
     const float x0 = C->x0, y0 = C->y0, dx = C->dx;
     const float inv_sid = 1.0f / sqrtf(x0 * x0 + y0 * y0);
     const float invdb = (float)(1.0 / (double)C->dbeta);
+    float *acc = (float *)malloc(sizeof(float) * (size_t)XN);
+    if (!acc) return NULL;
     for (int y = tk->tid; y < C->YN; y += tk->nthreads) {
-        for (int x = 0; x < XN; ++x) {
-            const int m = C->mask[y * XN + x];
-            for (int z = 0; z < SLN; ++z) {
-                const size_t o = ((size_t)y * XN + x) * SLN + z;
-                if (m == 0) {
-                    C->vol32[o] = 0.0f;
-                    continue;
-                }
-                const float vx = ((float)x - C->ocx) * dx, vy = ((float)y - C->ocy) * dx;
-                float summ = 0.0f;
-                for (int v = 0; v < PN; ++v) {
-                    const float c = C->cs[2 * ((size_t)v * SLN + z)];
-                    const float s = C->cs[2 * ((size_t)v * SLN + z) + 1];
-                    const float srcx = x0 * c - y0 * s, srcy = x0 * s + y0 * c;
+        const float vy = ((float)y - C->ocy) * dx;
+        for (int z = 0; z < SLN; ++z) {
+            for (int x = 0; x < XN; ++x) acc[x] = 0.0f;
+            for (int v = 0; v < PN; ++v) {
+                const float c = C->cs[2 * ((size_t)v * SLN + z)];
+                const float s = C->cs[2 * ((size_t)v * SLN + z) + 1];
+                const float srcx = x0 * c - y0 * s, srcy = x0 * s + y0 * c;
+                const int bxsel = fabsf(srcx) > fabsf(srcy);
+                const float *row = C->P32 + ((size_t)z * PN + v) * (DNU + 1);
+                for (int x = 0; x < XN; ++x) {
+                    if (C->mask[y * XN + x] == 0) continue;
+                    const float vx = ((float)x - C->ocx) * dx;
                     const float ddx = vx - srcx, ddy = vy - srcy;
                     const float l2 = ddx * ddx + ddy * ddy;
                     const float l = 1.0f / sqrtf(l2);
                     const float alpha = orc_asinf32(((srcy * ddx - srcx * ddy) * inv_sid) * l);
-                    const float ddv = fabsf(srcx) > fabsf(srcy) ? ddx : ddy;
+                    const float ddv = bxsel ? ddx : ddy;
                     const float dA = ((ddv / l2) * dx) * 0.5f;
                     const float cosVal = (dx / ddv) * sqrtf(l2);
                     const float idxL = ((alpha - dA) * invdb + C->detCntIdx) + 1.0f;
                     const float idxR = ((alpha + dA) * invdb + C->detCntIdx) + 1.0f;
-                    const float *row = C->P32 + ((size_t)z * PN + v) * (DNU + 1);
-                    summ += (orc_fetch32(row, DNU, idxR) - orc_fetch32(row, DNU, idxL)) * cosVal;
+                    acc[x] += (orc_fetch32(row, DNU, idxR) - orc_fetch32(row, DNU, idxL)) * cosVal;
                 }
-                C->vol32[o] = summ;
             }
+            for (int x = 0; x < XN; ++x)
+                C->vol32[((size_t)y * XN + x) * SLN + z] = C->mask[y * XN + x] == 0 ? 0.0f : acc[x];
         }
     }
+    free(acc);
     return NULL;
 }
 
