@@ -73,6 +73,8 @@ _SIGS = {
     "ddfan_set_default_sat_mode": [_i],
     "ddfan_proj": [_f, _f, _i, _p, _p, _f, _f, _p, _i, _i, _i, _i, _p, _p, _f, _p, _p, _i],
     "ddfan_back": [_f, _f, _i, _p, _p, _f, _f, _f, _p, _i, _i, _i, _i, _p, _p, _f, _p, _p, _i],
+    "ddfan_proj_noalloc": [_f, _f, _i, _p, _p, _f, _f, _p, _i, _i, _i, _i, _p, _p, _f, _p, _p,
+                           _i],
     "ddfan_helical_to_fan": [_p, _p, _p, _ll, _p, _i, _f, _f, _f, _i, _i, _f, _f, _i, _f, _f],
     "ddfan_detector_arc": [_i, _f, _f, _f, _f, _p, _p],
     "ddfan_recon_mask": [_i, _i, _p],

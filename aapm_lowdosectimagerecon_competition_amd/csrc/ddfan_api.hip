@@ -58,7 +58,7 @@ int ensure_vol_sat(ddfan_plan* P) {
 
 }  // namespace ddf
 
-namespace {
+namespace ddf {
 
 int check_subset(int PN, int first, int stride, int count) {
     CHECK(count >= 0 && stride >= 1 && first >= 0, DDFAN_EINVAL,
@@ -81,10 +81,10 @@ int check_start_idx(const int* startIdx, int nStart, int SLN) {
     return DDFAN_OK;
 }
 
-}  // namespace
+}  // namespace ddf
 
 // =================================================================== plan
-namespace {
+namespace ddf {
 
 void fill_geom(GeomDev& g, const ddfan_geom* G, int SLN) {
     g.x0 = G->x0;
@@ -106,6 +106,8 @@ void fill_geom(GeomDev& g, const ddfan_geom* G, int SLN) {
     g.DNU = G->DNU;
 }
 
+int default_sat_mode() { return g_default_sat.load(); }
+
 int validate_geom(const ddfan_geom* G) {
     CHECK(G, DDFAN_EINVAL, "geometry is NULL");
     CHECK(G->DNU >= 2, DDFAN_EINVAL, "DNU must be >= 2 (got %d)", G->DNU);
@@ -117,7 +119,7 @@ int validate_geom(const ddfan_geom* G) {
 }
 
 
-}  // namespace
+}  // namespace ddf
 
 extern "C" {
 

@@ -75,6 +75,14 @@ struct DBuf {
 int check_device(int* dev);
 int upload(DBuf& b, const void* src, size_t bytes, hipStream_t st);
 
+// Argument checks and kernel scalars shared by the entry points.
+int check_subset(int PN, int first, int stride, int count);
+int check_start_idx(const int* startIdx, int nStart, int SLN);
+int validate_geom(const ddfan_geom* G);
+void fill_geom(GeomDev& g, const ddfan_geom* G, int SLN);
+// SAT precision of the drop-in entry points (ddfan_set_default_sat_mode).
+int default_sat_mode();
+
 // Tables carry 256 bytes of slack: the staged projector and backprojector copy
 // rows in 16-byte pieces, which may end one 8-byte f32 entry past a row.
 inline size_t sat_vol_bytes(int mode, int nzc, int ZT, int XN, int YN) {

@@ -90,6 +90,22 @@ int ddfan_helical_to_fan(float* Proj, float* Views, const float* proj, long long
                          int DetWidth, int DetHeight, float PerDetW, float PerDetH, int DefTimes,
                          float DetCenterW, float SpiralPitchFactor);
 
+/* DD projector without device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under stream capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Arguments as
+ * ddfan_proj, except that d_vol ([YN][XN][SLN]) and d_prj ([PN][DNU][SLN]) are
+ * device buffers on the current device; the small host arrays (xds, yds,
+ * hangs, mask) are copied in.  The working tables live in a fixed device array
+ * of 512 MiB per GPU (held until the process ends): a projection whose tables
+ * need more (e.g. beyond 512x512x64 slices with the default f32 tables)
+ * returns DDFAN_EINVAL.  Calls on one GPU share that array and are serialised;
+ * each returns after its projection has completed.  Results are bit-identical
+ * to ddfan_proj. */
+int ddfan_proj_noalloc(float x0, float y0, int DNU, const float* xds, const float* yds,
+                       float imgXCenter, float imgYCenter, const float* hangs, int PN, int XN,
+                       int YN, int SLN, const float* d_vol, float* d_prj, float dx,
+                       const unsigned char* mask, const int* startIdx, int nStart);
+
 /* ------------------------------------------------------------ geometry tables */
 
 /* Equiangular arc detector centres, DD2MutiSlices.m:61-76 (MATLAB single
