@@ -145,3 +145,62 @@ def gaussian_count_sinogram(prj, dx, I0, seed=2, threads=None):
     with ThreadPoolExecutor(max_workers=n) as ex:
         list(ex.map(work, range(nchunk)))
     return out
+
+
+# ------------------------------------------------------------------ workloads
+MIX_XN = (256, 384, 512, 640, 768)
+MIX_PN = (720, 1152, 1536, 2304)
+
+
+def mix_specs(n, seed):
+    """Sizes of the uneven reconstruction mix: image 256^2-768^2 over a 500 mm
+    field of view, 8-128 slices, 720-2304 views, I0 log-uniform in
+    [5e4, 5e5]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        XN = int(rng.choice(MIX_XN))
+        out.append({"XN": XN, "SLN": int(rng.integers(8, 129)), "PN": int(rng.choice(MIX_PN)),
+                    "I0": float(np.exp(rng.uniform(np.log(5e4), np.log(5e5))))})
+    return out
+
+
+def helical_cylinder_counts(W, H, DefTimes, rotations, SO, step, detc, I0=1e5, gain=0.5,
+                            radius=200.0, mu=0.019, seed=5, noise=True, threads=None):
+    """Raw helical data raw[view, ch, row] (uint16) of a water cylinder of
+    `radius` mm on the rotation axis: expected counts gain*I0*exp(-mu*chord)
+    per channel (arc detector of `step`-radian channels centred on 1-based
+    channel `detc`, source radius SO), modulated by 5% within each rotation
+    and by a seeded +-2% row gain; with `noise`, every sample gets the normal
+    approximation of Poisson noise (lam + sqrt(lam)*N(0,1), each rotation
+    from its own child seed); rounded and clipped to [1, 65535] (the gain
+    keeps I0 = 1e5 inside 16 bits)."""
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+    beta = (np.arange(W) - (detc - 1)) * step
+    d = SO * np.abs(np.sin(beta))
+    chord = 2.0 * np.sqrt(np.maximum(radius ** 2 - d ** 2, 0.0))
+    base = gain * I0 * np.exp(-mu * chord)
+    rng = np.random.default_rng(seed)
+    mod = 1.0 + 0.05 * np.sin(6 * np.pi * np.arange(DefTimes) / DefTimes)
+    lam = (base[None, :, None] * mod[:, None, None] *
+           rng.uniform(0.98, 1.02, (1, 1, H))).astype(np.float32)
+    TV = DefTimes * rotations
+    raw = np.empty((TV, W, H), np.uint16)
+    if not noise:
+        one = np.clip(np.rint(lam), 1, 65535).astype(np.uint16)
+        for r0 in range(0, TV, DefTimes):
+            raw[r0:r0 + DefTimes] = one
+        return raw
+    seeds = np.random.SeedSequence(seed).spawn(rotations)
+    sd = np.sqrt(lam)
+
+    def work(r):
+        g = np.random.default_rng(seeds[r])
+        c = lam + sd * g.standard_normal(lam.shape, dtype=np.float32)
+        raw[r * DefTimes:(r + 1) * DefTimes] = np.clip(np.rint(c), 1, 65535)
+
+    n = threads or min(16, os.cpu_count() or 1)
+    with ThreadPoolExecutor(max_workers=n) as ex:
+        list(ex.map(work, range(rotations)))
+    return raw

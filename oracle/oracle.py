@@ -67,10 +67,17 @@ def _c(a, dtype):
 
 
 def nthreads_default() -> int:
+    """Host threads for the restatement: the CPUs this process may run on,
+    capped by OMP_NUM_THREADS when that is set (a container's CPU share)."""
     try:
-        return max(1, len(os.sched_getaffinity(0)))
+        n = max(1, len(os.sched_getaffinity(0)))
     except AttributeError:  # pragma: no cover
-        return os.cpu_count() or 1
+        n = os.cpu_count() or 1
+    try:
+        cap = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        cap = 0
+    return min(n, cap) if cap > 0 else n
 
 
 # ----------------------------------------------------------------- tables

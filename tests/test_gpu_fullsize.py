@@ -7,7 +7,7 @@ the GPU computes every view; the FP64 oracle checks an evenly spaced sample of
 random per view and averages out over views: 8 views alone gave 5.4e-6);
 linearity holds over the whole sinogram (<= 1e-6).
 
-OS-SART at config-1 size (2 iterations x 24 subsets of 96 views, FISTA, from
+OS-SART at config-1 size (10 iterations x 24 subsets of 96 views, FISTA, from
 the GPU's own projection of the phantom): slices are independent (OS-SART
 replicates slice 0's weights, OSSART_AAPM.m:63-64), so slices 0 and 63 of the
 64-slice GPU run are compared with the FP64 oracle run on those two slices;
@@ -73,8 +73,8 @@ def test_config1_projector_linearity():
 
 def test_config1_ossart_vs_fp64_oracle_on_two_slices(oracle):
     dd, g, hangs, vol, mask = _config1()
-    it, nos = 2, 24
-    print("\n[full] OS-SART 512x512x64, 2304 views, 2 it x 24 subsets: GPU all slices, "
+    it, nos = 10, 24
+    print("\n[full] OS-SART 512x512x64, 2304 views, 10 it x 24 subsets: GPU all slices, "
           "oracle slices 0 and 63", flush=True)
     with dd.Plan(g, hangs, mask, sat_mode="twofloat") as plan:
         meas = plan.proj_host(vol)
