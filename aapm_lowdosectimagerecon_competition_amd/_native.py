@@ -47,6 +47,8 @@ TUNE_USE_CACHE = 1
 BATCH_PROJ_FILL = 1
 BATCH_BACK_FILL = 2
 BATCH_SOLO_BYTES = 3
+BATCH_STREAMS = 4
+SERVER_GRAPH = 1
 
 
 class DDFanError(RuntimeError):
@@ -112,6 +114,7 @@ _SIGS = {
     "ddfan_server_submit": [_p, ctypes.POINTER(GeomStruct), _p, _i, _p, _p, _p, _i, _i, _i,
                             ctypes.POINTER(_ll)],
     "ddfan_server_poll": [_p, _ll, _ip],
+    "ddfan_server_set_param": [_p, _i, _ll],
     "ddfan_server_wait": [_p, _ll, _p, ctypes.POINTER(ctypes.c_double),
                           ctypes.POINTER(ctypes.c_double)],
     "ddfan_server_stats": [_p, ctypes.POINTER(_ll), ctypes.POINTER(_ll), _ip, ctypes.POINTER(_ll),

@@ -40,6 +40,12 @@ class Batch:
         if back_blocks is not None:
             N.call("ddfan_batch_set_param", self.handle, N.BATCH_BACK_FILL, int(back_blocks))
 
+    def set_streams(self, n: int):
+        """Split the reconstructions into `n` groups of about equal work that
+        run side by side, each on its own stream (1: one group; 0: four when
+        every reconstruction is small, else one)."""
+        N.call("ddfan_batch_set_param", self.handle, N.BATCH_STREAMS, int(n))
+
     def __len__(self):
         n = ctypes.c_int()
         N.call("ddfan_batch_size", self.handle, ctypes.byref(n))

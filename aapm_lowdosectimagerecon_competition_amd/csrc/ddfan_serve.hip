@@ -7,7 +7,8 @@
 // arrival order while a slot is free and their device footprint fits in the
 // HBM budget (ddf::Admission), creates their plans and uploads the queued
 // requests' inputs, and runs every admitted
-// reconstruction together, one batched OS-SART iteration (a captured hipGraph)
+// reconstruction together, one batched OS-SART iteration (launched directly, or
+// replayed as a captured hipGraph with DDFAN_SERVER_GRAPH)
 // at a time.  Requests join and leave at iteration boundaries, so a new one
 // starts while others are mid-run, and each request's arithmetic is the same
 // launch sequence it would get alone.  The worker keeps at most two batch
@@ -34,6 +35,7 @@
 #include "ddfan_guard.h"
 #include "ddfan_plan.h"
 #include "ddfan_sched.h"
+#include "ddfan_serve_hooks.h"
 
 using namespace ddf;
 
@@ -177,6 +179,8 @@ struct ddfan_server {
     std::condition_variable cvDead;
     long long nextId = 1;
     bool stop = false, reaperStop = false;
+    ddf::ServerHooks hooks;  // how the worker checks GPU completion (ddfan_serve_hooks.h)
+    bool graphs_ = false;    // DDFAN_SERVER_GRAPH: replay each membership's iteration as a hipGraph
     std::thread worker, reaper;
     // statistics
     long long completed = 0, failed = 0;
@@ -270,7 +274,9 @@ void ddfan_server::loop() {
     (void)hipSetDevice(device);
     std::vector<Request*> running;   // admitted, iterations left to enqueue
     std::vector<Request*> finishing; // last iteration enqueued
+    const bool useGraphs = graphs_;  // fixed before the first submission
     IterGraph graphs[3];             // current graph + up to two with launches in flight
+    std::vector<OsJob*> jobs;        // the running requests' jobs, in launch order
     hipEvent_t iterEv[2] = {nullptr, nullptr};
     (void)hipEventCreateWithFlags(&iterEv[0], hipEventDisableTiming);
     (void)hipEventCreateWithFlags(&iterEv[1], hipEventDisableTiming);
@@ -302,15 +308,29 @@ void ddfan_server::loop() {
         running.clear();
         finishing.clear();
     };
+    // A failed completion check means the device reported an error for work
+    // in flight: every admitted request fails with it (none of their results
+    // can be trusted) and the next iteration is captured afresh.
+    auto device_failed = [&](hipError_t e, const char* what) {
+        fail_all_running(DDFAN_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+        inFlight[0] = inFlight[1] = false;
+        graphIdx[0] = graphIdx[1] = -1;
+        cur = -1;
+    };
     // Copies back and completes the requests whose last iteration has run:
     // into the output of a caller already waiting for it, else into a host
-    // buffer of the request.
-    auto collect = [&]() {
+    // buffer of the request.  Returns false after a device error.
+    auto collect = [&]() -> bool {
         for (size_t i = 0; i < finishing.size();) {
             Request* R = finishing[i];
-            if (hipEventQuery(R->lastIter) != hipSuccess) {
+            const hipError_t q = hooks.eventQuery(R->lastIter);
+            if (q == hipErrorNotReady) {
                 ++i;
                 continue;
+            }
+            if (q != hipSuccess) {
+                device_failed(q, "checking a request's last iteration");
+                return false;
             }
             const size_t nv = (size_t)R->G.XN * R->G.YN * R->G.SLN;
             int st = DDFAN_OK;
@@ -325,7 +345,7 @@ void ddfan_server::loop() {
             if (dst) {
                 if (hipMemcpyAsync(dst, R->d_x.p, nv * sizeof(float), hipMemcpyDeviceToHost,
                                    copyStream) != hipSuccess ||
-                    hipStreamSynchronize(copyStream) != hipSuccess) {
+                    hooks.streamSync(copyStream) != hipSuccess) {
                     st = DDFAN_EDEVICE;
                     err = "result copy failed";
                 }
@@ -334,13 +354,14 @@ void ddfan_server::loop() {
                 err = "host allocation of the result failed";
             } else if (hipMemcpyAsync(R->result.data(), R->d_x.p, nv * sizeof(float),
                                       hipMemcpyDeviceToHost, copyStream) != hipSuccess ||
-                       hipStreamSynchronize(copyStream) != hipSuccess) {
+                       hooks.streamSync(copyStream) != hipSuccess) {
                 st = DDFAN_EDEVICE;
                 err = "result copy failed";
             }
             finish(R, st, err);
             finishing.erase(finishing.begin() + i);
         }
+        return true;
     };
 
     for (;;) {
@@ -384,36 +405,59 @@ void ddfan_server::loop() {
             }
         }
         if (running.empty()) {
-            (void)hipStreamSynchronize(stream);
-            collect();
+            const hipError_t e = hooks.streamSync(stream);
+            if (e != hipSuccess)
+                device_failed(e, "waiting for the last iterations");
+            else
+                collect();
             continue;
         }
         // one batch iteration of every running request
         const int slot = (int)(iterNo & 1);
         if (inFlight[slot]) {  // keep at most two iterations in flight
-            (void)hipEventSynchronize(iterEv[slot]);
+            const hipError_t e = hooks.eventSync(iterEv[slot]);
             inFlight[slot] = false;
             graphIdx[slot] = -1;
+            if (e != hipSuccess) {
+                device_failed(e, "waiting for a batch iteration");
+                continue;
+            }
         }
-        if (changed || cur < 0) {
-            int g = 0;  // a slot that no in-flight iteration uses
-            while (g == graphIdx[0] || g == graphIdx[1]) ++g;
-            std::vector<OsJob*> jobs;
+        if (!useGraphs) {
+            // Eager launches of the same kernels: no stream capture is ever in
+            // progress on the worker while other threads allocate, copy or free.
+            jobs.clear();
             for (Request* R : running) jobs.push_back(&R->job);
-            const int rc = graphs[g].capture(jobs.data(), (int)jobs.size(), mode, shape, stream);
-            if (rc != DDFAN_OK) {
+            const hipError_t e = enqueue_batch_iteration(jobs.data(), (int)jobs.size(), mode,
+                                                         shape, stream, nullptr);
+            if (e != hipSuccess) {
                 (void)hipStreamSynchronize(stream);
-                fail_all_running(rc, ddfan_last_error());
+                fail_all_running(DDFAN_EDEVICE, std::string("batch iteration launch: ") +
+                                                    hipGetErrorString(e));
+                continue;
+            }
+        } else {
+            if (changed || cur < 0) {
+                int g = 0;  // a slot that no in-flight iteration uses
+                while (g == graphIdx[0] || g == graphIdx[1]) ++g;
+                jobs.clear();
+                for (Request* R : running) jobs.push_back(&R->job);
+                const int rc =
+                    graphs[g].capture(jobs.data(), (int)jobs.size(), mode, shape, stream);
+                if (rc != DDFAN_OK) {
+                    (void)hipStreamSynchronize(stream);
+                    fail_all_running(rc, ddfan_last_error());
+                    cur = -1;
+                    continue;
+                }
+                cur = g;
+            }
+            if (graphs[cur].launch(stream) != DDFAN_OK) {
+                (void)hipStreamSynchronize(stream);
+                fail_all_running(DDFAN_EDEVICE, ddfan_last_error());
                 cur = -1;
                 continue;
             }
-            cur = g;
-        }
-        if (graphs[cur].launch(stream) != DDFAN_OK) {
-            (void)hipStreamSynchronize(stream);
-            fail_all_running(DDFAN_EDEVICE, ddfan_last_error());
-            cur = -1;
-            continue;
         }
         (void)hipEventRecord(iterEv[slot], stream);
         inFlight[slot] = true;
@@ -447,14 +491,21 @@ void ddfan_server::loop() {
         // wait for the previous iteration, then complete finished requests
         const int prev = slot ^ 1;
         if (inFlight[prev]) {
-            (void)hipEventSynchronize(iterEv[prev]);
+            const hipError_t e = hooks.eventSync(iterEv[prev]);
             inFlight[prev] = false;
             graphIdx[prev] = -1;
+            if (e != hipSuccess) {
+                device_failed(e, "waiting for a batch iteration");
+                continue;
+            }
         }
         collect();
     }
-    (void)hipStreamSynchronize(stream);
-    collect();
+    const hipError_t e = hooks.streamSync(stream);
+    if (e != hipSuccess)
+        device_failed(e, "waiting for the last iterations");
+    else
+        collect();
     for (auto& g : graphs) g.reset();
     (void)hipEventDestroy(iterEv[0]);
     (void)hipEventDestroy(iterEv[1]);
@@ -480,6 +531,11 @@ void ddfan_server::reap() {
         adm.release(d.bytes);
         cvWork.notify_all();
     }
+}
+
+void ddf::set_server_hooks(ddfan_server* S, const ServerHooks& h) {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->hooks = h;
 }
 
 extern "C" {
@@ -530,6 +586,22 @@ int ddfan_server_create(size_t hbm_budget, int max_slots, int sat_mode, ddfan_se
         }
         *out = S;
         return DDFAN_OK;
+    });
+}
+
+int ddfan_server_set_param(ddfan_server* S, int param, long long value) {
+    return ddf::guarded([&]() -> int {
+        CHECK(S, DDFAN_EINVAL, "server is NULL");
+        std::lock_guard<std::mutex> lk(S->mu);
+        CHECK(S->nextId == 1, DDFAN_EINVAL, "server parameters are set before the first submission");
+        switch (param) {
+            case DDFAN_SERVER_GRAPH:
+                CHECK(value == 0 || value == 1, DDFAN_EINVAL, "DDFAN_SERVER_GRAPH takes 0 or 1");
+                S->graphs_ = value != 0;
+                return DDFAN_OK;
+            default:
+                return fail(DDFAN_EINVAL, "unknown server parameter %d", param);
+        }
     });
 }
 

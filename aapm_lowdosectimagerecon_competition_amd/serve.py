@@ -26,12 +26,17 @@ class Server:
     """Wraps ``ddfan_server``.  ``hbm_budget`` 0 takes 80% of the free device
     memory."""
 
-    def __init__(self, hbm_budget: int = 0, max_slots: int = 16, sat_mode="twofloat"):
+    def __init__(self, hbm_budget: int = 0, max_slots: int = 16, sat_mode="twofloat",
+                 graphs: bool = False):
+        """``graphs``: replay each request set's batch iteration as a captured
+        hipGraph instead of launching its kernels directly (same results)."""
         h = ctypes.c_void_p()
         N.call("ddfan_server_create", ctypes.c_size_t(int(hbm_budget)), int(max_slots),
                _sat_mode(sat_mode), ctypes.byref(h))
         self.handle = h.value
         self._shapes = {}
+        if graphs:
+            N.call("ddfan_server_set_param", self.handle, N.SERVER_GRAPH, 1)
 
     def submit(self, geom: DetectorGeometry, hangs, meas, num_iter, num_os, use_fista=1,
                mask=None, x0=None) -> int:

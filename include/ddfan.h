@@ -250,6 +250,12 @@ int ddfan_batch_size(const ddfan_batch* batch, int* n);
 #define DDFAN_BATCH_PROJ_FILL 1
 #define DDFAN_BATCH_BACK_FILL 2
 #define DDFAN_BATCH_SOLO_BYTES 3
+/* DDFAN_BATCH_STREAMS (0..8): the reconstructions are split into that many
+ * groups of about equal work, each group running its own batch iterations on
+ * its own stream, so the kernels of different groups share the GPU.  0
+ * (default): 4 groups when every reconstruction's volume tables are below
+ * DDFAN_BATCH_SOLO_BYTES, else 1. */
+#define DDFAN_BATCH_STREAMS 4
 int ddfan_batch_set_param(ddfan_batch* batch, int param, long long value);
 /* Runs every reconstruction to completion (flags: DDFAN_OSSART_GRAPH). */
 int ddfan_batch_run(ddfan_batch* batch, int flags);
@@ -276,6 +282,13 @@ int ddfan_server_submit(ddfan_server* server, const ddfan_geom* geom, const floa
                         const unsigned char* mask, const float* meas, const float* x0,
                         int numIter, int numOS, int useFISTA, long long* id);
 int ddfan_server_poll(ddfan_server* server, long long id, int* done);
+/* Server parameters, set before the first submission.  DDFAN_SERVER_GRAPH:
+ * 1 captures each batch iteration of a request set as a hipGraph and replays
+ * it while the set is unchanged; 0 (default) launches every iteration's
+ * kernels directly from the worker, which keeps at most two iterations in
+ * flight either way.  Results are the same bit for bit. */
+#define DDFAN_SERVER_GRAPH 1
+int ddfan_server_set_param(ddfan_server* server, int param, long long value);
 /* Blocks until the request is done, copies its reconstruction to x (may be
  * NULL) and returns the milliseconds from submission to admission and to
  * completion.  Called before the request completes, the loop copies the result

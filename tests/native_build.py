@@ -1,0 +1,23 @@
+"""Host programs of the tests that link libddfan.so, built in-tree (by
+__graft_entry__.build() and, when missing, by the test that runs them)."""
+import os
+import subprocess
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+OUT = os.path.join(ROOT, "tests", "native", "_build")
+PKG = os.path.join(ROOT, "aapm_lowdosectimagerecon_competition_amd")
+
+
+def serve_fault_test(rebuild=False) -> str:
+    """tests/native/serve_fault_test.cpp against libddfan.so and the HIP runtime."""
+    exe = os.path.join(OUT, "serve_fault_test")
+    if rebuild or not os.path.exists(exe):
+        os.makedirs(OUT, exist_ok=True)
+        subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-D__HIP_PLATFORM_AMD__",
+                        "-I/opt/rocm/include", "-I" + os.path.join(ROOT, "include"),
+                        "-I" + os.path.join(PKG, "csrc"),
+                        os.path.join(ROOT, "tests", "native", "serve_fault_test.cpp"),
+                        "-o", exe, "-L" + PKG, "-lddfan",
+                        "-Wl,-rpath,$ORIGIN/../../../aapm_lowdosectimagerecon_competition_amd",
+                        "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"], check=True)
+    return exe

@@ -45,10 +45,11 @@ def alone(dd, J, sat="twofloat"):
         return plan.ossart_host(J["meas"], J["x0"], J["it"], J["nos"], J["fista"], graph=True)
 
 
-def run_batch(dd, jobs, sat="twofloat", graph=True, solo_bytes=None, proj_waves=None):
+def run_batch(dd, jobs, sat="twofloat", graph=True, solo_bytes=None, proj_waves=None, streams=1):
     plans, bufs = [], []
     with dd.Batch(sat) as B:
         B.set_fill(proj_waves=proj_waves, solo_bytes=solo_bytes)
+        B.set_streams(streams)
         for J in jobs:
             p = dd.Plan(J["g"], J["hangs"], J["mask"], sat_mode=sat)
             db = dd.DeviceBuffer.from_array(J["meas"])
@@ -67,16 +68,22 @@ def run_batch(dd, jobs, sat="twofloat", graph=True, solo_bytes=None, proj_waves=
     return outs
 
 
-@pytest.mark.parametrize("solo_bytes,proj_waves", [(0, None), (1 << 20, None), (None, 512)],
-                         ids=["all-back-to-back", "large-back-to-back", "wave-limit"])
-def test_batch_grouping_never_changes_results(solo_bytes, proj_waves):
+@pytest.mark.parametrize("solo_bytes,proj_waves,streams",
+                         [(0, None, 1), (1 << 20, None, 1), (None, 512, 1), (None, None, 2),
+                          (None, None, 4), (1 << 20, None, 3), (None, None, 0)],
+                         ids=["all-back-to-back", "large-back-to-back", "wave-limit",
+                              "two-streams", "four-streams", "three-streams-large-back-to-back",
+                              "automatic-streams"])
+def test_batch_grouping_never_changes_results(solo_bytes, proj_waves, streams):
     """Every job back to back, only the one with large tables (the 512^2 job,
-    ~8 MB) back to back with the rest sharing launches, and shared launches
-    under a wave limit: each result equals the job run alone."""
+    ~8 MB) back to back with the rest sharing launches, shared launches under a
+    wave limit, and the jobs split into 2, 3, 4 or automatically chosen groups
+    on streams of their own: each result equals the job run alone."""
     import aapm_lowdosectimagerecon_competition_amd as dd
     jobs = [make_job(s, 40 + i) for i, s in enumerate(JOBS)]
     ref = [alone(dd, J) for J in jobs]
-    got = run_batch(dd, jobs, graph=True, solo_bytes=solo_bytes, proj_waves=proj_waves)
+    got = run_batch(dd, jobs, graph=True, solo_bytes=solo_bytes, proj_waves=proj_waves,
+                    streams=streams)
     nd = [int(np.count_nonzero(a != b)) for a, b in zip(got, ref)]
     print(f"\n[batch-group] solo_bytes {solo_bytes} proj_waves {proj_waves}: voxels differing "
           f"per job {nd}", flush=True)
