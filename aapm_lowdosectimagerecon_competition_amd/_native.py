@@ -97,6 +97,8 @@ _SIGS = {
     "ddfan_plan_proj": [_p, _p, _p, _i, _i, _i],
     "ddfan_plan_back": [_p, _p, _p, _i, _i, _i],
     "ddfan_plan_sync": [_p],
+    "ddfan_plan_vol_tables": [_p, _p, _p, _p, ctypes.POINTER(_sz), ctypes.POINTER(_sz)],
+    "ddfan_plan_sino_table": [_p, _p, _i, _p, ctypes.POINTER(_sz)],
     "ddfan_ossart": [_p, _p, _p, _i, _i, _i, _i],
     "ddfan_ossart_host_plain": [_f, _f, _i, _p, _p, _f, _f, _f, _p, _i, _i, _i, _i, _p, _p, _f,
                                 _p, _i, _i, _i, _i],

@@ -453,6 +453,50 @@ int ddfan_plan_proj(ddfan_plan* P, const float* d_vol, float* d_prj, int first, 
     });
 }
 
+int ddfan_plan_vol_tables(ddfan_plan* P, const float* d_vol, void* d_satX, void* d_satY,
+                          size_t* bytes_x, size_t* bytes_y) {
+    return ddf::guarded([&]() -> int {
+        CHECK(P && bytes_x && bytes_y, DDFAN_EINVAL, "NULL argument");
+        const size_t bx = sat_vol_bytes(P->mode, P->nzc, P->ZT, P->g.XN, P->g.YN) - 256;
+        const size_t by = sat_vol_bytes(P->mode, P->nzc, P->ZT, P->g.YN, P->g.XN) - 256;
+        *bytes_x = bx;
+        *bytes_y = by;
+        if (!d_satX && !d_satY) return DDFAN_OK;
+        CHECK(d_vol && d_satX && d_satY, DDFAN_EINVAL, "NULL buffer");
+        HIPCALL(hipSetDevice(P->device));
+        TRY(ensure_vol_sat(P));
+        HIPCALL(launch_sat_vol(P->mode, P->ZT, P->g, P->nzc, d_vol, P->mask.as<uint8_t>(),
+                               P->satX.p, P->satY.p, P->stream));
+        HIPCALL(hipMemcpyAsync(d_satX, P->satX.p, bx, hipMemcpyDeviceToDevice, P->stream));
+        HIPCALL(hipMemcpyAsync(d_satY, P->satY.p, by, hipMemcpyDeviceToDevice, P->stream));
+        HIPCALL(hipStreamSynchronize(P->stream));
+        return DDFAN_OK;
+    });
+}
+
+int ddfan_plan_sino_table(ddfan_plan* P, const float* d_prj, int count, void* d_out,
+                          size_t* bytes) {
+    return ddf::guarded([&]() -> int {
+        CHECK(P && bytes, DDFAN_EINVAL, "NULL argument");
+        CHECK(count >= 1 && count <= P->PN, DDFAN_EINVAL, "count must be in [1, %d]", P->PN);
+        const size_t b = sat_sino_bytes(P->mode, P->nzcb, P->ZTb, P->g.DNU, count) - 256;
+        *bytes = b;
+        if (!d_out) return DDFAN_OK;
+        CHECK(d_prj, DDFAN_EINVAL, "NULL buffer");
+        HIPCALL(hipSetDevice(P->device));
+        const size_t need = sat_sino_bytes(P->mode, P->nzcb, P->ZTb, P->g.DNU, count);
+        if (need > P->satP.bytes) {
+            HIPCALL(hipStreamSynchronize(P->stream));
+            HIPCALL(P->satP.reserve(need));
+        }
+        HIPCALL(launch_sat_sino(P->mode, P->ZTb, P->g.DNU, P->g.SLN, P->nzcb, count, d_prj,
+                                P->satP.p, P->stream));
+        HIPCALL(hipMemcpyAsync(d_out, P->satP.p, b, hipMemcpyDeviceToDevice, P->stream));
+        HIPCALL(hipStreamSynchronize(P->stream));
+        return DDFAN_OK;
+    });
+}
+
 int ddfan_plan_back(ddfan_plan* P, const float* d_prj, float* d_vol, int first, int stride,
                     int count) {
     return ddf::guarded([&]() -> int {

@@ -97,6 +97,47 @@ class Plan:
             dx.download(x)
         return x
 
+    # summed-area tables (tests, diagnostics)
+    _ENTRY = {0: (np.float32, 2), 1: (np.float32, 4), 2: (np.float64, 2)}
+
+    def vol_tables(self, vol):
+        """The projector's volume tables of `vol` (YN, XN, SLN) as arrays
+        satX[chunk, y, zt, k, e] (k = 0..XN) and satY[chunk, x, zt, k, e]
+        (k = 0..YN), e the entry's fields (f32: S, v; two-float: S_hi, S_lo,
+        v, 0; f64: S, v)."""
+        g = self.geom
+        bx, by = ctypes.c_size_t(), ctypes.c_size_t()
+        N.call("ddfan_plan_vol_tables", self.handle, None, None, None, ctypes.byref(bx),
+               ctypes.byref(by))
+        dt, ne = self._ENTRY[self.sat_mode]
+        zt = self.proj_chunk
+        nzc = (g.SLN + zt - 1) // zt
+        v = np.ascontiguousarray(vol, np.float32)
+        with N.DeviceBuffer.from_array(v) as dv, N.DeviceBuffer(bx.value) as dX, \
+                N.DeviceBuffer(by.value) as dY:
+            N.call("ddfan_plan_vol_tables", self.handle, N.ptr(dv), N.ptr(dX), N.ptr(dY),
+                   ctypes.byref(bx), ctypes.byref(by))
+            X = dX.to_numpy((nzc, g.YN, zt, g.XN + 1, ne), dt)
+            Y = dY.to_numpy((nzc, g.XN, zt, g.YN + 1, ne), dt)
+        return X, Y
+
+    def sino_table(self, prj):
+        """The backprojector's table of a sinogram prj (count, DNU, SLN) as an
+        array [chunk, view, zt, k, e] (k = 0..DNU)."""
+        g = self.geom
+        p = np.ascontiguousarray(prj, np.float32)
+        count = p.shape[0]
+        b = ctypes.c_size_t()
+        dt, ne = self._ENTRY[self.sat_mode]
+        zt = self.back_chunk
+        nzc = (g.SLN + zt - 1) // zt
+        with N.DeviceBuffer.from_array(p) as dp:
+            N.call("ddfan_plan_sino_table", self.handle, N.ptr(dp), count, None, ctypes.byref(b))
+            with N.DeviceBuffer(b.value) as dT:
+                N.call("ddfan_plan_sino_table", self.handle, N.ptr(dp), count, N.ptr(dT),
+                       ctypes.byref(b))
+                return dT.to_numpy((nzc, count, zt, g.DNU + 1, ne), dt)
+
     # tunables
     def set_param(self, param, value):
         N.call("ddfan_plan_set_param", self.handle, int(param), int(value))

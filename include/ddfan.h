@@ -220,6 +220,26 @@ int ddfan_plan_back(ddfan_plan* plan, const float* d_prj, float* d_vol, int firs
                     int count);
 int ddfan_plan_sync(ddfan_plan* plan);
 
+/* The summed-area tables the operators read (for tests and diagnostics; the
+ * reference builds them inside its calls, MultiSlicesProjGPU.cu:53-97 and
+ * MultiSlicesBackGPU.cu:34-66).  Entry k of a row holds the prefix S[k] (sum of
+ * samples 0..k-1) and the sample v[k] (v[N] = 0): float2 (S, v) for
+ * DDFAN_SAT_F32, floats (S_hi, S_lo, v, 0) with S = S_hi + S_lo for
+ * DDFAN_SAT_TWOFLOAT, double2 (S, v) for DDFAN_SAT_F64.
+ * ddfan_plan_vol_tables builds the projector's two volume tables from d_vol
+ * (masked by the plan's mask) and copies them to d_satX / d_satY:
+ * E[chunk][row][zt][k], slice z = chunk*ZT + zt (ZT = the projector's slice
+ * chunk, DDFAN_PARAM_PROJ_CHUNK; slices past SLN are zero), rows y with k
+ * along x (k = 0..XN) in d_satX, rows x with k along y (k = 0..YN) in d_satY.
+ * ddfan_plan_sino_table builds the backprojector's table of a sinogram of
+ * `count` views d_prj[(v*DNU + u)*SLN + z] into d_out: E[chunk][v][zt][k],
+ * k = 0..DNU, ZT = the backprojector's slice chunk (DDFAN_PARAM_BACK_CHUNK).
+ * With NULL output buffers they only report the bytes. */
+int ddfan_plan_vol_tables(ddfan_plan* plan, const float* d_vol, void* d_satX, void* d_satY,
+                          size_t* bytes_x, size_t* bytes_y);
+int ddfan_plan_sino_table(ddfan_plan* plan, const float* d_prj, int count, void* d_out,
+                          size_t* bytes);
+
 /* OS-SART(+FISTA) of OSSART_AAPM.m:49-117 on device-resident data:
  * d_meas[(v*DNU + u)*SLN + z] over the plan's PN views, d_x in/out.
  * flags: DDFAN_OSSART_GRAPH replays one captured iteration as a hipGraph. */

@@ -1,0 +1,119 @@
+"""The summed-area tables themselves (MultiSlicesProjGPU.cu:53-97 for the
+volume, MultiSlicesBackGPU.cu:34-66 for the sinogram), read back from the
+device for each precision.
+
+* Integer samples (0..15, sums below 2^24): every entry of 513-entry volume
+  rows (512x512 image) and 737-entry sinogram rows (736 channels) equals the
+  exact prefix sum, in all three precisions; the entry's sample field equals
+  the (masked) sample, v[N] = 0 and S[0] = 0; slices past SLN are zero.
+* Real samples: each precision's largest relative error against the FP64
+  cumulative sum of the same FP32 samples is printed and bounded (f32 by the
+  prefix length times the FP32 unit round-off, two-float and f64 at the FP64
+  level): the error the two-float and f64 tables remove from the reference's
+  FP32 sequential prefix (MultiSlicesProjGPU.cu:42-51).
+Both builders are covered: volume tables of 64 slices (32 slices per block),
+24 slices (16 per block) and 5 slices (one row per block), and sinogram
+tables (one row per block)."""
+import numpy as np
+import pytest
+
+from helpers import small_conf
+
+pytestmark = pytest.mark.gpu
+
+MODES = ("f32", "twofloat", "f64")
+
+
+def _prefix(entries, mode):
+    if mode == "twofloat":
+        return entries[..., 0].astype(np.float64) + entries[..., 1].astype(np.float64)
+    return entries[..., 0].astype(np.float64)
+
+
+def _sample(entries, mode):
+    return entries[..., 2 if mode == "twofloat" else 1]
+
+
+def _exact_vol(vol, mask):
+    m = vol.astype(np.float64) * mask[:, :, None]          # [y, x, z]
+    X = np.concatenate([np.zeros((m.shape[0], 1, m.shape[2])), np.cumsum(m, axis=1)], axis=1)
+    Y = np.concatenate([np.zeros((1, m.shape[1], m.shape[2])), np.cumsum(m, axis=0)], axis=0)
+    return X.transpose(2, 0, 1), Y.transpose(2, 1, 0), m   # [z, y, k], [z, x, k]
+
+
+def _unchunk(T, SLN):
+    """[chunk, row, zt, k, e] -> [z, row, k, e] for the first SLN slices,
+    and the padded slices."""
+    nzc, rows, zt = T.shape[:3]
+    A = T.transpose(0, 2, 1, 3, 4).reshape(nzc * zt, rows, T.shape[3], T.shape[4])
+    return A[:SLN], A[SLN:]
+
+
+@pytest.mark.parametrize("SLN", [64, 24, 5])
+def test_volume_tables_integer_exact_and_error(SLN):
+    import aapm_lowdosectimagerecon_competition_amd as dd
+    from helpers import per_slice_angles
+    XN = 512
+    conf = small_conf(XN=XN, YN=XN, SLN=SLN, dx=1.0, DNU=64, bin=8)
+    g = dd.detector_geometry(conf)
+    hangs = per_slice_angles(16, SLN) if SLN == 5 else np.zeros((16, SLN), np.float32)
+    mask = dd.recon_mask(XN, XN)
+    rng = np.random.default_rng(SLN)
+    ints = rng.integers(0, 16, (XN, XN, SLN)).astype(np.float32)
+    reals = rng.uniform(0.0, 1.0, (XN, XN, SLN)).astype(np.float32)
+    for mode in MODES:
+        with dd.Plan(g, hangs, mask, sat_mode=mode) as plan:
+            X, Y = plan.vol_tables(ints)
+            Xr, Yr = plan.vol_tables(reals)
+            zt = plan.proj_chunk
+        eX, eY, m = _exact_vol(ints, mask)
+        for T, ex, name in ((X, eX, "X"), (Y, eY, "Y")):
+            A, pad = _unchunk(T, SLN)
+            assert np.array_equal(_prefix(A, mode), ex), (mode, name)
+            assert np.all(pad == 0)
+        ax, _ = _unchunk(X, SLN)
+        assert np.array_equal(_sample(ax, mode)[:, :, :XN], m.transpose(2, 0, 1))
+        assert np.all(_sample(ax, mode)[:, :, XN] == 0)
+        rX, rY, _ = _exact_vol(reals, mask)
+        err = 0.0
+        for T, ex in ((Xr, rX), (Yr, rY)):
+            A, _ = _unchunk(T, SLN)
+            err = max(err, float(np.max(np.abs(_prefix(A, mode) - ex) /
+                                        np.maximum(np.abs(ex), 1.0))))
+        print(f"\n[sat] volume {XN}x{XN}x{SLN} (chunk {zt}), {mode}: max rel error of the prefix "
+              f"vs FP64 {err:.2e}", flush=True)
+        bound = {"f32": XN * 2.0 ** -24, "twofloat": 1e-11, "f64": 1e-13}[mode]
+        assert err <= bound
+
+
+@pytest.mark.parametrize("SLN", [64, 5])
+def test_sinogram_table_integer_exact_and_error(SLN):
+    import aapm_lowdosectimagerecon_competition_amd as dd
+    DNU, PN = 736, 12
+    conf = small_conf(XN=64, YN=64, SLN=SLN, dx=4.0, DNU=DNU, bin=1)
+    g = dd.detector_geometry(conf)
+    hangs = np.zeros((PN, SLN), np.float32)
+    rng = np.random.default_rng(7 + SLN)
+    ints = rng.integers(0, 16, (PN, DNU, SLN)).astype(np.float32)
+    reals = rng.uniform(0.0, 1.0, (PN, DNU, SLN)).astype(np.float32)
+    for mode in MODES:
+        with dd.Plan(g, hangs, None, sat_mode=mode) as plan:
+            T = plan.sino_table(ints)
+            Tr = plan.sino_table(reals)
+            zt = plan.back_chunk
+        for tab, src, exact in ((T, ints, True), (Tr, reals, False)):
+            ex = np.concatenate([np.zeros((PN, 1, SLN)), np.cumsum(src.astype(np.float64),
+                                                                  axis=1)], axis=1)
+            ex = ex.transpose(2, 0, 1)                      # [z, v, k]
+            A, pad = _unchunk(tab, SLN)
+            got = _prefix(A, mode)
+            if exact:
+                assert np.array_equal(got, ex), mode
+                assert np.array_equal(_sample(A, mode)[:, :, :DNU], src.transpose(2, 0, 1))
+                assert np.all(pad == 0)
+            else:
+                err = float(np.max(np.abs(got - ex) / np.maximum(np.abs(ex), 1.0)))
+                print(f"\n[sat] sinogram {PN}x{DNU}x{SLN} (chunk {zt}), {mode}: max rel error "
+                      f"of the prefix vs FP64 {err:.2e}", flush=True)
+                bound = {"f32": DNU * 2.0 ** -24, "twofloat": 1e-11, "f64": 1e-13}[mode]
+                assert err <= bound
