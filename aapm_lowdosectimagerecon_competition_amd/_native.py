@@ -43,6 +43,7 @@ PARAM_PROJ_CHUNK = 3
 PARAM_BACK_CHUNK = 4
 PARAM_BACK_STAGE = 5
 PARAM_PROJ_STAGE = 6
+PARAM_BACK_PASS = 7
 TUNE_USE_CACHE = 1
 BATCH_PROJ_FILL = 1
 BATCH_BACK_FILL = 2

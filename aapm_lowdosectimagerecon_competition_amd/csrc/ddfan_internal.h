@@ -90,6 +90,8 @@ struct BackJob {
     const float* momTab;
     const int* counter;
     int NV = 0;  // views per LDS-staged group (k_back_lds); 0 = per-lane gathers (k_back)
+    int tabViews = 0;  // views per slice chunk of P (0: sub.count)
+    int accum = 0;     // 1: continue the sums already in vol (a later view pass)
 };
 
 hipError_t launch_sat_vol_batch(int mode, const SatVolJob* jobs, int n, hipStream_t st);
@@ -113,10 +115,12 @@ hipError_t launch_proj(int mode, int ZT, int epi, const GeomDev& g, int nzc, con
                        const void* satX, const void* satY, Subset sub, float* out,
                        const float* meas, const float* rowSum, int slabTile, int viewsPerWave,
                        int stage, hipStream_t st);
+// viewPass > 0 (plain epilogue only): launches of at most viewPass views each,
+// every one continuing the sums of the previous (same results as one launch).
 hipError_t launch_back(int mode, int ZT, int epi, const GeomDev& g, int nzc, const float2* cossin,
                        const uint8_t* mask, const void* P, Subset sub, float* vol,
                        const float* colSum, float* xprev, const float* momTab,
-                       const int* iterCounter, int stageViews, hipStream_t st);
+                       const int* iterCounter, int stageViews, hipStream_t st, int viewPass = 0);
 
 hipError_t launch_counter_inc(int* counter, hipStream_t st);
 hipError_t launch_fill(float* p, float v, size_t n, hipStream_t st);

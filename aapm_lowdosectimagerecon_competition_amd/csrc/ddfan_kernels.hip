@@ -19,6 +19,8 @@
 // arithmetic runs once per chunk and lanes read consecutive k.  ZT = 1 is the
 // general per-slice-angle path.  Geometry that decides an integer goes through
 // ddfan_index.h and is bit-identical to the host restatement.
+#include <algorithm>
+
 #include "ddfan_index.h"
 #include "ddfan_internal.h"
 
@@ -611,6 +613,8 @@ struct BackDev {
     GeomDev g;
     Subset sub;
     int nzc, ntx, nty, blk0, epi;
+    int tabViews;  // views per slice chunk of the table P (>= sub.count)
+    int accum;     // 1: the sums continue the values in vol (a later view pass)
     const float2* cossin;
     const uint8_t* mask;
     const void* P;
@@ -679,11 +683,13 @@ void k_back(const BackLaunch L) {
     const int DNU = g.DNU;
     const int K = DNU + 1;
     const size_t viewStride = (size_t)ZT * K;
-    const E* Pc = reinterpret_cast<const E*>(J.P) + (size_t)zc * J.sub.count * viewStride;
+    const E* Pc = reinterpret_cast<const E*>(J.P) + (size_t)zc * J.tabViews * viewStride;
 
     float acc[ZT];
 #pragma unroll
-    for (int q = 0; q < ZT; ++q) acc[q] = 0.0f;
+    for (int q = 0; q < ZT; ++q)
+        acc[q] = (J.accum && inb && z0 + q < g.SLN) ? J.vol[((size_t)y * g.XN + x) * g.SLN + z0 + q]
+                                                    : 0.0f;
     if (m != 0) {
         const float vx = ((float)x - g.ocxB) * g.dx;
         const float vy = ((float)y - g.ocyB) * g.dx;
@@ -810,11 +816,13 @@ void k_back_lds(const BackLaunch L) {
     const int K = DNU + 1;
     const int count = J.sub.count;
     const size_t viewStride = (size_t)ZT * K;
-    const E* Pc = reinterpret_cast<const E*>(J.P) + (size_t)zc * count * viewStride;
+    const E* Pc = reinterpret_cast<const E*>(J.P) + (size_t)zc * J.tabViews * viewStride;
 
     float acc[ZT];
 #pragma unroll
-    for (int q = 0; q < ZT; ++q) acc[q] = 0.0f;
+    for (int q = 0; q < ZT; ++q)
+        acc[q] = (J.accum && inb && z0 + q < g.SLN) ? J.vol[((size_t)y * g.XN + x) * g.SLN + z0 + q]
+                                                    : 0.0f;
     // Any unmasked voxel in the tile?  (Reduced through the first staging
     // buffer, so the block's LDS is exactly the two buffers.)
     {
@@ -1640,6 +1648,8 @@ static hipError_t launch_back_group(int mode, int ZT, int NV, const BackJob* con
         d.nty = (J.g.YN + 15) / 16;
         d.blk0 = (int)nw;
         d.epi = J.epi;
+        d.tabViews = J.tabViews > 0 ? J.tabViews : J.sub.count;
+        d.accum = J.accum;
         d.cossin = J.cossin;
         d.mask = J.mask;
         d.P = J.P;
@@ -1695,10 +1705,27 @@ hipError_t launch_back_batch(int mode, const BackJob* jobs, int n, hipStream_t s
 hipError_t launch_back(int mode, int ZT, int epi, const GeomDev& g, int nzc, const float2* cossin,
                        const uint8_t* mask, const void* P, Subset sub, float* vol,
                        const float* colSum, float* xprev, const float* momTab,
-                       const int* iterCounter, int stageViews, hipStream_t st) {
-    const BackJob j{g,      sub,   nzc,    ZT,          epi,       cossin, mask, P,
-                    vol,    colSum, xprev, momTab,      iterCounter, stageViews};
-    return launch_back_batch(mode, &j, 1, st);
+                       const int* iterCounter, int stageViews, hipStream_t st, int viewPass) {
+    if (viewPass <= 0 || viewPass >= sub.count || epi != BACK_PLAIN) {
+        const BackJob j{g,      sub,   nzc,    ZT,          epi,       cossin, mask, P,
+                        vol,    colSum, xprev, momTab,      iterCounter, stageViews};
+        return launch_back_batch(mode, &j, 1, st);
+    }
+    // View passes: pass p sums views [p*viewPass, (p+1)*viewPass) of the
+    // subset on top of the sums the previous passes left in vol (the same
+    // additions in the same order as one pass, so the same results).
+    const size_t eb = mode == SAT_F32 ? 8 : 16;
+    for (int j0 = 0; j0 < sub.count; j0 += viewPass) {
+        const Subset sp{sub.first + j0 * sub.stride, sub.stride, std::min(viewPass, sub.count - j0)};
+        const void* Pp = static_cast<const char*>(P) + (size_t)j0 * ZT * (g.DNU + 1) * eb;
+        BackJob j{g,   sp,     nzc,   ZT,     epi,         cossin,     mask, Pp,
+                  vol, colSum, xprev, momTab, iterCounter, stageViews};
+        j.tabViews = sub.count;
+        j.accum = j0 > 0;
+        const hipError_t e = launch_back_batch(mode, &j, 1, st);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_counter_inc_batch(int* const* counters, int n, hipStream_t st) {

@@ -106,6 +106,7 @@ struct ddfan_plan {
     int viewsPerWave = 4;  // projector views per wave (1, 2 or 4)
     int backStage = 0;     // backprojector views per LDS-staged group (0: per-lane gathers)
     int projStage = 0;     // 1: projector reads LDS-staged table rows (0: per-lane gathers)
+    int backPass = 0;      // backprojector views per pass of a plain call (0: one pass)
     std::vector<float> h_hangs, h_cs;
     ddf::DBuf cs, bx, by, xds, yds, mask;
     ddf::DBuf satX, satY, satP;

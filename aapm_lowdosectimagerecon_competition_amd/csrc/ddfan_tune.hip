@@ -38,6 +38,7 @@ struct TuneKey {
 struct TuneVal {
     int ZT, V, tile, ZTb, stage, pstage;
     double projMs, backMs;
+    int bpass = -1;  // backprojector views per pass (-1: not recorded, left as it is)
 };
 
 bool stage_fits(int stage, int ZTb) {
@@ -58,6 +59,7 @@ int apply(ddfan_plan* P, const TuneVal& v) {
     TRY(ddfan_plan_set_param(P, DDFAN_PARAM_BACK_CHUNK, v.ZTb));
     TRY(ddfan_plan_set_param(P, DDFAN_PARAM_BACK_STAGE, v.stage));
     TRY(ddfan_plan_set_param(P, DDFAN_PARAM_PROJ_STAGE, v.pstage));
+    if (v.bpass >= 0) TRY(ddfan_plan_set_param(P, DDFAN_PARAM_BACK_PASS, v.bpass));
     return DDFAN_OK;
 }
 
@@ -125,7 +127,7 @@ int ddfan_plan_autotune(ddfan_plan* P, int views, int reps, int flags, int* out_
             HIPCALL(launch_fill(prj.as<float>(), 1.0f, nprj, P->stream));
             // the plan's own settings are restored if tuning fails part-way
             const TuneVal saved{P->ZT,        P->viewsPerWave, P->slabTile, P->ZTb,
-                                P->backStage, P->projStage,    0.0,         0.0};
+                                P->backStage, P->projStage,    0.0,         0.0, P->backPass};
             int rc = [&]() -> int {
                 std::vector<int> chunks;
                 for (int z = std::min(P->ZTmax, 16); z >= 1; z >>= 1) chunks.push_back(z);
@@ -190,6 +192,22 @@ int ddfan_plan_autotune(ddfan_plan* P, int views, int reps, int flags, int* out_
                         }
                     }
                 }
+                TRY(ddfan_plan_set_param(P, DDFAN_PARAM_BACK_CHUNK, best.ZTb));
+                TRY(ddfan_plan_set_param(P, DDFAN_PARAM_BACK_STAGE, best.stage));
+                best.bpass = P->backPass;
+                // view passes of a long plain backprojection
+                if (views > 96) {
+                    for (int bp : {0, 288, 192, 144, 96}) {
+                        if (bp >= views) continue;
+                        TRY(ddfan_plan_set_param(P, DDFAN_PARAM_BACK_PASS, bp));
+                        double ms;
+                        TRY(time_ms(P->stream, reps, back, &ms));
+                        if (ms < bestB) {
+                            bestB = ms;
+                            best.bpass = bp;
+                        }
+                    }
+                }
                 best.backMs = bestB;
                 return DDFAN_OK;
             }();
@@ -235,22 +253,23 @@ int ddfan_plan_apply_tuned(ddfan_plan* P, int views, int* found) {
 }
 
 // One line per entry (bstage: backprojector staged views, pstage: projector
-// staging):
-// mode XN YN SLN DNU ZTmax views  ZT V tile ZTb bstage pstage  proj_ms back_ms
+// staging, bpass: backprojector views per pass; tables written before bpass
+// existed have 15 fields and leave it as it is):
+// mode XN YN SLN DNU ZTmax views  ZT V tile ZTb bstage pstage  proj_ms back_ms  bpass
 int ddfan_tune_save(const char* path) {
     return ddf::guarded([&]() -> int {
         CHECK(path, DDFAN_EINVAL, "path is NULL");
         FILE* f = fopen(path, "w");
         CHECK(f, DDFAN_EINVAL, "cannot write %s", path);
         fprintf(f, "# ddfan launch-shape table: mode XN YN SLN DNU ZTmax views  ZT V tile ZTb bstage "
-                   "pstage  proj_ms back_ms\n");
+                   "pstage  proj_ms back_ms  bpass\n");
         std::lock_guard<std::mutex> lk(g_tuneMu);
         for (const auto& kv : g_tune) {
             const TuneKey& k = kv.first;
             const TuneVal& v = kv.second;
-            fprintf(f, "%d %d %d %d %d %d %d  %d %d %d %d %d %d  %.4f %.4f\n", k.mode, k.XN, k.YN,
-                    k.SLN, k.DNU, k.ZTmax, k.views, v.ZT, v.V, v.tile, v.ZTb, v.stage, v.pstage,
-                    v.projMs, v.backMs);
+            fprintf(f, "%d %d %d %d %d %d %d  %d %d %d %d %d %d  %.4f %.4f  %d\n", k.mode, k.XN,
+                    k.YN, k.SLN, k.DNU, k.ZTmax, k.views, v.ZT, v.V, v.tile, v.ZTb, v.stage,
+                    v.pstage, v.projMs, v.backMs, v.bpass);
         }
         const int bad = ferror(f);
         fclose(f);
@@ -279,10 +298,12 @@ int ddfan_tune_load(const char* path, int* entries) {
                 ++tokens;
                 while (*c && *c != ' ' && *c != '\t' && *c != '\n' && *c != '\r') ++c;
             }
-            if (tokens != 15 ||
-                sscanf(line, "%d %d %d %d %d %d %d %d %d %d %d %d %d %lf %lf", &k.mode, &k.XN, &k.YN,
-                       &k.SLN, &k.DNU, &k.ZTmax, &k.views, &v.ZT, &v.V, &v.tile, &v.ZTb, &v.stage,
-                       &v.pstage, &v.projMs, &v.backMs) != 15) {
+            v.bpass = -1;
+            if ((tokens != 15 && tokens != 16) ||
+                sscanf(line, "%d %d %d %d %d %d %d %d %d %d %d %d %d %lf %lf %d", &k.mode, &k.XN,
+                       &k.YN, &k.SLN, &k.DNU, &k.ZTmax, &k.views, &v.ZT, &v.V, &v.tile, &v.ZTb,
+                       &v.stage, &v.pstage, &v.projMs, &v.backMs, &v.bpass) != tokens ||
+                v.bpass < -1) {
                 bad = lineNo;
                 break;
             }

@@ -197,6 +197,15 @@ class Plan:
     def back_stage(self, v):
         self.set_param(N.PARAM_BACK_STAGE, v)
 
+    @property
+    def back_pass(self):
+        """Backprojector views per launch of a plain call (0: one launch)."""
+        return self.get_param(N.PARAM_BACK_PASS)
+
+    @back_pass.setter
+    def back_pass(self, v):
+        self.set_param(N.PARAM_BACK_PASS, v)
+
     def autotune(self, views=None, reps=3, use_cache=True) -> dict:
         """Times launch shapes for calls of `views` views (default PN), sets
         the fastest and records it (see tune_save / tune_load)."""

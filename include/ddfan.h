@@ -179,6 +179,10 @@ int ddfan_plan_query(const ddfan_plan* plan, int* slice_chunk, int* n_chunks, in
  * into LDS for four view groups at a time; 0 = every lane reads the table in
  * global memory.  Results are bit-identical. */
 #define DDFAN_PARAM_PROJ_STAGE 6
+/* DDFAN_PARAM_BACK_PASS: a plain backprojection of more views than this runs
+ * as launches of at most this many views, each continuing the sums the
+ * previous one stored (0 = one launch).  Results are bit-identical. */
+#define DDFAN_PARAM_BACK_PASS 7
 int ddfan_plan_set_param(ddfan_plan* plan, int param, int value);
 int ddfan_plan_get_param(const ddfan_plan* plan, int param, int* value);
 /* Autotuner: times the projector over slice chunks x views per wave x LDS
