@@ -125,6 +125,7 @@ struct Request {
     DBuf d_meas, d_x;
     OsJob job;
     int left = 0;                    // iterations not yet enqueued
+    int steps = 1;                   // iterations of the current batch step
     hipEvent_t uploaded = nullptr;   // recorded on the copy stream after each input upload
     hipEvent_t lastIter = nullptr;   // recorded after its last iteration
     // outcome
@@ -192,6 +193,14 @@ struct ddfan_server {
 };
 
 namespace {
+
+// Iterations of one request per batch step of the request loop, at most.
+constexpr int kMaxSteps = 8;
+
+// Work of one OS-SART iteration of a request: voxel-views per operator.
+double request_work(const Request& R) {
+    return (double)R.G.XN * R.G.YN * R.G.SLN * (double)R.PN;
+}
 
 // Device bytes one request holds while admitted: its plan with OS-SART state,
 // the measured sinogram and the iterate.
@@ -424,12 +433,28 @@ void ddfan_server::loop() {
             }
         }
         if (!useGraphs) {
-            // Eager launches of the same kernels: no stream capture is ever in
+            // Direct launches of the same kernels: no stream capture is ever in
             // progress on the worker while other threads allocate, copy or free.
-            jobs.clear();
-            for (Request* R : running) jobs.push_back(&R->job);
-            const hipError_t e = enqueue_batch_iteration(jobs.data(), (int)jobs.size(), mode,
-                                                         shape, stream, nullptr);
+            // One batch step runs `steps` iterations of each request: the
+            // largest running request one, a smaller one about as many as fit
+            // in the same work (at most kMaxSteps and what it has left), each
+            // step's iterations as batch iterations of the requests still due.
+            double wmax = 0.0;
+            for (Request* R : running) wmax = std::max(wmax, request_work(*R));
+            int smax = 1;
+            for (Request* R : running) {
+                const double r = wmax / std::max(request_work(*R), 1.0);
+                R->steps = std::min(R->left, (int)std::min<double>(kMaxSteps, std::max(1.0, r)));
+                smax = std::max(smax, R->steps);
+            }
+            hipError_t e = hipSuccess;
+            for (int k = 0; k < smax && e == hipSuccess; ++k) {
+                jobs.clear();
+                for (Request* R : running)
+                    if (R->steps > k) jobs.push_back(&R->job);
+                e = enqueue_batch_iteration(jobs.data(), (int)jobs.size(), mode, shape, stream,
+                                            nullptr);
+            }
             if (e != hipSuccess) {
                 (void)hipStreamSynchronize(stream);
                 fail_all_running(DDFAN_EDEVICE, std::string("batch iteration launch: ") +
@@ -437,6 +462,7 @@ void ddfan_server::loop() {
                 continue;
             }
         } else {
+            for (Request* R : running) R->steps = 1;
             if (changed || cur < 0) {
                 int g = 0;  // a slot that no in-flight iteration uses
                 while (g == graphIdx[0] || g == graphIdx[1]) ++g;
@@ -465,14 +491,12 @@ void ddfan_server::loop() {
         ++iterNo;
         {
             std::lock_guard<std::mutex> lk(mu);
-            for (Request* R : running) {
-                const GeomDev& g = R->plan->g;
-                busyVoxelViews += 2.0 * g.XN * g.YN * g.SLN * (double)R->PN;
-            }
+            for (Request* R : running) busyVoxelViews += 2.0 * request_work(*R) * R->steps;
         }
         for (size_t i = 0; i < running.size();) {
             Request* R = running[i];
-            if (--R->left == 0) {
+            R->left -= R->steps;
+            if (R->left == 0) {
                 (void)hipEventRecord(R->lastIter, stream);
                 finishing.push_back(R);
                 running.erase(running.begin() + i);

@@ -288,6 +288,7 @@ def bench_batch(dd, sat, n=16, iters=10, seed=3, specs=None, streams=(0, 1, 2, 4
            "sizes": [f"{sp['XN']}x{sp['XN']}x{sp['SLN']}x{sp['PN']}" for sp in specs],
            "sequential_s": sum(alone_s), "batch_s": runs[0],
            "batch_s_by_streams": {("auto" if k == 0 else str(k)): v for k, v in runs.items()},
+           "alone_s": alone_s,
            "speedup_vs_sequential": sum(alone_s) / runs[0],
            "voxel_views_per_s": vv / runs[0], "bitwise_equal_to_alone": bool(same),
            "noise": "normal approximation of Poisson counts, I0 log-uniform in [5e4, 5e5]"}

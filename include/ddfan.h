@@ -294,7 +294,11 @@ int ddfan_batch_destroy(ddfan_batch* batch);
  * rejected at submission (DDFAN_EBUSY).  Admitted requests keep their
  * sinogram and iterate in device memory and run together as a batch
  * (ddfan_batch_run's iteration), joining and leaving at iteration boundaries
- * while others run.  The reference has no serving layer. */
+ * while others run; in each step of the batch the largest running request
+ * runs one iteration and a smaller one up to 8, about as many as fit in the
+ * same work, so small requests do not wait for a large one's iterations.
+ * Every request's result is that of running it alone.  The reference has no
+ * serving layer. */
 typedef struct ddfan_server ddfan_server;
 int ddfan_server_create(size_t hbm_budget, int max_slots, int sat_mode, ddfan_server** out);
 /* Copies the request's host inputs (meas [PN][DNU][SLN], x0 [YN][XN][SLN] or
@@ -308,9 +312,11 @@ int ddfan_server_submit(ddfan_server* server, const ddfan_geom* geom, const floa
 int ddfan_server_poll(ddfan_server* server, long long id, int* done);
 /* Server parameters, set before the first submission.  DDFAN_SERVER_GRAPH:
  * 1 captures each batch iteration of a request set as a hipGraph and replays
- * it while the set is unchanged; 0 (default) launches every iteration's
- * kernels directly from the worker, which keeps at most two iterations in
- * flight either way.  Results are the same bit for bit. */
+ * it while the set is unchanged (every request then runs one iteration per
+ * step); 0 (default) launches every step's kernels directly from the worker,
+ * with the smaller requests running several iterations per step.  The worker
+ * keeps at most two steps in flight either way.  Results are the same bit
+ * for bit. */
 #define DDFAN_SERVER_GRAPH 1
 int ddfan_server_set_param(ddfan_server* server, int param, long long value);
 /* Blocks until the request is done, copies its reconstruction to x (may be
