@@ -56,10 +56,15 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def traffic_from_profiles(kernel: str):
+# Table precision the committed counter summaries were recorded with; runs with
+# other tables report no counter-derived figures.
+PROFILED_SAT = "twofloat"
+
+
+def traffic_from_profiles(kernel: str, sat: str = PROFILED_SAT):
     """HBM bytes per launch measured by rocprofv3 --pmc (committed summary)."""
     path = os.path.join(ROOT, "profiles", "traffic.txt")
-    if not os.path.exists(path):
+    if sat != PROFILED_SAT or not os.path.exists(path):
         return None
     for line in open(path):
         parts = line.split()
@@ -71,10 +76,10 @@ def traffic_from_profiles(kernel: str):
     return None
 
 
-def valu_from_profiles(kernel: str):
+def valu_from_profiles(kernel: str, sat: str = PROFILED_SAT):
     """Wave-level VALU instructions per launch measured by rocprofv3 --pmc
     SQ_INSTS_VALU (committed summary, "valu key count" lines)."""
-    path = os.path.join(ROOT, "profiles", "counters_512x512x64_2304v_twofloat.txt")
+    path = os.path.join(ROOT, "profiles", f"counters_512x512x64_2304v_{sat}.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -616,8 +621,8 @@ def main():
     per_kernel = {}
     for k, ms in kms.items():
         sec = ms * 1e-3
-        wi = valu_from_profiles(k)
-        tb = traffic_from_profiles(k)
+        wi = valu_from_profiles(k, args.sat)
+        tb = traffic_from_profiles(k, args.sat)
         per_kernel[k] = {
             "ms": ms,
             "valu_lane_ops_per_s": None if wi is None else wi * 64 / sec,
@@ -634,18 +639,20 @@ def main():
     roof = {"bound": "valu",
             "achieved": None if d["valu_lane_ops_per_s"] is None else d["valu_lane_ops_per_s"] / 1e12,
             "peak": VALU_PEAK_LANE_OPS / 1e12, "unit": "T lane-instructions/s (FP32 VALU issue)",
-            "frac": d["valu_frac"], "traffic": traffic_from_profiles(dom), "kernel": dom,
+            "frac": d["valu_frac"], "traffic": traffic_from_profiles(dom, args.sat), "kernel": dom,
             "hbm_GBs": d["hbm_GBs"], "hbm_frac": d["hbm_frac"],
             "model_GBs": d["model_GBs"], "model_frac": d["model_frac"],
             "model_bytes_per_launch": model_bytes[dom],
-            "step_hbm_GBs": (None if any(traffic_from_profiles(k) is None for k in
+            "step_hbm_GBs": (None if any(traffic_from_profiles(k, args.sat) is None for k in
                                          ("ddfan_proj", "ddfan_back", "ddfan_sat_sino",
                                           "ddfan_sat_vol"))
-                             else sum(traffic_from_profiles(k) for k in
+                             else sum(traffic_from_profiles(k, args.sat) for k in
                                       ("ddfan_proj", "ddfan_back", "ddfan_sat_sino",
                                        "ddfan_sat_vol")) / (ms_step * 1e-3) / 1e9),
             "per_kernel": per_kernel,
-            "counters": "profiles/counters_512x512x64_2304v_twofloat.txt, profiles/traffic.txt"}
+            "counters": (f"profiles/counters_512x512x64_2304v_{PROFILED_SAT}.txt, "
+                         "profiles/traffic.txt" if args.sat == PROFILED_SAT else
+                         f"none: the committed counters are of {PROFILED_SAT} tables")}
 
     out = {"metric": METRIC, "value": 2.0 * vv / (ms_step * 1e-3), "unit": "voxel-views/s",
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
