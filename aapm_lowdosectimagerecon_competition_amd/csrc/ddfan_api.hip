@@ -118,6 +118,16 @@ int validate_geom(const ddfan_geom* G) {
     return DDFAN_OK;
 }
 
+int check_proj_shape(const ddfan_plan* P) {
+    CHECK(proj_shape_ok(P->mode, P->g, P->ZT, P->viewsPerWave, P->projStage), DDFAN_EINVAL,
+          "projector launch shape (chunk %d, %d per wave, stage %d) does not fit this plan: "
+          "stages 0 and 1 take 1, 2 or 4 views per wave; the whole-row projector (stage 2: 4 "
+          "slices per block, 3: 2) takes 5, 6, 9 or 12 tasks per wave, a chunk that is a multiple "
+          "of its slices and images up to 775 voxels wide",
+          P->ZT, P->viewsPerWave, P->projStage);
+    return DDFAN_OK;
+}
+
 
 }  // namespace ddf
 
@@ -369,8 +379,11 @@ int ddfan_plan_set_param(ddfan_plan* P, int param, int value) {
                 P->slabTile = value;
                 return DDFAN_OK;
             case DDFAN_PARAM_VIEWS_PER_WAVE:
-                CHECK(value == 1 || value == 2 || value == 4, DDFAN_EINVAL,
-                      "views per wave must be 1, 2 or 4");
+                CHECK(value == 1 || value == 2 || value == 4 || value == 5 || value == 6 ||
+                          value == 9 || value == 12,
+                      DDFAN_EINVAL,
+                      "views per wave must be 1, 2 or 4 (tasks per wave 5, 6, 9 or 12 for the "
+                      "whole-row projector)");
                 P->viewsPerWave = value;
                 return DDFAN_OK;
             case DDFAN_PARAM_PROJ_CHUNK:
@@ -396,7 +409,7 @@ int ddfan_plan_set_param(ddfan_plan* P, int param, int value) {
                 P->backPass = value;
                 return DDFAN_OK;
             case DDFAN_PARAM_PROJ_STAGE:
-                CHECK(value == 0 || value == 1, DDFAN_EINVAL, "projector stage must be 0 or 1");
+                CHECK(value >= 0 && value <= 3, DDFAN_EINVAL, "projector stage must be 0, 1, 2 or 3");
                 HIPCALL(hipSetDevice(P->device));
                 HIPCALL(hipStreamSynchronize(P->stream));
                 P->projStage = value;
@@ -445,6 +458,7 @@ int ddfan_plan_proj(ddfan_plan* P, const float* d_vol, float* d_prj, int first, 
         CHECK(P && d_vol && d_prj, DDFAN_EINVAL, "NULL argument");
         TRY(check_subset(P->PN, first, stride, count));
         if (count == 0) return DDFAN_OK;
+        TRY(check_proj_shape(P));
         HIPCALL(hipSetDevice(P->device));
         TRY(ensure_vol_sat(P));
         const Subset sub{first, stride, count};
@@ -456,7 +470,8 @@ int ddfan_plan_proj(ddfan_plan* P, const float* d_vol, float* d_prj, int first, 
             return launch_proj(P->mode, P->ZT, PROJ_PLAIN, P->g, P->nzc, P->cs.as<float2>(),
                                P->bx.as<float>(), P->by.as<float>(), P->xds.as<float>(),
                                P->yds.as<float>(), P->satX.p, P->satY.p, sub, d_prj, nullptr, nullptr,
-                               P->slabTile, P->viewsPerWave, P->projStage, P->stream);
+                               P->slabTile, P->viewsPerWave, P->projStage, P->stream,
+                               P->h_cs.data());
         }));
         return DDFAN_OK;
     });

@@ -64,7 +64,7 @@ struct ProjJob {
     GeomDev g;
     Subset sub;
     int nzc, ZT, slabTile, epi;
-    int V;  // views per wave: 1, 2 or 4
+    int V;  // views per wave: 1, 2 or 4 (stages 0, 1); tasks per wave 5, 6, 9 or 12 (stages 2, 3)
     const float2* cossin;
     const float* bx;
     const float* by;
@@ -75,7 +75,12 @@ struct ProjJob {
     float* out;
     const float* meas;
     const float* rowSum;
-    int stage = 0;  // 1: LDS-staged table rows (k_proj_lds); 0: per-lane gathers (k_proj)
+    int stage = 0;  // 0: per-lane gathers (k_proj); 1: LDS-staged runs (k_proj_lds);
+                    // 2, 3: whole rows in LDS, 4 or 2 slices per block (k_proj_rows)
+    // Host copy of the (cos, sin) table, [view][slice] pairs: sizes the
+    // whole-row kernel's grid by the subset's runs of one slab orientation
+    // (without it, one block per view).
+    const float* hcs = nullptr;
 };
 struct BackJob {
     GeomDev g;
@@ -114,7 +119,7 @@ hipError_t launch_proj(int mode, int ZT, int epi, const GeomDev& g, int nzc, con
                        const float* bx, const float* by, const float* xds, const float* yds,
                        const void* satX, const void* satY, Subset sub, float* out,
                        const float* meas, const float* rowSum, int slabTile, int viewsPerWave,
-                       int stage, hipStream_t st);
+                       int stage, hipStream_t st, const float* hcs = nullptr);
 // viewPass > 0 (plain epilogue only): launches of at most viewPass views each,
 // every one continuing the sums of the previous (same results as one launch).
 hipError_t launch_back(int mode, int ZT, int epi, const GeomDev& g, int nzc, const float2* cossin,
@@ -123,6 +128,11 @@ hipError_t launch_back(int mode, int ZT, int epi, const GeomDev& g, int nzc, con
                        const int* iterCounter, int stageViews, hipStream_t st, int viewPass = 0);
 
 hipError_t launch_counter_inc(int* counter, hipStream_t st);
+// The projector launch shape fits the geometry: stages 0 and 1 take 1, 2 or 4
+// views per wave; the whole-row kernel (stage 2: 4 slices per block, 3: 2)
+// takes 5, 6, 9 or 12 tasks per wave, a slice chunk that is a multiple of its
+// slices and rows of at most 776 entries.
+bool proj_shape_ok(int mode, const GeomDev& g, int ZT, int V, int stage);
 hipError_t launch_fill(float* p, float v, size_t n, hipStream_t st);
 
 // Index traces for the bit-exactness tests (same device functions as the

@@ -398,6 +398,7 @@ struct ProjDev {
     GeomDev g;
     Subset sub;
     int nzc, nug, nvg, blk0, ii0, ii1, first, last, epi;
+    int zt, vb;  // k_proj_rows: the tables' slice chunk and the views of a block
     const float2* cossin;
     const float* bx;
     const float* by;
@@ -429,37 +430,57 @@ struct ProjLaunch {
 #endif
 
 
-// Writes one detector's ZT sums of the V views of view group jg: plain (the
+// Writes one detector's NQ sums (slices z0..) of subset view j: plain (the
 // operator; later slab passes add to the earlier ones) or, on the last pass,
 // the OS-SART ratio (b - p) / rowSum (OSSART_AAPM.m:81-88).
+template <int NQ>
+__device__ __forceinline__ void proj_store_view(const ProjDev& J, const GeomDev& g,
+                                                const Subset& sub, int j, int u, int z0,
+                                                const float (&acc)[NQ], float cval) {
+    const bool ratio = J.epi == PROJ_RATIO && J.last;
+    float* __restrict__ out = J.out;
+    const int vg = sub.first + j * sub.stride;
+    const size_t o0 = ((size_t)j * g.DNU + u) * g.SLN + z0;
+    const float w = ratio ? J.rowSum[(size_t)j * g.DNU + u] : 0.0f;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int z = z0 + q;
+        if (z < g.SLN) {
+            float p = acc[q] * cval;
+            if (!J.first) p = p + out[o0 + q];
+            if (!ratio) {
+                out[o0 + q] = p;
+            } else {
+                const float b = J.meas[((size_t)vg * g.DNU + u) * g.SLN + z];
+                out[o0 + q] = (w != 0.0f) ? (b - p) / w : p;
+            }
+        }
+    }
+}
+
+// The ZT sums of the V views of view group jg.
 template <int ZT, int V>
 __device__ __forceinline__ void proj_store(const ProjDev& J, const GeomDev& g, const Subset& sub,
                                            int jg, int u, int z0, const float (&acc)[V][ZT],
                                            const float (&cval)[V]) {
-    const bool ratio = J.epi == PROJ_RATIO && J.last;
-    float* __restrict__ out = J.out;
 #pragma unroll
     for (int v = 0; v < V; ++v) {
         const int j = jg * V + v;
         if (j >= sub.count) break;
-        const int vg = sub.first + j * sub.stride;
-        const size_t o0 = ((size_t)j * g.DNU + u) * g.SLN + z0;
-        const float w = ratio ? J.rowSum[(size_t)j * g.DNU + u] : 0.0f;
-#pragma unroll
-        for (int q = 0; q < ZT; ++q) {
-            const int z = z0 + q;
-            if (z < g.SLN) {
-                float p = acc[v][q] * cval[v];
-                if (!J.first) p = p + out[o0 + q];
-                if (!ratio) {
-                    out[o0 + q] = p;
-                } else {
-                    const float b = J.meas[((size_t)vg * g.DNU + u) * g.SLN + z];
-                    out[o0 + q] = (w != 0.0f) ? (b - p) / w : p;
-                }
-            }
-        }
+        proj_store_view<ZT>(J, g, sub, j, u, z0, acc[v], cval[v]);
     }
+}
+
+// Continuous factor of detector u in a view with angle (c, s): dx / the
+// |normalize(det - src)| component along the slab normal
+// (MultiSlicesProjGPU.cu:139,158,181).
+__device__ __forceinline__ float proj_cval(const GeomDev& g, float c, float s, bool xslab,
+                                           float xdu, float ydu) {
+    const float srcx = g.x0 * c - g.y0 * s, srcy = g.x0 * s + g.y0 * c;
+    const float ddx = (xdu * c - ydu * s) - srcx;
+    const float ddy = (xdu * s + ydu * c) - srcy;
+    const float nrm = sqrtf(fmaf(ddx, ddx, ddy * ddy));
+    return g.dx * nrm / fabsf(xslab ? ddx : ddy);
 }
 
 template <int MODE, int ZT, int V>
@@ -502,12 +523,7 @@ void k_proj(const ProjLaunch L) {
         const int vg = sub.first + j * sub.stride;
         const float2 cs = J.cossin[(size_t)vg * g.SLN + min(z0, g.SLN - 1)];
         const ProjBoundary pb = proj_boundary(g.x0, g.y0, cs.x, cs.y, bxb, byb);
-        // continuous: dx / |normalize(det - src)| component (MultiSlicesProjGPU.cu:139,158,181)
-        const float srcx = g.x0 * cs.x - g.y0 * cs.y, srcy = g.x0 * cs.y + g.y0 * cs.x;
-        const float ddx = (xdu * cs.x - ydu * cs.y) - srcx;
-        const float ddy = (xdu * cs.y + ydu * cs.x) - srcy;
-        const float nrm = sqrtf(fmaf(ddx, ddx, ddy * ddy));
-        cval[v] = g.dx * nrm / fabsf(pb.xslab ? ddx : ddy);
+        cval[v] = proj_cval(g, cs.x, cs.y, pb.xslab, xdu, ydu);
         pa[v] = pb.a;
         pf[v] = pb.f;
         pbb[v] = pb.b;
@@ -1013,11 +1029,7 @@ void k_proj_lds(const ProjLaunch L) {
         const int vg = sub.first + j * sub.stride;
         const float2 cs = J.cossin[(size_t)vg * g.SLN + zs];
         const ProjBoundary pb = proj_boundary(g.x0, g.y0, cs.x, cs.y, bxb, byb);
-        const float srcx = g.x0 * cs.x - g.y0 * cs.y, srcy = g.x0 * cs.y + g.y0 * cs.x;
-        const float ddx = (xdu * cs.x - ydu * cs.y) - srcx;
-        const float ddy = (xdu * cs.y + ydu * cs.x) - srcy;
-        const float nrm = sqrtf(fmaf(ddx, ddx, ddy * ddy));
-        cval[v] = g.dx * nrm / fabsf(pb.xslab ? ddx : ddy);
+        cval[v] = proj_cval(g, cs.x, cs.y, pb.xslab, xdu, ydu);
         pa[v] = pb.a;
         pf[v] = pb.f;
         pbb[v] = pb.b;
@@ -1126,6 +1138,199 @@ void k_proj_lds(const ProjLaunch L) {
     }
     if (!act) return;
     proj_store<ZT, V>(J, g, sub, jg, u, z0, acc, cval);
+}
+
+// Whole-row projector for view subsets whose neighbouring views are far apart
+// (OS-SART subsets k, k + numOS, ...): no two views of a block cross the same
+// short stretch of a row, so k_proj_lds's staged runs do not serve them, and
+// the per-lane gathers of k_proj keep the texture address unit saturated.
+// Here a block of NW waves owns up to VB views of the subset, all their
+// detector boundaries and ZS slices of a table chunk, and copies, slab after
+// slab, the ZS whole table rows of that slab into LDS (global-to-LDS copies,
+// two buffers alternating: the next slab's rows are in flight while the
+// current one is summed); every boundary then reads its entries from LDS,
+// where a slab's rows serve all the block's views.  A block's views share one
+// slab orientation (one table): the subset is split into runs of consecutive
+// views of one orientation and each run into blocks of VB views, so no block
+// sweeps the slabs twice.  Tasks are (view, group of 63 detectors): slot s of
+// wave w is task NW*s + w, T slots per wave.  Each task's terms and their
+// order are those of k_proj for the same slice chunk, so the results are
+// bit-identical.  Rows of up to KW entries (images up to KW - 1 voxels wide).
+// Waves per SIMD: the blocks the two staging buffers leave room for in 160 KiB
+// of LDS, times NW / 4, at most 4 (the register budget of 12 tasks).
+constexpr int proj_rows_waves(int MODE, int ZS, int KW, int NW) {
+    return (160 * 1024) / (2 * ZS * KW * (MODE == SAT_F32 ? 8 : 16)) * NW / 4 < 4
+               ? (160 * 1024) / (2 * ZS * KW * (MODE == SAT_F32 ? 8 : 16)) * NW / 4
+               : 4;
+}
+template <int MODE, int ZS, int T, int KW, int NW>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(proj_rows_waves(MODE, ZS, KW, NW))))
+void k_proj_rows(const ProjLaunch L) {
+    using E = typename Sat<MODE>::E;
+    constexpr int PER = 16 / (int)sizeof(E);  // entries per lane per copy
+    static_assert(KW % PER == 0, "row capacity in whole 16-byte copies");
+    __shared__ __attribute__((aligned(16))) E bufA[ZS * KW];
+    __shared__ __attribute__((aligned(16))) E bufB[ZS * KW];
+    using LE = DDF_LDS E;
+    LE* const ldsA = (LE*)bufA;
+    LE* const ldsB = (LE*)bufB;
+    const int lin = xcd_linear(blockIdx.x, gridDim.x);
+    if (lin >= L.nwork) return;  // whole block
+    int ji = 0;
+    for (int t = 1; t < L.njobs; ++t)
+        if (lin >= L.j[t].blk0) ji = t;
+    const ProjDev& J = L.j[ji];
+    const GeomDev g = J.g;
+    const Subset sub = J.sub;
+    const int ZT = J.zt, VB = J.vb, nug = J.nug;
+    const int local = lin - J.blk0;
+    const int jb = local % J.nvg;  // view block (the fastest: blocks of one chunk share rows)
+    const int rest = local / J.nvg;
+    const int nh = ZT / ZS;
+    const int hz = rest % nh;      // which ZS slices of the chunk
+    const int zc = rest / nh;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int z0 = zc * ZT;  // the chunk's slices share their angles: read at its first
+    const int zs = min(z0, g.SLN - 1);
+
+    // The block's views [jv0, jv0 + nv) and their orientation xo (block-uniform):
+    // block jb of the runs of one orientation, each cut into blocks of VB views.
+    // The orientation bits of 64 views at a time come from one ballot.
+    int jv0 = 0, nv = 0;
+    bool xo = false;
+    {
+        int b = jb, start = 0;
+        bool ro = false, found = false;
+        for (int w0 = 0; w0 < sub.count && !found; w0 += 64) {
+            bool o = false;
+            if (w0 + lane < sub.count) {
+                const float2 cs = J.cossin[(size_t)(sub.first + (w0 + lane) * sub.stride) * g.SLN + zs];
+                o = proj_xslab(cs.x, cs.y);
+            }
+            const unsigned long long m = __ballot(o);
+            const int n = min(64, sub.count - w0);
+            for (int i = 0; i < n; ++i) {
+                const bool oi = (m >> i) & 1ull;
+                const int jj = w0 + i;
+                if (jj == 0) {
+                    ro = oi;
+                } else if (oi != ro) {  // the run [start, jj) ends
+                    const int nb = (jj - start + VB - 1) / VB;
+                    if (b < nb) {
+                        jv0 = start + b * VB;
+                        nv = min(VB, jj - jv0);
+                        xo = ro;
+                        found = true;
+                        break;
+                    }
+                    b -= nb;
+                    start = jj;
+                    ro = oi;
+                }
+            }
+        }
+        if (!found) {  // the last run [start, count)
+            const int nb = (sub.count - start + VB - 1) / VB;
+            if (b < nb) {
+                jv0 = start + b * VB;
+                nv = min(VB, sub.count - jv0);
+                xo = ro;
+            }
+        }
+    }
+    if (nv == 0) return;  // past the subset's blocks (whole block)
+
+    // The wave's tasks: boundary rays of this lane; bit s of `on`: slot s holds
+    // a task (wave-uniform).
+    float pa[T], pf[T], pbb[T];
+    unsigned on = 0;
+#pragma unroll
+    for (int s = 0; s < T; ++s) {
+        const int t = s * NW + wave;
+        const int tv = t / nug, tug = t - tv * nug;
+        pa[s] = 0.0f;
+        pf[s] = 0.0f;
+        pbb[s] = 0.0f;
+        if (tv < nv) {
+            on |= 1u << s;
+            const int j = jv0 + tv;
+            const float2 cs = J.cossin[(size_t)(sub.first + j * sub.stride) * g.SLN + zs];
+            const int bnd = min(tug * 63 + lane, g.DNU);
+            const ProjBoundary pb = proj_boundary(g.x0, g.y0, cs.x, cs.y, J.bx[bnd], J.by[bnd]);
+            pa[s] = pb.a;
+            pf[s] = pb.f;
+            pbb[s] = pb.b;
+        }
+    }
+
+    float acc[T][ZS];
+#pragma unroll
+    for (int s = 0; s < T; ++s)
+#pragma unroll
+        for (int q = 0; q < ZS; ++q) acc[s][q] = 0.0f;
+
+    const int Nk = xo ? g.YN : g.XN, K = Nk + 1, Ns = xo ? g.XN : g.YN;
+    const float ocS = xo ? g.ocxP : g.ocyP, ocK = xo ? g.ocyP : g.ocxP;
+    const E* tab = reinterpret_cast<const E*>(xo ? J.satY : J.satX) +
+                   (size_t)zc * Ns * (size_t)ZT * K + (size_t)hz * ZS * K;
+    const int ncp = (K + 64 * PER - 1) / (64 * PER);  // 64-lane copies per row
+    const int iB = J.ii0, iE = min(J.ii1, Ns);
+    // copy c = q * ncp + cp (row q, piece cp) is issued by wave c % NW
+    auto stage = [&](int ii, LE* buf) __attribute__((always_inline)) {
+        const E* rows = tab + (size_t)ii * ZT * K;
+#pragma unroll
+        for (int q = 0; q < ZS; ++q)
+            for (int cp = (wave - q * ncp) & (NW - 1); cp < ncp; cp += NW) {
+                const int e0 = cp * 64 * PER;
+                stage_row(rows + (size_t)q * K + e0, (DDF_LDS void*)&buf[q * KW + e0], K - e0,
+                          lane);
+            }
+    };
+    auto sum = [&](int ii, const LE* buf) __attribute__((always_inline)) {
+        const float obj = slab_obj(ii, ocS, g.dx);
+#pragma unroll
+        for (int s = 0; s < T; ++s) {
+            if (!((on >> s) & 1u)) continue;  // uniform across the wave
+            const float r = slab_r(obj, pa[s], pf[s], pbb[s]);
+            const Fetch f = fetch_pos<E>(slab_coord(r, g.invdx, ocK), Nk);
+            const float invlen = __builtin_amdgcn_rcpf(r - lane_prev(r));
+#pragma unroll
+            for (int q = 0; q < ZS; ++q) {
+                const Samp<MODE> Tq(lds_entry(&buf[q * KW + f.i0]), f.t);
+                acc[s][q] = fmaf(Tq.minus_prev_lane(), invlen, acc[s][q]);
+            }
+        }
+    };
+    if (iB < iE) {
+        stage(iB, ldsA);
+        wait_vmem();
+        __syncthreads();
+    }
+    for (int ii = iB; ii < iE; ii += 2) {
+        if (ii + 1 < iE) stage(ii + 1, ldsB);
+        sum(ii, ldsA);
+        wait_vmem();
+        __syncthreads();
+        if (ii + 1 >= iE) break;
+        if (ii + 2 < iE) stage(ii + 2, ldsA);
+        sum(ii + 1, ldsB);
+        wait_vmem();
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int s = 0; s < T; ++s) {
+        if (!((on >> s) & 1u)) continue;
+        const int t = s * NW + wave;
+        const int tv = t / nug, tug = t - tv * nug;
+        const int j = jv0 + tv;
+        const int u = tug * 63 + lane - 1;
+        if (lane < 1 || u >= g.DNU) continue;
+        const float2 cs = J.cossin[(size_t)(sub.first + j * sub.stride) * g.SLN + zs];
+        const float cval = proj_cval(g, cs.x, cs.y, xo, J.xds[u], J.yds[u]);
+        proj_store_view<ZS>(J, g, sub, j, u, z0 + hz * ZS, acc[s], cval);
+    }
 }
 
 // ------------------------------------------------------------------ index traces
@@ -1490,18 +1695,80 @@ hipError_t launch_sat_sino(int mode, int ZT, int DNU, int SLN, int nzc, int PNs,
     return launch_sat_sino_batch(mode, &j, 1, st);
 }
 
+// Shapes of k_proj_rows by the plan's tasks-per-wave value: 9 or 12 tasks in
+// blocks of 4 waves, 5 or 6 tasks in blocks of 8 waves.
+static int proj_rows_nw(int V) { return V >= 9 ? 4 : 8; }
+static int proj_rows_t(int V) { return V; }
+
+// Blocks per slice chunk of k_proj_rows: the subset's runs of views of one slab
+// orientation, each cut into blocks of VB views (the most over the chunks;
+// without the host angle table, one block per view).
+static int proj_rows_nvg(const ProjJob& J, int VB) {
+    const int n = J.sub.count;
+    if (!J.hcs) return n;
+    int most = 0;
+    for (int zc = 0; zc < J.nzc; ++zc) {
+        const int zs = std::min(zc * J.ZT, J.g.SLN - 1);
+        int blocks = 0, start = 0;
+        bool ro = false;
+        for (int j = 0; j <= n; ++j) {
+            bool o = false;
+            if (j < n) {
+                const size_t v = (size_t)(J.sub.first + (long long)j * J.sub.stride) * J.g.SLN + zs;
+                o = proj_xslab(J.hcs[2 * v], J.hcs[2 * v + 1]);
+            }
+            if (j == 0) {
+                ro = o;
+            } else if (j == n || o != ro) {
+                blocks += (j - start + VB - 1) / VB;
+                start = j;
+                ro = o;
+            }
+        }
+        most = std::max(most, blocks);
+    }
+    return most;
+}
+
+// Row capacity of k_proj_rows for rows of K entries (0: too long).
+static int proj_rows_kw(int mode, int K) {
+    const int per = mode == SAT_F32 ? 2 : 1;
+    const int k = (K + per - 1) / per * per;
+    return k <= 520 ? 520 : k <= 776 ? 776 : 0;
+}
+
+// Launch shape checks of the projector kernels: stage 0/1 take 1, 2 or 4 views
+// per wave; the whole-row kernel (stage 2: 4 slices per block, 3: 2) takes 5,
+// 6, 9 or 12 tasks per wave, a slice chunk that is a multiple of its slices,
+// rows within its capacity and a view block of at least one view.
+bool proj_shape_ok(int mode, const GeomDev& g, int ZT, int V, int stage) {
+    if (stage == 0 || stage == 1) return V == 1 || V == 2 || V == 4;
+    if (stage != 2 && stage != 3) return false;
+    const int ZS = stage == 2 ? 4 : 2;
+    const int nug = (g.DNU + 62) / 63;
+    return (V == 5 || V == 6 || V == 9 || V == 12) && ZT >= ZS && ZT % ZS == 0 &&
+           proj_rows_kw(mode, (g.XN > g.YN ? g.XN : g.YN) + 1) > 0 &&
+           proj_rows_nw(V) * proj_rows_t(V) >= nug;
+}
+
 // Projector launches: jobs of one slice chunk ZT go together, kBatch at a time;
 // pass p of every job that still has slabs left shares one launch.
 static hipError_t launch_proj_group(int mode, int ZT, int V, int stage,
                                     const ProjJob* const* jobs, int n, hipStream_t st) {
     int maxPass = 0;
     int T[kBatch], Nmax[kBatch];
+    int KW = 0;  // row capacity of the whole-row kernel: the launch's longest rows
     for (int i = 0; i < n; ++i) {
         Nmax[i] = jobs[i]->g.XN > jobs[i]->g.YN ? jobs[i]->g.XN : jobs[i]->g.YN;
         T[i] = (jobs[i]->slabTile <= 0 || jobs[i]->slabTile > Nmax[i]) ? Nmax[i] : jobs[i]->slabTile;
         const int np = (Nmax[i] + T[i] - 1) / T[i];
         maxPass = np > maxPass ? np : maxPass;
+        if (!proj_shape_ok(mode, jobs[i]->g, ZT, V, stage)) return hipErrorInvalidValue;
+        const int kw = proj_rows_kw(mode, Nmax[i] + 1);
+        KW = kw > KW ? kw : KW;
     }
+    const bool rows = stage >= 2;
+    const int ZS = stage == 2 ? 4 : 2;
     for (int p = 0; p < maxPass; ++p) {
         ProjLaunch L{};
         long long nw = 0;
@@ -1515,8 +1782,10 @@ static hipError_t launch_proj_group(int mode, int ZT, int V, int stage,
             d.sub = J.sub;
             d.nzc = J.nzc;
             d.nug = (J.g.DNU + 62) / 63;  // 63 detectors per wave (64 boundaries)
-            d.nvg = (J.sub.count + V - 1) / V;
-            if (stage) d.nvg = (d.nvg + 3) / 4;  // blocks of four view groups
+            d.zt = ZT;
+            d.vb = rows ? proj_rows_nw(V) * proj_rows_t(V) / d.nug : V;
+            d.nvg = rows ? proj_rows_nvg(J, d.vb) : (J.sub.count + d.vb - 1) / d.vb;
+            if (stage == 1) d.nvg = (d.nvg + 3) / 4;  // blocks of four view groups
             d.blk0 = (int)nw;
             d.ii0 = ii0;
             d.ii1 = ii0 + T[i];
@@ -1533,7 +1802,7 @@ static hipError_t launch_proj_group(int mode, int ZT, int V, int stage,
             d.out = J.out;
             d.meas = J.meas;
             d.rowSum = J.rowSum;
-            nw += (long long)d.nug * d.nvg * d.nzc;
+            nw += rows ? (long long)d.nvg * d.nzc * (ZT / ZS) : (long long)d.nug * d.nvg * d.nzc;
         }
         if (m == 0) continue;
         if (nw > 0x7fffff00LL) return hipErrorInvalidValue;
@@ -1546,13 +1815,37 @@ static hipError_t launch_proj_group(int mode, int ZT, int V, int stage,
     } else {                                                                               \
         DDF_MODE_SWITCH(mode, DDF_ZT_SWITCH(ZT, (k_proj<MC, ZTC, VV><<<grid, 64, 0, st>>>(L))))      \
     }
-        if (V == 4) {
+#define DDF_ROWS(ZSC, TT, NWC)                                                             \
+    if (KW == 520) {                                                                       \
+        DDF_MODE_SWITCH(mode, (k_proj_rows<MC, ZSC, TT, 520, NWC><<<grid, 64 * NWC, 0, st>>>(L))) \
+    } else {                                                                               \
+        DDF_MODE_SWITCH(mode, (k_proj_rows<MC, ZSC, TT, 776, NWC><<<grid, 64 * NWC, 0, st>>>(L))) \
+    }
+#define DDF_ROWS_T(ZSC)                                                                    \
+    if (V == 12) {                                                                         \
+        DDF_ROWS(ZSC, 12, 4)                                                               \
+    } else if (V == 9) {                                                                   \
+        DDF_ROWS(ZSC, 9, 4)                                                                \
+    } else if (V == 6) {                                                                   \
+        DDF_ROWS(ZSC, 6, 8)                                                                \
+    } else {                                                                               \
+        DDF_ROWS(ZSC, 5, 8)                                                                \
+    }
+        if (rows) {
+            if (ZS == 4) {
+                DDF_ROWS_T(4);
+            } else {
+                DDF_ROWS_T(2);
+            }
+        } else if (V == 4) {
             DDF_PROJ(4);
         } else if (V == 2) {
             DDF_PROJ(2);
         } else {
             DDF_PROJ(1);
         }
+#undef DDF_ROWS_T
+#undef DDF_ROWS
 #undef DDF_PROJ
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
@@ -1567,17 +1860,18 @@ static hipError_t launch_proj_group(int mode, int ZT, int V, int stage,
 hipError_t launch_proj_batch(int mode, const ProjJob* jobs, int n, hipStream_t st,
                              long long fillWaves) {
     for (int i = 0; i < n; ++i)
-        if (jobs[i].stage != 0 && jobs[i].stage != 1) return hipErrorInvalidValue;
-    for (int stage : {0, 1})
-        for (int V : {1, 2, 4}) {
+        if (jobs[i].sub.count > 0 && !proj_shape_ok(mode, jobs[i].g, jobs[i].ZT, jobs[i].V,
+                                                    jobs[i].stage))
+            return hipErrorInvalidValue;
+    for (int stage : {0, 1, 2, 3})
+        for (int V : {1, 2, 4, 5, 6, 9, 12}) {
             for (int ZT : {1, 2, 4, 8, 16}) {
                 const ProjJob* grp[kBatch];
                 int m = 0;
                 long long work = 0;
                 for (int i = 0; i < n; ++i) {
                     const ProjJob& J = jobs[i];
-                    const int Vj = J.V == 2 || J.V == 4 ? J.V : 1;
-                    if (J.ZT != ZT || Vj != V || J.stage != stage || J.sub.count <= 0) continue;
+                    if (J.ZT != ZT || J.V != V || J.stage != stage || J.sub.count <= 0) continue;
                     const long long w =
                         (long long)((J.g.DNU + 62) / 63) * ((J.sub.count + V - 1) / V) * J.nzc;
                     if (m == kBatch || (m > 0 && work + w > fillWaves)) {
@@ -1602,10 +1896,10 @@ hipError_t launch_proj(int mode, int ZT, int epi, const GeomDev& g, int nzc, con
                        const float* bx, const float* by, const float* xds, const float* yds,
                        const void* satX, const void* satY, Subset sub, float* out,
                        const float* meas, const float* rowSum, int slabTile, int viewsPerWave,
-                       int stage, hipStream_t st) {
+                       int stage, hipStream_t st, const float* hcs) {
     if (sub.count <= 0) return hipSuccess;
     const ProjJob j{g,   sub,  nzc, ZT,   slabTile, epi,  viewsPerWave, cossin, bx,    by,
-                    xds, yds,  satX, satY, out,     meas, rowSum,       stage};
+                    xds, yds,  satX, satY, out,     meas, rowSum,       stage,  hcs};
     return launch_proj_batch(mode, &j, 1, st);
 }
 

@@ -22,6 +22,7 @@ int ossart_prepare(OsJob& J, hipStream_t st) {
     CHECK(P && J.meas && J.x, DDFAN_EINVAL, "NULL argument");
     CHECK(J.numIter >= 0, DDFAN_EINVAL, "numIter must be >= 0");
     CHECK(J.numOS >= 1 && J.numOS <= P->PN, DDFAN_EINVAL, "numOS must be in [1, %d]", P->PN);
+    TRY(check_proj_shape(P));
     const GeomDev& g = P->g;
     const int DNU = g.DNU, XN = g.XN, YN = g.YN, SLN = g.SLN, mode = P->mode;
     const int numOS = J.numOS;
@@ -140,7 +141,7 @@ hipError_t enqueue_batch_iteration(OsJob* const* jobs, int n, int mode, const Ba
                                  P->bx.as<float>(), P->by.as<float>(), P->xds.as<float>(),
                                  P->yds.as<float>(), P->satX.p, P->satY.p, P->rbuf.as<float>(),
                                  J.meas, P->rowSum.as<float>() + (size_t)k * J.maxc * g.DNU,
-                                 P->projStage});
+                                 P->projStage, P->h_cs.data()});
             ss.push_back(SatSinoJob{g.DNU, g.SLN, P->nzcb, P->ZTb, J.cnt[k], P->rbuf.as<float>(),
                                     P->satP.p});
             bj.push_back(BackJob{g, sub, P->nzcb, P->ZTb, epi, P->cs.as<float2>(),

@@ -78,6 +78,8 @@ int upload(DBuf& b, const void* src, size_t bytes, hipStream_t st);
 // Argument checks and kernel scalars shared by the entry points.
 int check_subset(int PN, int first, int stride, int count);
 int check_start_idx(const int* startIdx, int nStart, int SLN);
+// The plan's projector launch shape fits its geometry (DDFAN_EINVAL if not).
+int check_proj_shape(const ddfan_plan* P);
 int validate_geom(const ddfan_geom* G);
 void fill_geom(GeomDev& g, const ddfan_geom* G, int SLN);
 // SAT precision of the drop-in entry points (ddfan_set_default_sat_mode).
@@ -103,9 +105,9 @@ struct ddfan_plan {
     int ZT = 1, nzc = 1;   // projector slice chunk and chunk count
     int ZTb = 1, nzcb = 1; // backprojector slice chunk and chunk count
     int slabTile = 0;      // projector slabs per pass (0: all slabs in one pass)
-    int viewsPerWave = 4;  // projector views per wave (1, 2 or 4)
+    int viewsPerWave = 4;  // projector views per wave (1, 2 or 4); tasks per wave (5, 6, 9, 12) at stage >= 2
     int backStage = 0;     // backprojector views per LDS-staged group (0: per-lane gathers)
-    int projStage = 0;     // 1: projector reads LDS-staged table rows (0: per-lane gathers)
+    int projStage = 0;     // 0: per-lane gathers, 1: staged runs, 2/3: whole rows (4/2 slices)
     int backPass = 0;      // backprojector views per pass of a plain call (0: one pass)
     std::vector<float> h_hangs, h_cs;
     ddf::DBuf cs, bx, by, xds, yds, mask;

@@ -135,29 +135,44 @@ int ddfan_plan_autotune(ddfan_plan* P, int views, int reps, int flags, int* out_
                 // subset (consecutive views when views = PN); the spacing decides
                 // how much of a table row neighbouring views of a launch share
                 const int stride = std::max(1, P->PN / views);
+                // spaced launches: three subsets (first views 0, stride/3, 2 stride/3),
+                // since where a subset's views fall against the slab-orientation
+                // changes at 45 degrees decides the whole-row projector's blocks
+                const int nsub = stride >= 3 ? 3 : 1;
                 auto proj = [&]() {
-                    return ddfan_plan_proj(P, vol.as<float>(), prj.as<float>(), 0, stride, views);
+                    for (int k = 0; k < nsub; ++k)
+                        TRY(ddfan_plan_proj(P, vol.as<float>(), prj.as<float>(), k * stride / 3,
+                                            stride, views));
+                    return (int)DDFAN_OK;
                 };
                 auto back = [&]() {
                     return ddfan_plan_back(P, prj.as<float>(), vol.as<float>(), 0, stride, views);
                 };
                 double bestP = 1e30;
                 TRY(ddfan_plan_set_param(P, DDFAN_PARAM_SLAB_TILE, 0));
+                // the whole-row projector (stages 2, 3) only for spaced views, whose
+                // neighbours share no stretch of a row
+                std::vector<std::pair<int, int>> shapes;  // (views or tasks per wave, stage)
+                for (int v : {4, 2, 1})
+                    for (int ps : {1, 0}) shapes.push_back({v, ps});
+                if (stride > 1)
+                    for (int ps : {2, 3})
+                        for (int v : {12, 9, 6, 5}) shapes.push_back({v, ps});
                 for (int z : chunks) {
                     TRY(ddfan_plan_set_param(P, DDFAN_PARAM_PROJ_CHUNK, z));
-                    for (int v : {4, 2, 1}) {
+                    for (const auto& sh : shapes) {
+                        const int v = sh.first, ps = sh.second;
+                        if (!proj_shape_ok(P->mode, g, z, v, ps)) continue;
                         TRY(ddfan_plan_set_param(P, DDFAN_PARAM_VIEWS_PER_WAVE, v));
-                        for (int ps : {1, 0}) {
-                            TRY(ddfan_plan_set_param(P, DDFAN_PARAM_PROJ_STAGE, ps));
-                            double ms;
-                            TRY(time_ms(P->stream, reps, proj, &ms));
-                            if (ms < bestP) {
-                                bestP = ms;
-                                best.ZT = z;
-                                best.V = v;
-                                best.tile = 0;
-                                best.pstage = ps;
-                            }
+                        TRY(ddfan_plan_set_param(P, DDFAN_PARAM_PROJ_STAGE, ps));
+                        double ms;
+                        TRY(time_ms(P->stream, reps, proj, &ms));
+                        if (ms < bestP) {
+                            bestP = ms;
+                            best.ZT = z;
+                            best.V = v;
+                            best.tile = 0;
+                            best.pstage = ps;
                         }
                     }
                 }
@@ -175,7 +190,7 @@ int ddfan_plan_autotune(ddfan_plan* P, int views, int reps, int flags, int* out_
                         best.tile = t;
                     }
                 }
-                best.projMs = bestP;
+                best.projMs = bestP / nsub;  // per call
                 double bestB = 1e30;
                 for (int z : chunks) {
                     TRY(ddfan_plan_set_param(P, DDFAN_PARAM_BACK_STAGE, 0));
@@ -309,8 +324,13 @@ int ddfan_tune_load(const char* path, int* entries) {
             }
             const bool pow2 = v.ZT >= 1 && v.ZT <= 16 && (v.ZT & (v.ZT - 1)) == 0 && v.ZTb >= 1 &&
                               v.ZTb <= 16 && (v.ZTb & (v.ZTb - 1)) == 0;
-            if (!pow2 || v.ZT > k.ZTmax || v.ZTb > k.ZTmax || !(v.V == 1 || v.V == 2 || v.V == 4) ||
-                v.tile < 0 || !stage_fits(v.stage, v.ZTb) || (v.pstage != 0 && v.pstage != 1)) {
+            const bool pshape = (v.pstage == 0 || v.pstage == 1)
+                                    ? (v.V == 1 || v.V == 2 || v.V == 4)
+                                    : (v.pstage == 2 || v.pstage == 3) &&
+                                          (v.V == 5 || v.V == 6 || v.V == 9 || v.V == 12) &&
+                                          v.ZT % (v.pstage == 2 ? 4 : 2) == 0;
+            if (!pow2 || v.ZT > k.ZTmax || v.ZTb > k.ZTmax || !pshape || v.tile < 0 ||
+                !stage_fits(v.stage, v.ZTb)) {
                 bad = lineNo;
                 break;
             }

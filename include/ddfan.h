@@ -162,7 +162,9 @@ int ddfan_plan_query(const ddfan_plan* plan, int* slice_chunk, int* n_chunks, in
  * slabs); smaller passes keep the table band each pass reads inside L2. */
 #define DDFAN_PARAM_SLAB_TILE 1
 /* DDFAN_PARAM_VIEWS_PER_WAVE: consecutive views one projector wave handles
- * (1, 2 or 4); they share table rows through L1. */
+ * (1, 2 or 4); they share table rows through L1.  With DDFAN_PARAM_PROJ_STAGE
+ * 2 or 3 it counts the whole-row projector's tasks (view x 63 detectors) per
+ * wave instead: 9 or 12 in blocks of 4 waves, 5 or 6 in blocks of 8. */
 #define DDFAN_PARAM_VIEWS_PER_WAVE 2
 /* DDFAN_PARAM_PROJ_CHUNK / DDFAN_PARAM_BACK_CHUNK: slices per chunk (a power
  * of two up to the widest chunk whose angles agree) of the projector and of
@@ -177,7 +179,11 @@ int ddfan_plan_query(const ddfan_plan* plan, int* slice_chunk, int* n_chunks, in
 #define DDFAN_PARAM_BACK_STAGE 5
 /* DDFAN_PARAM_PROJ_STAGE: 1 = the projector copies each slab's table run
  * into LDS for four view groups at a time; 0 = every lane reads the table in
- * global memory.  Results are bit-identical. */
+ * global memory; 2 / 3 = whole table rows in LDS for the views of a block
+ * (4 / 2 slices per block; for spaced views such as OS-SART subsets; needs a
+ * projector chunk that is a multiple of 4 / 2 and images up to 775 voxels
+ * wide, else the projector call fails with DDFAN_EINVAL).  Results are
+ * bit-identical for every value. */
 #define DDFAN_PARAM_PROJ_STAGE 6
 /* DDFAN_PARAM_BACK_PASS: a plain backprojection of more views than this runs
  * as launches of at most this many views, each continuing the sums the

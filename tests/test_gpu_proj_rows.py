@@ -1,0 +1,171 @@
+"""Whole-row projector (stage 2: 4 slices per block, stage 3: 2) against the
+per-lane gather kernel: every slice chunk x tasks per wave x slab pass shape
+gives bit-identical sinograms (tolerance 0) in all three table precisions, for
+all views and for strided subsets (the case the kernel is for), including
+non-square images (blocks sweeping both slab orientations), rows in the
+larger of the two LDS row capacities (images over 519 voxels), view blocks
+larger than the subset, the OS-SART ratio epilogue, and shapes that do not fit
+the kernel, which the plan rejects."""
+import numpy as np
+import pytest
+
+from helpers import small_conf
+
+pytestmark = pytest.mark.gpu
+
+# 'coarse': the test scanner; 'oblong': 72 x 40 voxels (row lengths differ per
+# orientation); 'wide': 8-channel bins over 1 mm voxels; 'big': 600 x 520
+# voxels, rows of 601 / 521 entries (the 776-entry row capacity).
+CASES = [
+    ("coarse", dict(XN=64, YN=64, SLN=8), 37),
+    ("oblong", dict(XN=72, YN=40, SLN=8), 33),
+    ("wide", dict(XN=160, YN=160, SLN=4, dx=1.0, DNU=64, bin=8), 23),
+    ("big", dict(XN=600, YN=520, SLN=4, dx=0.8, DNU=736, bin=1), 12),
+]
+
+
+def _setup(kw, PN, seed=8):
+    import aapm_lowdosectimagerecon_competition_amd as dd
+    from aapm_lowdosectimagerecon_competition_amd import phantoms
+    conf = small_conf(**kw)
+    g = dd.detector_geometry(conf)
+    hangs = phantoms.uniform_angles(PN, g.SLN)
+    vol = phantoms.ellipsoid_phantom(g.XN, g.YN, g.SLN, conf.recon.dx, seed=seed)
+    return dd, g, hangs, vol, dd.recon_mask(g.XN, g.YN)
+
+
+def _calls(plan, vol, PN):
+    return (plan.proj_host(vol),
+            plan.proj_host(vol, first=1, stride=4, count=(PN - 1 + 3) // 4))
+
+
+@pytest.mark.parametrize("sat", ["f32", "twofloat", "f64"])
+@pytest.mark.parametrize("name,kw,PN", CASES, ids=[c[0] for c in CASES])
+def test_rows_projector_equals_gather(sat, name, kw, PN):
+    dd, g, hangs, vol, mask = _setup(kw, PN)
+    print(f"\n[prows] {name} {sat}: {g.XN}x{g.YN}x{g.SLN}, DNU {g.DNU}, {PN} views", flush=True)
+    ran = 0
+    with dd.Plan(g, hangs, mask, sat_mode=sat) as plan:
+        widest = plan.slice_chunk
+        for zt in (8, 4, 2):
+            if zt > widest:
+                continue
+            plan.proj_chunk = zt
+            for tile in (0, 16):
+                plan.slab_tile = tile
+                plan.proj_stage = 0
+                plan.views_per_wave = 1
+                ref = _calls(plan, vol, PN)
+                for st in (2, 3):
+                    if zt % (4 if st == 2 else 2):
+                        continue
+                    for T in (12, 9, 6, 5):
+                        plan.views_per_wave = T
+                        plan.proj_stage = st
+                        got = _calls(plan, vol, PN)
+                        plan.proj_stage = 0
+                        plan.views_per_wave = 1
+                        assert np.array_equal(got[0], ref[0]), (zt, tile, st, T)
+                        assert np.array_equal(got[1], ref[1]), (zt, tile, st, T)
+                        ran += 1
+    print(f"[prows] {ran} chunk x stage x tasks x pass shapes bit-identical", flush=True)
+    assert ran >= 12
+
+
+def test_rows_projector_in_ossart():
+    """The ratio epilogue (b - Ax) / rowSum on the last slab pass: OS-SART with
+    the whole-row projector equals the per-lane one bit for bit."""
+    dd, g, hangs, vol, mask = _setup(dict(XN=64, YN=64, SLN=8), 48)
+    with dd.Plan(g, hangs, mask, sat_mode="twofloat") as plan:
+        meas = plan.proj_host(vol)
+    x0 = np.zeros_like(vol)
+    res = {}
+    print("\n[prows] OS-SART 3 iterations x 4 subsets, whole-row vs per-lane projector", flush=True)
+    for st, V in ((0, 4), (2, 9), (3, 12)):
+        with dd.Plan(g, hangs, mask, sat_mode="twofloat") as plan:
+            plan.proj_chunk = 8
+            plan.views_per_wave = V
+            plan.proj_stage = st
+            plan.slab_tile = 16  # ratio on the last of four passes
+            res[st] = plan.ossart_host(meas, x0, 3, 4, use_fista=1)
+    assert np.array_equal(res[0], res[2])
+    assert np.array_equal(res[0], res[3])
+
+
+def test_rows_projector_full_size_subset():
+    """512x512x64, 736 channels, 96-view OS-SART subsets (views k, k+24, ...;
+    k = 0 puts views exactly on the 45-degree orientation changes): bit-identical
+    to the per-lane kernel at the same slice chunk."""
+    dd, g, hangs, vol, mask = _setup(dict(XN=512, YN=512, SLN=64, dx=0.9765625, DNU=736,
+                                          bin=1), 2304, seed=1)
+    with dd.Plan(g, hangs, mask, sat_mode="twofloat") as plan:
+        plan.proj_chunk = 8
+        for first in (0, 5, 13):
+            plan.proj_stage = 0
+            plan.views_per_wave = 4
+            ref = plan.proj_host(vol, first=first, stride=24, count=96)
+            for st, T in ((2, 9), (2, 5), (3, 6), (3, 12)):
+                plan.views_per_wave = T
+                plan.proj_stage = st
+                got = plan.proj_host(vol, first=first, stride=24, count=96)
+                plan.proj_stage = 0
+                plan.views_per_wave = 4
+                assert np.array_equal(got, ref), (first, st, T)
+    print("\n[prows] 512x512x64, 96-view subsets: bit-identical", flush=True)
+
+
+def test_rows_projector_unsorted_views():
+    """Views in no angular order (orientation changing from view to view) and a
+    subset of one view: every block still sees one orientation."""
+    import aapm_lowdosectimagerecon_competition_amd as dd
+    from aapm_lowdosectimagerecon_competition_amd import phantoms
+    conf = small_conf(XN=64, YN=48, SLN=8)
+    g = dd.detector_geometry(conf)
+    rng = np.random.default_rng(3)
+    PN = 40
+    hangs = np.repeat(rng.uniform(0, 2 * np.pi, (PN, 1)), g.SLN, axis=1).astype(np.float32)
+    vol = phantoms.ellipsoid_phantom(g.XN, g.YN, g.SLN, conf.recon.dx, seed=4)
+    with dd.Plan(g, hangs, dd.recon_mask(g.XN, g.YN), sat_mode="f32") as plan:
+        plan.proj_chunk = 8
+        for first, stride, count in ((0, 1, PN), (3, 2, 18), (7, 5, 1)):
+            plan.proj_stage = 0
+            plan.views_per_wave = 1
+            ref = plan.proj_host(vol, first=first, stride=stride, count=count)
+            for st, T in ((2, 5), (3, 12)):
+                plan.views_per_wave = T
+                plan.proj_stage = st
+                got = plan.proj_host(vol, first=first, stride=stride, count=count)
+                assert np.array_equal(got, ref), (first, stride, count, st, T)
+
+
+def test_rows_projector_shape_checks():
+    dd, g, hangs, vol, mask = _setup(dict(XN=32, YN=32, SLN=8), 8)
+    from aapm_lowdosectimagerecon_competition_amd import _native as N
+    with dd.Plan(g, hangs, mask, sat_mode="twofloat") as plan:
+        for bad in (4, -1):
+            with pytest.raises(N.DDFanError):
+                plan.proj_stage = bad
+        for bad in (3, 7, 16):
+            with pytest.raises(N.DDFanError):
+                plan.views_per_wave = bad
+        plan.proj_chunk = 8
+        plan.views_per_wave = 4
+        plan.proj_stage = 2            # 4 views per wave: not a whole-row shape
+        with pytest.raises(N.DDFanError):
+            plan.proj_host(vol)
+        plan.views_per_wave = 9
+        plan.proj_chunk = 2            # a 2-slice chunk cannot feed 4 slices per block
+        with pytest.raises(N.DDFanError):
+            plan.proj_host(vol)
+        plan.proj_stage = 3            # ... but 2 slices per block
+        ok = plan.proj_host(vol)
+        plan.proj_stage = 0
+        plan.views_per_wave = 1
+        assert np.array_equal(ok, plan.proj_host(vol))
+    dd2, g2, h2, v2, m2 = _setup(dict(XN=800, YN=64, SLN=4, dx=0.6, DNU=184), 4)
+    with dd2.Plan(g2, h2, m2, sat_mode="twofloat") as plan:
+        plan.proj_chunk = 4
+        plan.views_per_wave = 9
+        plan.proj_stage = 2            # rows of 801 entries exceed the row capacity
+        with pytest.raises(N.DDFanError):
+            plan.proj_host(v2)
