@@ -1263,6 +1263,7 @@ void k_proj_rows(const ProjLaunch L) {
             pbb[s] = pb.b;
         }
     }
+    on = __builtin_amdgcn_readfirstlane(on);
 
     float acc[T][ZS];
 #pragma unroll
