@@ -122,7 +122,7 @@ int check_proj_shape(const ddfan_plan* P) {
     CHECK(proj_shape_ok(P->mode, P->g, P->ZT, P->viewsPerWave, P->projStage), DDFAN_EINVAL,
           "projector launch shape (chunk %d, %d per wave, stage %d) does not fit this plan: "
           "stages 0 and 1 take 1, 2 or 4 views per wave; the whole-row projector (stage 2: 4 "
-          "slices per block, 3: 2) takes 5, 6, 9 or 12 tasks per wave, a chunk that is a multiple "
+          "slices per block, 3: 2) takes 3, 4, 5, 6, 9 or 12 tasks per wave, a chunk that is a multiple "
           "of its slices and images up to 775 voxels wide",
           P->ZT, P->viewsPerWave, P->projStage);
     return DDFAN_OK;
@@ -379,11 +379,11 @@ int ddfan_plan_set_param(ddfan_plan* P, int param, int value) {
                 P->slabTile = value;
                 return DDFAN_OK;
             case DDFAN_PARAM_VIEWS_PER_WAVE:
-                CHECK(value == 1 || value == 2 || value == 4 || value == 5 || value == 6 ||
-                          value == 9 || value == 12,
+                CHECK(value == 1 || value == 2 || (value >= 3 && value <= 6) || value == 9 ||
+                          value == 12,
                       DDFAN_EINVAL,
-                      "views per wave must be 1, 2 or 4 (tasks per wave 5, 6, 9 or 12 for the "
-                      "whole-row projector)");
+                      "views per wave must be 1, 2 or 4 (tasks per wave 3, 4, 5, 6, 9 or 12 for "
+                      "the whole-row projector)");
                 P->viewsPerWave = value;
                 return DDFAN_OK;
             case DDFAN_PARAM_PROJ_CHUNK:

@@ -64,7 +64,7 @@ struct ProjJob {
     GeomDev g;
     Subset sub;
     int nzc, ZT, slabTile, epi;
-    int V;  // views per wave: 1, 2 or 4 (stages 0, 1); tasks per wave 5, 6, 9 or 12 (stages 2, 3)
+    int V;  // views per wave: 1, 2 or 4 (stages 0, 1); tasks per wave 3-6, 9 or 12 (stages 2, 3)
     const float2* cossin;
     const float* bx;
     const float* by;
@@ -130,7 +130,7 @@ hipError_t launch_back(int mode, int ZT, int epi, const GeomDev& g, int nzc, con
 hipError_t launch_counter_inc(int* counter, hipStream_t st);
 // The projector launch shape fits the geometry: stages 0 and 1 take 1, 2 or 4
 // views per wave; the whole-row kernel (stage 2: 4 slices per block, 3: 2)
-// takes 5, 6, 9 or 12 tasks per wave, a slice chunk that is a multiple of its
+// takes 3, 4, 5, 6, 9 or 12 tasks per wave, a slice chunk that is a multiple of its
 // slices and rows of at most 776 entries.
 bool proj_shape_ok(int mode, const GeomDev& g, int ZT, int V, int stage);
 hipError_t launch_fill(float* p, float v, size_t n, hipStream_t st);

@@ -1157,11 +1157,13 @@ void k_proj_lds(const ProjLaunch L) {
 // order are those of k_proj for the same slice chunk, so the results are
 // bit-identical.  Rows of up to KW entries (images up to KW - 1 voxels wide).
 // Waves per SIMD: the blocks the two staging buffers leave room for in 160 KiB
-// of LDS, times NW / 4, at most 4 (the register budget of 12 tasks).
+// of LDS, times NW / 4, at most 4 (the register budget of 5-12 tasks) or 6 in
+// blocks of 12 waves (3-4 tasks).
 constexpr int proj_rows_waves(int MODE, int ZS, int KW, int NW) {
-    return (160 * 1024) / (2 * ZS * KW * (MODE == SAT_F32 ? 8 : 16)) * NW / 4 < 4
+    return (160 * 1024) / (2 * ZS * KW * (MODE == SAT_F32 ? 8 : 16)) * NW / 4 <
+                   (NW == 12 ? 6 : 4)
                ? (160 * 1024) / (2 * ZS * KW * (MODE == SAT_F32 ? 8 : 16)) * NW / 4
-               : 4;
+               : (NW == 12 ? 6 : 4);
 }
 template <int MODE, int ZS, int T, int KW, int NW>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(proj_rows_waves(MODE, ZS, KW, NW))))
@@ -1282,7 +1284,7 @@ void k_proj_rows(const ProjLaunch L) {
         const E* rows = tab + (size_t)ii * ZT * K;
 #pragma unroll
         for (int q = 0; q < ZS; ++q)
-            for (int cp = (wave - q * ncp) & (NW - 1); cp < ncp; cp += NW) {
+            for (int cp = ((wave - q * ncp) % NW + NW) % NW; cp < ncp; cp += NW) {
                 const int e0 = cp * 64 * PER;
                 stage_row(rows + (size_t)q * K + e0, (DDF_LDS void*)&buf[q * KW + e0], K - e0,
                           lane);
@@ -1697,8 +1699,8 @@ hipError_t launch_sat_sino(int mode, int ZT, int DNU, int SLN, int nzc, int PNs,
 }
 
 // Shapes of k_proj_rows by the plan's tasks-per-wave value: 9 or 12 tasks in
-// blocks of 4 waves, 5 or 6 tasks in blocks of 8 waves.
-static int proj_rows_nw(int V) { return V >= 9 ? 4 : 8; }
+// blocks of 4 waves, 5 or 6 in blocks of 8, 3 or 4 in blocks of 12.
+static int proj_rows_nw(int V) { return V >= 9 ? 4 : V >= 5 ? 8 : 12; }
 static int proj_rows_t(int V) { return V; }
 
 // Blocks per slice chunk of k_proj_rows: the subset's runs of views of one slab
@@ -1739,15 +1741,16 @@ static int proj_rows_kw(int mode, int K) {
 }
 
 // Launch shape checks of the projector kernels: stage 0/1 take 1, 2 or 4 views
-// per wave; the whole-row kernel (stage 2: 4 slices per block, 3: 2) takes 5,
-// 6, 9 or 12 tasks per wave, a slice chunk that is a multiple of its slices,
+// per wave; the whole-row kernel (stage 2: 4 slices per block, 3: 2) takes 3,
+// 4, 5, 6, 9 or 12 tasks per wave, a slice chunk that is a multiple of its slices,
 // rows within its capacity and a view block of at least one view.
 bool proj_shape_ok(int mode, const GeomDev& g, int ZT, int V, int stage) {
     if (stage == 0 || stage == 1) return V == 1 || V == 2 || V == 4;
     if (stage != 2 && stage != 3) return false;
     const int ZS = stage == 2 ? 4 : 2;
     const int nug = (g.DNU + 62) / 63;
-    return (V == 5 || V == 6 || V == 9 || V == 12) && ZT >= ZS && ZT % ZS == 0 &&
+    return (V == 3 || V == 4 || V == 5 || V == 6 || V == 9 || V == 12) && ZT >= ZS &&
+           ZT % ZS == 0 &&
            proj_rows_kw(mode, (g.XN > g.YN ? g.XN : g.YN) + 1) > 0 &&
            proj_rows_nw(V) * proj_rows_t(V) >= nug;
 }
@@ -1829,8 +1832,12 @@ static hipError_t launch_proj_group(int mode, int ZT, int V, int stage,
         DDF_ROWS(ZSC, 9, 4)                                                                \
     } else if (V == 6) {                                                                   \
         DDF_ROWS(ZSC, 6, 8)                                                                \
-    } else {                                                                               \
+    } else if (V == 5) {                                                                   \
         DDF_ROWS(ZSC, 5, 8)                                                                \
+    } else if (V == 4) {                                                                   \
+        DDF_ROWS(ZSC, 4, 12)                                                               \
+    } else {                                                                               \
+        DDF_ROWS(ZSC, 3, 12)                                                               \
     }
         if (rows) {
             if (ZS == 4) {
@@ -1865,7 +1872,7 @@ hipError_t launch_proj_batch(int mode, const ProjJob* jobs, int n, hipStream_t s
                                                     jobs[i].stage))
             return hipErrorInvalidValue;
     for (int stage : {0, 1, 2, 3})
-        for (int V : {1, 2, 4, 5, 6, 9, 12}) {
+        for (int V : {1, 2, 3, 4, 5, 6, 9, 12}) {
             for (int ZT : {1, 2, 4, 8, 16}) {
                 const ProjJob* grp[kBatch];
                 int m = 0;

@@ -105,7 +105,7 @@ struct ddfan_plan {
     int ZT = 1, nzc = 1;   // projector slice chunk and chunk count
     int ZTb = 1, nzcb = 1; // backprojector slice chunk and chunk count
     int slabTile = 0;      // projector slabs per pass (0: all slabs in one pass)
-    int viewsPerWave = 4;  // projector views per wave (1, 2 or 4); tasks per wave (5, 6, 9, 12) at stage >= 2
+    int viewsPerWave = 4;  // projector views per wave (1, 2 or 4); tasks per wave (3-6, 9, 12) at stage >= 2
     int backStage = 0;     // backprojector views per LDS-staged group (0: per-lane gathers)
     int projStage = 0;     // 0: per-lane gathers, 1: staged runs, 2/3: whole rows (4/2 slices)
     int backPass = 0;      // backprojector views per pass of a plain call (0: one pass)

@@ -157,7 +157,7 @@ int ddfan_plan_autotune(ddfan_plan* P, int views, int reps, int flags, int* out_
                     for (int ps : {1, 0}) shapes.push_back({v, ps});
                 if (stride > 1)
                     for (int ps : {2, 3})
-                        for (int v : {12, 9, 6, 5}) shapes.push_back({v, ps});
+                        for (int v : {12, 9, 6, 5, 4, 3}) shapes.push_back({v, ps});
                 for (int z : chunks) {
                     TRY(ddfan_plan_set_param(P, DDFAN_PARAM_PROJ_CHUNK, z));
                     for (const auto& sh : shapes) {
@@ -327,7 +327,7 @@ int ddfan_tune_load(const char* path, int* entries) {
             const bool pshape = (v.pstage == 0 || v.pstage == 1)
                                     ? (v.V == 1 || v.V == 2 || v.V == 4)
                                     : (v.pstage == 2 || v.pstage == 3) &&
-                                          (v.V == 5 || v.V == 6 || v.V == 9 || v.V == 12) &&
+                                          ((v.V >= 3 && v.V <= 6) || v.V == 9 || v.V == 12) &&
                                           v.ZT % (v.pstage == 2 ? 4 : 2) == 0;
             if (!pow2 || v.ZT > k.ZTmax || v.ZTb > k.ZTmax || !pshape || v.tile < 0 ||
                 !stage_fits(v.stage, v.ZTb)) {

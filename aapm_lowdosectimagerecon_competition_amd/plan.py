@@ -184,7 +184,7 @@ class Plan:
         """Projector kernel: 0 per-lane table reads, 1 table runs staged through
         LDS, 2 / 3 whole table rows in LDS with 4 / 2 slices per block (for
         spaced views; views_per_wave then counts tasks per wave: 9 or 12 in
-        blocks of 4 waves, 5 or 6 in blocks of 8)."""
+        blocks of 4 waves, 5 or 6 in blocks of 8, 3 or 4 in blocks of 12)."""
         return self.get_param(N.PARAM_PROJ_STAGE)
 
     @proj_stage.setter

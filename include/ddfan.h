@@ -164,7 +164,8 @@ int ddfan_plan_query(const ddfan_plan* plan, int* slice_chunk, int* n_chunks, in
 /* DDFAN_PARAM_VIEWS_PER_WAVE: consecutive views one projector wave handles
  * (1, 2 or 4); they share table rows through L1.  With DDFAN_PARAM_PROJ_STAGE
  * 2 or 3 it counts the whole-row projector's tasks (view x 63 detectors) per
- * wave instead: 9 or 12 in blocks of 4 waves, 5 or 6 in blocks of 8. */
+ * wave instead: 9 or 12 in blocks of 4 waves, 5 or 6 in blocks of 8, 3 or 4 in
+ * blocks of 12. */
 #define DDFAN_PARAM_VIEWS_PER_WAVE 2
 /* DDFAN_PARAM_PROJ_CHUNK / DDFAN_PARAM_BACK_CHUNK: slices per chunk (a power
  * of two up to the widest chunk whose angles agree) of the projector and of

@@ -163,7 +163,9 @@ def test_launch_shape_table_round_trip(tmp_path):
                  "1 512 512 64 736 16 2304  8 3 0 16 1 1  1 1",   # 3 views per wave
                  "1 512 512 64 736 16 2304  8 4 0 16 4 1  1 1",   # 4 staged views x chunk 16 > 32
                  "1 512 512 64 736 16 2304  8 4 0 16 3 1  1 1",   # 3 staged views
-                 "1 512 512 64 736 16 2304  8 4 0 16 1 2  1 1",   # projector stage 2
+                 "1 512 512 64 736 16 2304  8 4 0 16 1 4  1 1",   # projector stage 4
+                 "1 512 512 64 736 16 2304  8 2 0 16 1 2  1 1",   # whole rows: 2 tasks per wave
+                 "1 512 512 64 736 16 2304  2 9 0 16 1 2  1 1",   # whole rows: 4 slices, chunk 2
                  "1 512 512 64 736 16 2304  8 4 0 16 1  1.5 2"):  # no projector stage column
         bad.write_text(line + "\n")
         with pytest.raises(N.DDFanError):

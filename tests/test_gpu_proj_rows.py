@@ -59,7 +59,7 @@ def test_rows_projector_equals_gather(sat, name, kw, PN):
                 for st in (2, 3):
                     if zt % (4 if st == 2 else 2):
                         continue
-                    for T in (12, 9, 6, 5):
+                    for T in (12, 9, 6, 5, 4, 3):
                         plan.views_per_wave = T
                         plan.proj_stage = st
                         got = _calls(plan, vol, PN)
@@ -104,7 +104,7 @@ def test_rows_projector_full_size_subset():
             plan.proj_stage = 0
             plan.views_per_wave = 4
             ref = plan.proj_host(vol, first=first, stride=24, count=96)
-            for st, T in ((2, 9), (2, 5), (3, 6), (3, 12)):
+            for st, T in ((2, 9), (2, 5), (2, 3), (3, 6), (3, 4), (3, 12)):
                 plan.views_per_wave = T
                 plan.proj_stage = st
                 got = plan.proj_host(vol, first=first, stride=24, count=96)
@@ -145,12 +145,12 @@ def test_rows_projector_shape_checks():
         for bad in (4, -1):
             with pytest.raises(N.DDFanError):
                 plan.proj_stage = bad
-        for bad in (3, 7, 16):
+        for bad in (0, 7, 16):
             with pytest.raises(N.DDFanError):
                 plan.views_per_wave = bad
         plan.proj_chunk = 8
-        plan.views_per_wave = 4
-        plan.proj_stage = 2            # 4 views per wave: not a whole-row shape
+        plan.views_per_wave = 2
+        plan.proj_stage = 2            # 2 per wave: not a whole-row shape
         with pytest.raises(N.DDFanError):
             plan.proj_host(vol)
         plan.views_per_wave = 9
