@@ -111,6 +111,7 @@ _SIGS = {
     "ddfan_batch_add": [_p, _p, _p, _p, _i, _i, _i, _ip],
     "ddfan_batch_size": [_p, _ip],
     "ddfan_batch_set_param": [_p, _i, _ll],
+    "ddfan_batch_tune": [_p, _i, ctypes.POINTER(ctypes.c_double)],
     "ddfan_batch_run": [_p, _i],
     "ddfan_batch_destroy": [_p],
     "ddfan_server_create": [_sz, _i, _i, ctypes.POINTER(_p)],

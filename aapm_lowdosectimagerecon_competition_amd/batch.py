@@ -46,6 +46,16 @@ class Batch:
         every reconstruction is small, else one)."""
         N.call("ddfan_batch_set_param", self.handle, N.BATCH_STREAMS, int(n))
 
+    def tune(self, reps=2) -> dict:
+        """Throughput launch shapes for the reconstructions that share launches
+        (``ddfan_batch_tune``): one projector and one backprojector shape for
+        all of them, timed on their shared launches.  Returns the launch time
+        per subset round before and after, in ms."""
+        ms = (ctypes.c_double * 4)()
+        N.call("ddfan_batch_tune", self.handle, int(reps), ms)
+        return {"proj_ms_before": ms[0], "back_ms_before": ms[1],
+                "proj_ms": ms[2], "back_ms": ms[3]}
+
     def __len__(self):
         n = ctypes.c_int()
         N.call("ddfan_batch_size", self.handle, ctypes.byref(n))

@@ -358,9 +358,257 @@ int run_groups(ddfan_batch* B, int ng, bool graph) {
     return DDFAN_OK;
 }
 
+// Median of `reps` timed runs of f on `st` (after one untimed run), in ms.
+template <class F>
+int median_ms(hipStream_t st, int reps, F&& f, double* ms) {
+    hipEvent_t a = nullptr, b = nullptr;
+    HIPCALL(hipEventCreate(&a));
+    struct EvPair {
+        hipEvent_t& a;
+        hipEvent_t& b;
+        ~EvPair() {
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
+        }
+    } guard{a, b};
+    HIPCALL(hipEventCreate(&b));
+    HIPCALL(f());
+    std::vector<double> t;
+    for (int r = 0; r < reps; ++r) {
+        HIPCALL(hipEventRecord(a, st));
+        HIPCALL(f());
+        HIPCALL(hipEventRecord(b, st));
+        HIPCALL(hipEventSynchronize(b));
+        float x = 0.0f;
+        HIPCALL(hipEventElapsedTime(&x, a, b));
+        t.push_back(x);
+    }
+    std::sort(t.begin(), t.end());
+    *ms = t[t.size() / 2];
+    return DDFAN_OK;
+}
+
+// Launch parameters of one plan (restored if batch tuning fails part-way).
+struct Shape {
+    int ZT, V, pstage, ZTb, bstage;
+};
+Shape shape_of(const ddfan_plan* P) {
+    return Shape{P->ZT, P->viewsPerWave, P->projStage, P->ZTb, P->backStage};
+}
+int set_shape(ddfan_plan* P, const Shape& s) {
+    TRY(ddfan_plan_set_param(P, DDFAN_PARAM_PROJ_CHUNK, s.ZT));
+    TRY(ddfan_plan_set_param(P, DDFAN_PARAM_VIEWS_PER_WAVE, s.V));
+    TRY(ddfan_plan_set_param(P, DDFAN_PARAM_PROJ_STAGE, s.pstage));
+    TRY(ddfan_plan_set_param(P, DDFAN_PARAM_BACK_CHUNK, s.ZTb));
+    TRY(ddfan_plan_set_param(P, DDFAN_PARAM_BACK_STAGE, s.bstage));
+    return DDFAN_OK;
+}
+
+// Throughput shapes of the shared launches.  A plan's own autotuning times
+// its launches alone, where a small reconstruction leaves most of the GPU idle
+// and the shape with the most waves wins even if each wave does little; in a
+// batch the shared launches fill the GPU together, so work per wave decides.
+// This times the shared reconstructions' launches together (volume tables +
+// projector, sinogram table + backprojector, on two of each one's subsets)
+// for one projector shape (slice chunk, views or tasks per wave, staging) and
+// one backprojector shape (slice chunk, staged views) applied to all of them,
+// keeps the fastest, and applies it when it beats the shapes in place.
+// Outputs go to scratch buffers; the iterates are only read.  Slab tiles,
+// which group the projector's sums, are left as they are, so results do not
+// change.
+int batch_tune(ddfan_batch* B, int reps, double* out_ms) {
+    std::vector<OsJob*> sh;
+    for (OsJob& J : B->jobs)
+        if ((long long)(J.P->satX.bytes + J.P->satY.bytes) < B->shape.soloBytes) sh.push_back(&J);
+    const int m = (int)sh.size();
+    if (out_ms)
+        for (int i = 0; i < 4; ++i) out_ms[i] = 0.0;
+    if (m == 0) return DDFAN_OK;
+    const int mode = B->mode;
+    hipStream_t st = B->stream;
+    std::vector<Shape> saved(m);
+    std::vector<std::vector<int>> ks(m), cnt(m);  // timed subsets and their view counts
+    std::vector<DBuf> sino(m), vol(m);
+    int ZTcap = 16;
+    for (int i = 0; i < m; ++i) {
+        ddfan_plan* P = sh[i]->P;
+        saved[i] = shape_of(P);
+        ZTcap = std::min(ZTcap, P->ZTmax);
+        const int nO = sh[i]->numOS;
+        ks[i] = nO > 1 ? std::vector<int>{0, nO / 2} : std::vector<int>{0};
+        int maxc = 0;
+        for (int k : ks[i]) {
+            cnt[i].push_back(subset_views(P->PN, nO, k, nullptr));
+            maxc = std::max(maxc, cnt[i].back());
+        }
+        const GeomDev& g = P->g;
+        HIPCALL(sino[i].reserve(sizeof(float) * (size_t)maxc * g.DNU * g.SLN));
+        HIPCALL(vol[i].reserve(sizeof(float) * (size_t)g.XN * g.YN * g.SLN));
+        HIPCALL(launch_fill(sino[i].as<float>(), 0.0f, (size_t)maxc * g.DNU * g.SLN, st));
+    }
+    int nsub = 0;
+    for (int i = 0; i < m; ++i) nsub = std::max(nsub, (int)ks[i].size());
+    auto proj = [&]() -> hipError_t {
+        std::vector<SatVolJob> sv;
+        std::vector<ProjJob> pj;
+        for (int s = 0; s < nsub; ++s) {
+            sv.clear();
+            pj.clear();
+            for (int i = 0; i < m; ++i) {
+                if (s >= (int)ks[i].size()) continue;
+                ddfan_plan* P = sh[i]->P;
+                const Subset sub{ks[i][s], sh[i]->numOS, cnt[i][s]};
+                sv.push_back(SatVolJob{P->g, P->nzc, P->ZT, sh[i]->x, P->mask.as<uint8_t>(),
+                                       P->satX.p, P->satY.p});
+                pj.push_back(ProjJob{P->g, sub, P->nzc, P->ZT, P->slabTile, PROJ_PLAIN,
+                                     P->viewsPerWave, P->cs.as<float2>(), P->bx.as<float>(),
+                                     P->by.as<float>(), P->xds.as<float>(), P->yds.as<float>(),
+                                     P->satX.p, P->satY.p, sino[i].as<float>(), nullptr, nullptr,
+                                     P->projStage, P->h_cs.data()});
+            }
+            hipError_t e = launch_sat_vol_batch(mode, sv.data(), (int)sv.size(), st);
+            if (e == hipSuccess)
+                e = launch_proj_batch(mode, pj.data(), (int)pj.size(), st, B->shape.projFill);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    };
+    auto back = [&]() -> hipError_t {
+        std::vector<SatSinoJob> ss;
+        std::vector<BackJob> bj;
+        for (int s = 0; s < nsub; ++s) {
+            ss.clear();
+            bj.clear();
+            for (int i = 0; i < m; ++i) {
+                if (s >= (int)ks[i].size()) continue;
+                ddfan_plan* P = sh[i]->P;
+                const Subset sub{ks[i][s], sh[i]->numOS, cnt[i][s]};
+                ss.push_back(SatSinoJob{P->g.DNU, P->g.SLN, P->nzcb, P->ZTb, cnt[i][s],
+                                        sino[i].as<float>(), P->satP.p});
+                bj.push_back(BackJob{P->g, sub, P->nzcb, P->ZTb, BACK_PLAIN, P->cs.as<float2>(),
+                                     P->mask.as<uint8_t>(), P->satP.p, vol[i].as<float>(), nullptr,
+                                     nullptr, nullptr, nullptr, P->backStage});
+            }
+            hipError_t e = launch_sat_sino_batch(mode, ss.data(), (int)ss.size(), st);
+            if (e == hipSuccess)
+                e = launch_back_batch(mode, bj.data(), (int)bj.size(), st, B->shape.backFill);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    };
+    // Tables sized for the current chunks of every shared plan.
+    auto fit_tables = [&]() -> int {
+        for (int i = 0; i < m; ++i) {
+            ddfan_plan* P = sh[i]->P;
+            TRY(ensure_vol_sat(P));
+            int maxc = 0;
+            for (int c : cnt[i]) maxc = std::max(maxc, c);
+            const size_t b = sat_sino_bytes(mode, P->nzcb, P->ZTb, P->g.DNU, maxc);
+            if (b > P->satP.bytes) {
+                HIPCALL(hipStreamSynchronize(st));
+                HIPCALL(P->satP.reserve(b));
+            }
+        }
+        return DDFAN_OK;
+    };
+    int rc = [&]() -> int {
+        TRY(fit_tables());
+        double p0, b0;
+        TRY(median_ms(st, reps, proj, &p0));
+        TRY(median_ms(st, reps, back, &b0));
+        double bestP = p0, bestB = b0;
+        Shape best = shape_of(sh[0]->P);
+        bool projUniform = false, backUniform = false;
+        std::vector<int> chunks;
+        for (int z = ZTcap; z >= 1; z >>= 1) chunks.push_back(z);
+        std::vector<std::pair<int, int>> shapes;  // (views or tasks per wave, stage)
+        for (int v : {4, 2, 1})
+            for (int ps : {1, 0}) shapes.push_back({v, ps});
+        for (int ps : {2, 3})
+            for (int v : {12, 9, 6, 5, 4, 3}) shapes.push_back({v, ps});
+        for (int z : chunks)
+            for (const auto& s : shapes) {
+                bool ok = true;
+                for (int i = 0; i < m && ok; ++i) ok = proj_shape_ok(mode, sh[i]->P->g, z, s.first, s.second);
+                if (!ok) continue;
+                for (int i = 0; i < m; ++i) {
+                    ddfan_plan* P = sh[i]->P;
+                    TRY(ddfan_plan_set_param(P, DDFAN_PARAM_PROJ_CHUNK, z));
+                    TRY(ddfan_plan_set_param(P, DDFAN_PARAM_VIEWS_PER_WAVE, s.first));
+                    TRY(ddfan_plan_set_param(P, DDFAN_PARAM_PROJ_STAGE, s.second));
+                }
+                TRY(fit_tables());
+                double ms;
+                TRY(median_ms(st, reps, proj, &ms));
+                if (ms < bestP) {
+                    bestP = ms;
+                    best.ZT = z;
+                    best.V = s.first;
+                    best.pstage = s.second;
+                    projUniform = true;
+                }
+            }
+        for (int i = 0; i < m; ++i) {  // projector: the winner, or each plan's own shape
+            ddfan_plan* P = sh[i]->P;
+            const Shape& s = projUniform ? best : saved[i];
+            TRY(ddfan_plan_set_param(P, DDFAN_PARAM_PROJ_CHUNK, s.ZT));
+            TRY(ddfan_plan_set_param(P, DDFAN_PARAM_VIEWS_PER_WAVE, s.V));
+            TRY(ddfan_plan_set_param(P, DDFAN_PARAM_PROJ_STAGE, s.pstage));
+        }
+        TRY(fit_tables());
+        TRY(proj());  // a projection of each iterate into the sinograms the backprojector reads
+        for (int z : chunks)
+            for (int bs : {1, 2, 4, 0}) {
+                if (bs * z > 32) continue;
+                for (int i = 0; i < m; ++i) {
+                    ddfan_plan* P = sh[i]->P;
+                    TRY(ddfan_plan_set_param(P, DDFAN_PARAM_BACK_CHUNK, z));
+                    TRY(ddfan_plan_set_param(P, DDFAN_PARAM_BACK_STAGE, bs));
+                }
+                TRY(fit_tables());
+                double ms;
+                TRY(median_ms(st, reps, back, &ms));
+                if (ms < bestB) {
+                    bestB = ms;
+                    best.ZTb = z;
+                    best.bstage = bs;
+                    backUniform = true;
+                }
+            }
+        for (int i = 0; i < m; ++i) {
+            ddfan_plan* P = sh[i]->P;
+            const Shape& s = backUniform ? best : saved[i];
+            TRY(ddfan_plan_set_param(P, DDFAN_PARAM_BACK_CHUNK, s.ZTb));
+            TRY(ddfan_plan_set_param(P, DDFAN_PARAM_BACK_STAGE, s.bstage));
+        }
+        TRY(fit_tables());
+        HIPCALL(hipStreamSynchronize(st));
+        if (out_ms) {
+            out_ms[0] = p0 / nsub;
+            out_ms[1] = b0 / nsub;
+            out_ms[2] = bestP / nsub;
+            out_ms[3] = bestB / nsub;
+        }
+        return DDFAN_OK;
+    }();
+    (void)hipStreamSynchronize(st);  // before the scratch buffers are freed
+    if (rc != DDFAN_OK)
+        for (int i = 0; i < m; ++i) (void)set_shape(sh[i]->P, saved[i]);
+    return rc;
+}
+
 }  // namespace
 
 extern "C" {
+
+int ddfan_batch_tune(ddfan_batch* B, int reps, double* out_ms) {
+    return ddf::guarded([&]() -> int {
+        CHECK(B, DDFAN_EINVAL, "batch is NULL");
+        HIPCALL(hipSetDevice(B->device));
+        reps = std::max(1, std::min(reps, 20));
+        return batch_tune(B, reps, out_ms);
+    });
+}
 
 int ddfan_ossart(ddfan_plan* P, const float* d_meas, float* d_x, int numIter, int numOS,
                  int useFISTA, int flags) {

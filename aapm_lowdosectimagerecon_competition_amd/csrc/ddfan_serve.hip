@@ -182,6 +182,14 @@ struct ddfan_server {
     bool stop = false, reaperStop = false;
     ddf::ServerHooks hooks;  // how the worker checks GPU completion (ddfan_serve_hooks.h)
     bool graphs_ = false;    // DDFAN_SERVER_GRAPH: replay each membership's iteration as a hipGraph
+    // With graphs on, the worker's stream capture and graph instantiation, the
+    // reaper's frees (hipFree waits for all device work, the capturing stream
+    // included) and the submit path's allocations and copies hold this, so no
+    // other library thread calls the runtime while a capture is in progress.
+    std::mutex captureMu;
+    std::unique_lock<std::mutex> capture_guard() {
+        return graphs_ ? std::unique_lock<std::mutex>(captureMu) : std::unique_lock<std::mutex>();
+    }
     std::thread worker, reaper;
     // statistics
     long long completed = 0, failed = 0;
@@ -468,8 +476,11 @@ void ddfan_server::loop() {
                 while (g == graphIdx[0] || g == graphIdx[1]) ++g;
                 jobs.clear();
                 for (Request* R : running) jobs.push_back(&R->job);
-                const int rc =
-                    graphs[g].capture(jobs.data(), (int)jobs.size(), mode, shape, stream);
+                int rc;
+                {
+                    auto cg = capture_guard();
+                    rc = graphs[g].capture(jobs.data(), (int)jobs.size(), mode, shape, stream);
+                }
                 if (rc != DDFAN_OK) {
                     (void)hipStreamSynchronize(stream);
                     fail_all_running(rc, ddfan_last_error());
@@ -546,11 +557,14 @@ void ddfan_server::reap() {
         Dead d = dead.front();
         dead.pop_front();
         lk.unlock();
-        if (d.plan) (void)ddfan_plan_destroy(d.plan);
-        if (d.meas) (void)hipFree(d.meas);
-        if (d.x) (void)hipFree(d.x);
-        for (hipEvent_t e : d.ev)
-            if (e) (void)hipEventDestroy(e);
+        {
+            auto cg = capture_guard();
+            if (d.plan) (void)ddfan_plan_destroy(d.plan);
+            if (d.meas) (void)hipFree(d.meas);
+            if (d.x) (void)hipFree(d.x);
+            for (hipEvent_t e : d.ev)
+                if (e) (void)hipEventDestroy(e);
+        }
         lk.lock();
         adm.release(d.bytes);
         cvWork.notify_all();
@@ -668,7 +682,11 @@ int ddfan_server_submit(ddfan_server* S, const ddfan_geom* G, const float* hangs
         if (now) {
             // Admitted: copy the inputs from the caller's buffers to the device.
             R->tAdmit = Clock::now();
-            const int rc = upload_inputs(*R, meas, nm, x0, nv, S->device, S->upStream);
+            int rc;
+            {
+                auto cg = S->capture_guard();
+                rc = upload_inputs(*R, meas, nm, x0, nv, S->device, S->upStream);
+            }
             std::lock_guard<std::mutex> lk(S->mu);
             if (rc != DDFAN_OK) {
                 S->adm.release(bytes);

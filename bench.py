@@ -256,12 +256,14 @@ def small_specs(n, seed):
              "I0": float(np.exp(rng.uniform(np.log(5e4), np.log(5e5))))} for _ in range(n)]
 
 
-def bench_batch(dd, sat, n=16, iters=10, seed=3, specs=None, streams=(0, 1, 2, 4)):
-    """Uneven reconstructions: one at a time (device OS-SART, graph) against
-    one batch (shared launches, graph per batch iteration), with the jobs in
-    the automatic grouping (0) and in 1, 2 or 4 groups side by side on their
-    own streams; results must agree bit for bit.  `batch_s` is the automatic
-    grouping's time."""
+def bench_batch(dd, sat, n=16, iters=10, seed=3, specs=None, streams=(0, 1, 2, 4), untuned=True):
+    """Uneven reconstructions: one at a time (device OS-SART, graph, each plan's
+    own autotuned shapes) against one batch (shared launches, graph per batch
+    iteration), with the jobs in the automatic grouping (0) and in 1, 2 or 4
+    groups side by side on their own streams; results must agree bit for bit.
+    With `untuned`, the batch first runs in one stream with the shapes in
+    place; then it tunes its shared launches for throughput (ddfan_batch_tune, outside the
+    timed runs).  `batch_s` is the automatic grouping's time."""
     import numpy as np
     log(f"[bench] batch: generating {n} jobs")
     specs = specs or mix_specs(n, seed)
@@ -277,14 +279,20 @@ def bench_batch(dd, sat, n=16, iters=10, seed=3, specs=None, streams=(0, 1, 2, 4
         ref.append(j.result())
     runs = {}
     same = True
-    for ns in streams:
+    tuned = None
+    for ns in (("untuned",) if untuned else ()) + tuple(streams):
         for j in jobs:
             j.reset()
         log(f"[bench] batch: one batch, {ns} stream(s)")
         with dd.Batch(sat) as B:
-            B.set_streams(ns)
+            B.set_streams(1 if ns == "untuned" else ns)
             for j in jobs:
                 B.add(j.plan, j.meas, j.x, j.iters, j.nos, 1)
+            if ns != "untuned" and tuned is None:
+                t0 = time.perf_counter()
+                tuned = B.tune(2)
+                tuned["tune_s"] = time.perf_counter() - t0
+                log(f"[bench] batch: shared launches tuned {tuned}")
             t0 = time.perf_counter()
             B.run(graph=True)
             runs[ns] = time.perf_counter() - t0
@@ -292,7 +300,10 @@ def bench_batch(dd, sat, n=16, iters=10, seed=3, specs=None, streams=(0, 1, 2, 4
     out = {"jobs": n, "iters": iters, "subset_views": 48,
            "sizes": [f"{sp['XN']}x{sp['XN']}x{sp['SLN']}x{sp['PN']}" for sp in specs],
            "sequential_s": sum(alone_s), "batch_s": runs[0],
-           "batch_s_by_streams": {("auto" if k == 0 else str(k)): v for k, v in runs.items()},
+           "batch_s_by_streams": {("auto" if k == 0 else str(k)): v for k, v in runs.items()
+                                  if k != "untuned"},
+           "batch_s_untuned_one_stream": runs.get("untuned"),
+           "shared_launch_tuning": tuned,
            "alone_s": alone_s,
            "speedup_vs_sequential": sum(alone_s) / runs[0],
            "voxel_views_per_s": vv / runs[0], "bitwise_equal_to_alone": bool(same),
@@ -743,7 +754,7 @@ def main():
         for j in small:
             j.close()
         del small
-        jobs, alone_s, out["batch"] = bench_batch(dd, args.sat)
+        jobs, alone_s, out["batch"] = bench_batch(dd, args.sat, untuned=False)
         log(f"[bench] batch {out['batch']}")
         big = big_request(dd, args.sat)
         out["serve"] = bench_serve(dd, jobs, alone_s, args.sat, load=args.serve_load,

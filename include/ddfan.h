@@ -288,6 +288,17 @@ int ddfan_batch_size(const ddfan_batch* batch, int* n);
  * DDFAN_BATCH_SOLO_BYTES, else 1. */
 #define DDFAN_BATCH_STREAMS 4
 int ddfan_batch_set_param(ddfan_batch* batch, int param, long long value);
+/* Throughput launch shapes for the reconstructions that share launches (volume
+ * tables below DDFAN_BATCH_SOLO_BYTES).  A plan's own autotuning times it
+ * alone, where a small reconstruction leaves most of the GPU idle; here one
+ * projector shape (slice chunk, views or tasks per wave, staging) and one
+ * backprojector shape (slice chunk, staged views) for all of them are timed on
+ * their shared launches (median of reps, on two subsets of each) and the
+ * fastest is applied to their plans when it beats the shapes in place.
+ * Iterates are only read, slab tiles are kept, results do not change.
+ * out_ms (4 doubles, may be NULL): projector and backprojector launch time per
+ * subset round before, then after. */
+int ddfan_batch_tune(ddfan_batch* batch, int reps, double* out_ms);
 /* Runs every reconstruction to completion (flags: DDFAN_OSSART_GRAPH). */
 int ddfan_batch_run(ddfan_batch* batch, int flags);
 int ddfan_batch_destroy(ddfan_batch* batch);
