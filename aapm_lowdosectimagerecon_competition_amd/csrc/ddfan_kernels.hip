@@ -271,17 +271,23 @@ __global__ __launch_bounds__(256) void k_sat_rows(const SatLaunch L) {
 // leave through LDS as contiguous runs of each slice's row.  The summation
 // tree depends only on G, which the launcher picks from the table's own
 // slice count, so a table's entries do not depend on what shares its launch.
+// The chunk's samples (tin) are dead once scanned, before its entries (tout)
+// are written, so the two share LDS: 19.2 KB per block at G = 32 (two-float),
+// eight blocks per CU, with the register budget of eight waves per SIMD.
 template <int MODE, int G>
-__global__ __launch_bounds__(256) void k_sat_tiles(const SatLaunch L) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
+void k_sat_tiles(const SatLaunch L) {
     using S = Scan<MODE>;
     using Lt = typename S::L;
     using E = typename Sat<MODE>::E;
     constexpr int P = 256 / G;  // parts per slice
     constexpr int LP = 4;       // samples per part
     constexpr int KC = P * LP;  // samples per chunk
-    __shared__ float tin[KC][G + 1];
+    constexpr size_t kIn = sizeof(float) * KC * (G + 1), kOut = sizeof(E) * G * (KC + 1);
+    __shared__ __align__(16) unsigned char tio[kIn > kOut ? kIn : kOut];
+    float(*const tin)[G + 1] = reinterpret_cast<float(*)[G + 1]>(tio);
+    E(*const tout)[KC + 1] = reinterpret_cast<E(*)[KC + 1]>(tio);
     __shared__ Lt tot[P][G];
-    __shared__ E tout[G][KC + 1];
     __shared__ Lt carry[G];
     const int bid = xcd_linear(blockIdx.x, gridDim.x);
     if (bid >= L.nwork) return;  // whole block
@@ -791,17 +797,28 @@ __device__ __forceinline__ int quad_max(int v) {
     return max(v, __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false));
 }
 
-// Blocks per CU the two staging buffers (2 * NV * ZT KiB) leave room for in
-// 160 KiB of LDS; each block puts one wave on each SIMD.
-constexpr int back_lds_waves(int NV, int ZT) {
-    return 160 / (2 * NV * ZT) < 8 ? 160 / (2 * NV * ZT) : 8;
+// Staged rows hold 64 entries: 1 KiB of two-float or f64 entries, 512 bytes of
+// f32 ones.  Blocks per CU the two staging buffers (2 * NV * ZT rows) leave
+// room for in 160 KiB of LDS, each block putting one wave on each SIMD, at
+// most 8 (6 for f32 rows, whose small buffers would otherwise leave the
+// compiler a 64-register budget).  Measured on MI355X at 512x512x64, 2304
+// views, 16-slice chunks, one view per group: f32 15.7 ms with 128-entry rows
+// (5 waves), 14.9 ms with 64-entry rows (6 waves); two-float 16.9 ms with
+// 64-entry rows, 18.6 ms with 48 and 28.6 ms with 32 (tiles whose detector
+// footprint is wider than the row read their entries from global memory).
+constexpr int kBackWent = 64;
+constexpr int back_lds_waves(int MODE, int NV, int ZT) {
+    const int rowBytes = kBackWent * (MODE == SAT_F32 ? 8 : 16);
+    const int w = 160 * 1024 / (2 * NV * ZT * rowBytes);
+    const int cap = MODE == SAT_F32 ? 6 : 8;
+    return w < cap ? w : cap;
 }
 
 template <int MODE, int ZT, int NV>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(back_lds_waves(NV, ZT))))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(back_lds_waves(MODE, NV, ZT))))
 void k_back_lds(const BackLaunch L) {
     using E = typename Sat<MODE>::E;
-    constexpr int WENT = 1024 / (int)sizeof(E);  // entries per staged row
+    constexpr int WENT = kBackWent;  // entries per staged row
     constexpr int ROWS = NV * ZT;
     static_assert(NV >= 1 && NV <= 16, "four corner lanes per view");
     __shared__ __attribute__((aligned(16))) E bufA[ROWS * WENT];
@@ -872,26 +889,28 @@ void k_back_lds(const BackLaunch L) {
 #pragma unroll
             for (int v = 0; v < NV; ++v) w0[v] = __builtin_amdgcn_readlane(wl, 4 * ((j0 + v) & 15));
         };
-        // Row (view v, slice q) is copied by wave (v*ZT + q) mod 4.
+        // Row (view v, slice q) is copied by wave (v*ZT + q) mod 4: its WENT
+        // entries, or the rest of the table row if shorter.
         auto stage = [&](int grp, LE* buf, const int* w0) __attribute__((always_inline)) {
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 const int j = grp * NV + v;
                 if (j >= count) break;
                 const E* src = Pc + (size_t)j * viewStride + w0[v];
+                const int nrow = min(K - w0[v], WENT);
                 if constexpr (ZT % 4 == 0) {
 #pragma unroll
                     for (int i = 0; i < ZT / 4; ++i) {
                         const int q = 4 * i + wave;
                         stage_row(src + (size_t)q * K, (DDF_LDS void*)(buf + (v * ZT + q) * WENT),
-                                      K - w0[v], lane);
+                                      nrow, lane);
                     }
                 } else {
 #pragma unroll
                     for (int q = 0; q < ZT; ++q)
                         if (((v * ZT + q) & 3) == wave)
                             stage_row(src + (size_t)q * K,
-                                          (DDF_LDS void*)(buf + (v * ZT + q) * WENT), K - w0[v],
+                                          (DDF_LDS void*)(buf + (v * ZT + q) * WENT), nrow,
                                           lane);
                 }
             }

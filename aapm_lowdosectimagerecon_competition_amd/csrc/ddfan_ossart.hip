@@ -411,7 +411,8 @@ int set_shape(ddfan_plan* P, const Shape& s) {
 // This times the shared reconstructions' launches together (volume tables +
 // projector, sinogram table + backprojector, on two of each one's subsets)
 // for one projector shape (slice chunk, views or tasks per wave, staging) and
-// one backprojector shape (slice chunk, staged views) applied to all of them,
+// one backprojector shape (slice chunk, staged views) applied to all of them
+// (a plan whose slices allow a narrower chunk only takes that one),
 // keeps the fastest, and applies it when it beats the shapes in place.
 // Outputs go to scratch buffers; the iterates are only read.  Slab tiles,
 // which group the projector's sums, are left as they are, so results do not
@@ -429,11 +430,11 @@ int batch_tune(ddfan_batch* B, int reps, double* out_ms) {
     std::vector<Shape> saved(m);
     std::vector<std::vector<int>> ks(m), cnt(m);  // timed subsets and their view counts
     std::vector<DBuf> sino(m), vol(m);
-    int ZTcap = 16;
+    int ZTcap = 1;  // widest chunk any shared plan allows (each plan takes min(chunk, its own))
     for (int i = 0; i < m; ++i) {
         ddfan_plan* P = sh[i]->P;
         saved[i] = shape_of(P);
-        ZTcap = std::min(ZTcap, P->ZTmax);
+        ZTcap = std::max(ZTcap, std::min(P->ZTmax, 16));
         const int nO = sh[i]->numOS;
         ks[i] = nO > 1 ? std::vector<int>{0, nO / 2} : std::vector<int>{0};
         int maxc = 0;
@@ -526,14 +527,17 @@ int batch_tune(ddfan_batch* B, int reps, double* out_ms) {
             for (int ps : {1, 0}) shapes.push_back({v, ps});
         for (int ps : {2, 3})
             for (int v : {12, 9, 6, 5, 4, 3}) shapes.push_back({v, ps});
+        // a candidate chunk z gives each plan min(z, its widest chunk)
+        auto zc = [&](int i, int z) { return std::min(z, sh[i]->P->ZTmax); };
         for (int z : chunks)
             for (const auto& s : shapes) {
                 bool ok = true;
-                for (int i = 0; i < m && ok; ++i) ok = proj_shape_ok(mode, sh[i]->P->g, z, s.first, s.second);
+                for (int i = 0; i < m && ok; ++i)
+                    ok = proj_shape_ok(mode, sh[i]->P->g, zc(i, z), s.first, s.second);
                 if (!ok) continue;
                 for (int i = 0; i < m; ++i) {
                     ddfan_plan* P = sh[i]->P;
-                    TRY(ddfan_plan_set_param(P, DDFAN_PARAM_PROJ_CHUNK, z));
+                    TRY(ddfan_plan_set_param(P, DDFAN_PARAM_PROJ_CHUNK, zc(i, z)));
                     TRY(ddfan_plan_set_param(P, DDFAN_PARAM_VIEWS_PER_WAVE, s.first));
                     TRY(ddfan_plan_set_param(P, DDFAN_PARAM_PROJ_STAGE, s.second));
                 }
@@ -542,7 +546,7 @@ int batch_tune(ddfan_batch* B, int reps, double* out_ms) {
                 TRY(median_ms(st, reps, proj, &ms));
                 if (ms < bestP) {
                     bestP = ms;
-                    best.ZT = z;
+                    best.ZT = z;  // each plan: zc(i, best.ZT)
                     best.V = s.first;
                     best.pstage = s.second;
                     projUniform = true;
@@ -551,7 +555,7 @@ int batch_tune(ddfan_batch* B, int reps, double* out_ms) {
         for (int i = 0; i < m; ++i) {  // projector: the winner, or each plan's own shape
             ddfan_plan* P = sh[i]->P;
             const Shape& s = projUniform ? best : saved[i];
-            TRY(ddfan_plan_set_param(P, DDFAN_PARAM_PROJ_CHUNK, s.ZT));
+            TRY(ddfan_plan_set_param(P, DDFAN_PARAM_PROJ_CHUNK, projUniform ? zc(i, s.ZT) : s.ZT));
             TRY(ddfan_plan_set_param(P, DDFAN_PARAM_VIEWS_PER_WAVE, s.V));
             TRY(ddfan_plan_set_param(P, DDFAN_PARAM_PROJ_STAGE, s.pstage));
         }
@@ -559,10 +563,10 @@ int batch_tune(ddfan_batch* B, int reps, double* out_ms) {
         TRY(proj());  // a projection of each iterate into the sinograms the backprojector reads
         for (int z : chunks)
             for (int bs : {1, 2, 4, 0}) {
-                if (bs * z > 32) continue;
+                if (bs * z > 32) continue;  // also bs * zc(i, z) <= 32 for every plan
                 for (int i = 0; i < m; ++i) {
                     ddfan_plan* P = sh[i]->P;
-                    TRY(ddfan_plan_set_param(P, DDFAN_PARAM_BACK_CHUNK, z));
+                    TRY(ddfan_plan_set_param(P, DDFAN_PARAM_BACK_CHUNK, zc(i, z)));
                     TRY(ddfan_plan_set_param(P, DDFAN_PARAM_BACK_STAGE, bs));
                 }
                 TRY(fit_tables());
@@ -578,7 +582,7 @@ int batch_tune(ddfan_batch* B, int reps, double* out_ms) {
         for (int i = 0; i < m; ++i) {
             ddfan_plan* P = sh[i]->P;
             const Shape& s = backUniform ? best : saved[i];
-            TRY(ddfan_plan_set_param(P, DDFAN_PARAM_BACK_CHUNK, s.ZTb));
+            TRY(ddfan_plan_set_param(P, DDFAN_PARAM_BACK_CHUNK, backUniform ? zc(i, s.ZTb) : s.ZTb));
             TRY(ddfan_plan_set_param(P, DDFAN_PARAM_BACK_STAGE, s.bstage));
         }
         TRY(fit_tables());

@@ -292,7 +292,8 @@ int ddfan_batch_set_param(ddfan_batch* batch, int param, long long value);
  * tables below DDFAN_BATCH_SOLO_BYTES).  A plan's own autotuning times it
  * alone, where a small reconstruction leaves most of the GPU idle; here one
  * projector shape (slice chunk, views or tasks per wave, staging) and one
- * backprojector shape (slice chunk, staged views) for all of them are timed on
+ * backprojector shape (slice chunk, staged views) for all of them (a plan
+ * whose slices allow only narrower chunks takes its widest) are timed on
  * their shared launches (median of reps, on two subsets of each) and the
  * fastest is applied to their plans when it beats the shapes in place.
  * Iterates are only read, slab tiles are kept, results do not change.
