@@ -42,8 +42,7 @@ class Batch:
 
     def set_streams(self, n: int):
         """Split the reconstructions into `n` groups of about equal work that
-        run side by side, each on its own stream (1: one group; 0: four when
-        every reconstruction is small, else one)."""
+        run side by side, each on its own stream (1: one group; 0: two)."""
         N.call("ddfan_batch_set_param", self.handle, N.BATCH_STREAMS, int(n))
 
     def tune(self, reps=2) -> dict:

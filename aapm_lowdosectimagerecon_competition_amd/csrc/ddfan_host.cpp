@@ -100,7 +100,8 @@ int choose_slice_chunk(const float* hangs, int PN, int SLN, int limit) {
     for (int ZT : cands) {
         if (limit > 0 && ZT > limit) continue;
         const int nzc = (SLN + ZT - 1) / ZT;
-        if ((long long)(nzc * ZT - SLN) * 8 > SLN) continue;
+        const bool oneChunk = nzc == 1 && ZT < 2 * SLN;
+        if (!oneChunk && (long long)(nzc * ZT - SLN) * 8 > SLN) continue;
         bool ok = true;
         for (int v = 0; v < PN && ok; ++v) {
             const float* h = hangs + (size_t)v * SLN;

@@ -49,7 +49,12 @@ void rebin_sm_table(int W, float SO, float DetCenterW, float PerDetA, float* SM)
 
 // Largest slice chunk ZT in {16,8,4,2,1} such that every chunk of ZT
 // consecutive slices has bit-identical angles in every view and the padding
-// (nzc*ZT - SLN) stays within an eighth of SLN.  limit caps ZT (0: none).
+// (nzc*ZT - SLN) stays within an eighth of SLN, or, for a volume that fits one
+// chunk (ZT < 2*SLN), below half the chunk: the backprojector computes each
+// voxel's detector footprint (about 100 VALU operations) once per view and
+// chunk and then about 6 per slice, so 13 slices in one chunk of 16 take about
+// 196 operations per voxel and view against 7 x 112 in chunks of 2.  limit
+// caps ZT (0: none).
 int choose_slice_chunk(const float* hangs, int PN, int SLN, int limit);
 
 }  // namespace ddf

@@ -1635,16 +1635,19 @@ static hipError_t launch_sat_tile_group(int mode, int G, SatDev* t, int n, hipSt
     return hipGetLastError();
 }
 
-// Slices per block of the tiled builder for a volume table of SLNp (padded)
-// slices: 32 or 16, or 0 for the row-per-block builder (tables of fewer
-// slices, and sinogram tables).  Measured on MI355X at 512x512x64, 2304 views
-// (two-float tables): volume tables 0.236 ms row per block, 0.157 ms tiled;
-// the sinogram table 0.534 ms row per block, 0.601 ms tiled (both builders
-// then move 3.6-4.3 TB/s, most of it written entries).
+// Slices per block of the tiled builder for a volume table of SLN slices: 32
+// or 16, or 0 for the row-per-block builder (tables of fewer slices, and
+// sinogram tables).  The two builders group a row's sums differently, so the
+// choice follows the volume's own slice count, never its padded count
+// nzc * ZT, which depends on the slice chunk (13 slices in chunks of 2 or of
+// 4 would otherwise get different tables).  The tiled builder's sums do not
+// depend on its slices per block.  Measured on MI355X at 512x512x64, 2304
+// views (two-float tables): volume tables 0.236 ms row per block, 0.157 ms
+// tiled; the sinogram table 0.534 ms row per block, 0.601 ms tiled (both
+// builders then move 3.6-4.3 TB/s, most of it written entries).
 static int sat_tile_group(const SatDev& t) {
     if (!t.tiled) return 0;
-    const int SLNp = t.nzc * t.ZT;
-    return SLNp >= 32 ? 32 : SLNp >= 16 ? 16 : 0;
+    return t.SLN >= 32 ? 32 : t.SLN >= 16 ? 16 : 0;
 }
 
 // Up to 2*kBatch tables.  The builder and its slices per block fix how a row

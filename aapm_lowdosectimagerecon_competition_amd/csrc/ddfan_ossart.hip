@@ -275,12 +275,14 @@ struct ddfan_batch {
 
 namespace {
 
-// Groups of an automatic split.  Measured on MI355X (16 jobs of 256^2 x 8-16
-// slices, 720 views, 10 iterations of 48-view subsets): one group 0.391 s,
-// two 0.337 s, four 0.311 s (jobs one after another 0.414 s); the uneven mix of
-// 256^2-768^2 x 8-123 slices ran slower side by side (7.22 s in one group,
-// 7.59 s in two, 7.87 s in four: large jobs' tables then share the caches).
-constexpr int kAutoStreams = 4;
+// Groups of an automatic split.  Measured on MI355X, 10 iterations of 48-view
+// subsets: 16 jobs of 256^2 x 8-16 slices, 720 views, with throughput-tuned
+// shared launches: one group 0.224 s, two 0.185 s, four 0.193-0.201 s (one
+// after another 0.404 s); the uneven mix of 256^2-768^2 x 8-123 slices, whose
+// jobs run their kernels back to back: one group 6.13 s, two 5.88-5.94 s,
+// four 6.03-6.10 s (one after another 6.10-6.12 s): one group's launch tails
+// and set-up overlap the other group's kernels.
+constexpr int kAutoStreams = 2;
 
 // Runs every job of the batch in `ng` groups, one stream each, side by side:
 // jobs go to the group with the least work so far (largest first; work =
@@ -560,7 +562,7 @@ int batch_tune(ddfan_batch* B, int reps, double* out_ms) {
             TRY(ddfan_plan_set_param(P, DDFAN_PARAM_PROJ_STAGE, s.pstage));
         }
         TRY(fit_tables());
-        TRY(proj());  // a projection of each iterate into the sinograms the backprojector reads
+        HIPCALL(proj());  // a projection of each iterate into the sinograms the backprojector reads
         for (int z : chunks)
             for (int bs : {1, 2, 4, 0}) {
                 if (bs * z > 32) continue;  // also bs * zc(i, z) <= 32 for every plan
@@ -723,12 +725,7 @@ int ddfan_batch_run(ddfan_batch* B, int flags) {
         SyncOnExit sync{B->stream};
         for (OsJob& J : B->jobs) TRY(ossart_prepare(J, B->stream));
         int ng = B->nstreams;
-        if (ng == 0) {  // automatic: groups side by side only when every job is small
-            bool small = true;
-            for (const OsJob& J : B->jobs)
-                small = small && (long long)(J.P->satX.bytes + J.P->satY.bytes) < B->shape.soloBytes;
-            ng = small ? kAutoStreams : 1;
-        }
+        if (ng == 0) ng = kAutoStreams;
         ng = std::min(ng, n);
         while ((int)B->extra.size() < ng - 1) {
             hipStream_t s = nullptr;

@@ -126,6 +126,8 @@ def cpu_baseline(g, hangs, vol, mask, nsample=288):
     import numpy as np
 
     from oracle import oracle as O
+    # built for this host's CPU unless something already loaded the prebuilt one
+    march = O.use_host_tuned_build() if O._lib is None else "x86-64-v3"
     nth = cpu_nthreads()
     gd = g.as_dict()
     sel = np.linspace(0, PN - 1, nsample).astype(np.int64)
@@ -139,9 +141,10 @@ def cpu_baseline(g, hangs, vol, mask, nsample=288):
     return {"value": 2.0 * XN * YN * SLN * nsample / med, "unit": "voxel-views/s",
             "cores": nth, "kind": "port", "cpu_model": cpu_model(),
             "sample": f"{nsample} of {PN} views (evenly spaced) of the 512x512x64 workload, FP32 "
-                      f"projector + backprojector restatement (oracle/, -O3, no fast-math), "
+                      f"projector + backprojector restatement (oracle/, gcc -O3 -march={march}, "
+                      f"no fast-math), "
                       f"median of 3 after a warm-up: {med:.2f} s",
-            "runs_s": ts}
+            "build": f"gcc -O3 -march={march} -ffp-contract=off", "runs_s": ts}
 
 
 def bench_config0(dd, nth):
@@ -521,6 +524,10 @@ def main():
                     help="only the helical workload (device rebin, then --helical-iters OS-SART "
                          "iterations with the reference's and with the aligned backprojector); "
                          "prints no JSON (for rocprofv3 runs)")
+    ap.add_argument("--batch-pass", action="store_true",
+                    help="only the batch of 16 small reconstructions (256x256 x 8-16 slices, 720 "
+                         "views, 48-view subsets): shared launches tuned, then 2 iterations in "
+                         "one stream with direct launches; prints no JSON (for rocprofv3 runs)")
     ap.add_argument("--ossart-eager", action="store_true",
                     help="with --ossart-pass: launch the iterations eagerly, without hipGraphs")
     ap.add_argument("--serve-load", type=float, default=0.8,
@@ -544,6 +551,22 @@ def main():
 
     if dd.device_count() < 1:
         raise RuntimeError("no GPU visible")
+    if args.batch_pass:
+        specs = small_specs(16, 7)
+        jobs = [DeviceJob(dd, sp, args.sat, 1000 + i, 2) for i, sp in enumerate(specs)]
+        with dd.Batch(args.sat) as B:
+            B.set_streams(1)
+            for j in jobs:
+                j.reset()
+                B.add(j.plan, j.meas, j.x, j.iters, j.nos, 1)
+            tuned = B.tune(2)
+            ta = time.perf_counter()
+            B.run(graph=False)
+            tb = time.perf_counter()
+        for j in jobs:
+            j.close()
+        log(f"[bench] batch pass done: tuned {tuned}, 2 iterations {tb - ta:.4f} s")
+        return
     if args.rebin_pass:
         bench_helical(dd, args.sat, 0)
         log("[bench] rebin pass done")

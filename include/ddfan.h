@@ -284,8 +284,7 @@ int ddfan_batch_size(const ddfan_batch* batch, int* n);
 /* DDFAN_BATCH_STREAMS (0..8): the reconstructions are split into that many
  * groups of about equal work, each group running its own batch iterations on
  * its own stream, so the kernels of different groups share the GPU.  0
- * (default): 4 groups when every reconstruction's volume tables are below
- * DDFAN_BATCH_SOLO_BYTES, else 1. */
+ * (default): 2 groups. */
 #define DDFAN_BATCH_STREAMS 4
 int ddfan_batch_set_param(ddfan_batch* batch, int param, long long value);
 /* Throughput launch shapes for the reconstructions that share launches (volume

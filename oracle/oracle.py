@@ -36,6 +36,29 @@ def build() -> str:
     return _LIB_PATH
 
 
+def use_host_tuned_build() -> str:
+    """Builds the restatement for the CPU this process runs on (gcc -O3
+    -march=native; contraction and fast-math stay off, so every result is the
+    same as the prebuilt x86-64-v3 library's) and makes lib() load it.  Call
+    before the first lib().  Returns the -march used: "native", or
+    "x86-64-v3" when the compiler is unavailable and the prebuilt library
+    stays."""
+    global _LIB_PATH
+    if _lib is not None:
+        raise RuntimeError("the restatement is already loaded")
+    out = os.path.join(_HERE, "_build", "libddfan_oracle_native.so")
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    cmd = ["gcc", "-O3", "-march=native", "-fPIC", "-std=c11", "-ffp-contract=off",
+           "-fno-fast-math", "-shared", "-o", out, os.path.join(_HERE, "ddfan_oracle.c"),
+           "-lm", "-lpthread"]
+    try:
+        subprocess.run(cmd, check=True, capture_output=True)
+    except (OSError, subprocess.CalledProcessError):
+        return "x86-64-v3"
+    _LIB_PATH = out
+    return "native"
+
+
 def lib():
     global _lib
     if _lib is None:

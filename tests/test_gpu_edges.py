@@ -44,10 +44,11 @@ def test_zero_views():
         assert np.all(dv.to_numpy(vol.shape) == 0.0)
 
 
-@pytest.mark.parametrize("SLN", [1, 9, 30])
+@pytest.mark.parametrize("SLN", [1, 9, 17, 30])
 def test_slice_counts_and_nonsquare_without_mask(oracle, SLN):
-    """SLN 1 (one chunk of one slice), 9 (chunks of 2, last one half empty),
-    30 (chunks of 16, last one 14 slices); 37x53 image, no mask."""
+    """SLN 1 (one chunk of one slice), 9 (one chunk of 16, 7 slices of it
+    padding), 17 (chunks of 2, last one half empty), 30 (chunks of 16, last one
+    14 slices); 37x53 image, no mask."""
     dd, conf, g, hangs, vol, _ = _geom(37, 53, SLN, 20, mask=False)
     print(f"\n[edges] SLN {SLN}, 37x53, no mask", flush=True)
     with dd.Plan(g, hangs, None, sat_mode="twofloat") as plan:
@@ -59,8 +60,34 @@ def test_slice_counts_and_nonsquare_without_mask(oracle, SLN):
     ref_b = oracle.back(g.as_dict(), hangs, prj, ones, 37, 53)
     ep, eb = rel_l2(prj, ref_p), rel_l2(bp, ref_b)
     print(f"[edges] chunk {zt}: projector {ep:.2e}, backprojector {eb:.2e}", flush=True)
-    assert {1: 1, 9: 2, 30: 16}[SLN] == zt
+    assert {1: 1, 9: 16, 17: 2, 30: 16}[SLN] == zt
     assert ep <= 5e-6 and eb <= 5e-6
+
+
+@pytest.mark.parametrize("sat", ["f32", "twofloat"])
+@pytest.mark.parametrize("SLN", [13, 15])
+def test_results_do_not_depend_on_slice_chunk(sat, SLN):
+    """Fewer than 16 slices, padded to 16 by some chunks and not by others
+    (13 slices: 13, 14 or 16 table slices; 15: 15 or 16): every projector chunk
+    and every backprojector chunk gives the same bits, since the volume tables'
+    builder follows the volume's own slice count."""
+    dd, conf, g, hangs, vol, mask = _geom(48, 40, SLN, 36)
+    print(f"\n[edges] chunk independence, SLN {SLN}, {sat} tables", flush=True)
+    with dd.Plan(g, hangs, mask, sat_mode=sat) as plan:
+        assert plan.slice_chunk == 16
+        prj, bp = {}, {}
+        for z in (1, 2, 4, 8, 16):
+            plan.proj_chunk = z
+            plan.proj_stage = 0
+            prj[z] = plan.proj_host(vol)
+        ref = prj[1]
+        for z in (1, 2, 4, 8, 16):
+            plan.back_chunk = z
+            plan.back_stage = 0
+            bp[z] = plan.back_host(ref)
+    for z in (2, 4, 8, 16):
+        assert np.array_equal(prj[z], prj[1]), ("projector", z)
+        assert np.array_equal(bp[z], bp[1]), ("backprojector", z)
 
 
 def test_fewer_views_than_a_wave_groups(oracle):
