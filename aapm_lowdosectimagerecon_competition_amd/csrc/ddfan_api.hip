@@ -311,7 +311,7 @@ int ddfan_plan_create(const ddfan_geom* G, const float* hangs, int PN, const uns
             // per lane), the backprojector runs on the widest chunk with its
             // detector windows staged one view at a time (narrow chunks stage
             // 4 / chunk views per group, so each wave copies a row).
-            P->ZT = std::min(P->ZTmax, 8);
+            P->ZT = default_proj_chunk(P->ZTmax, SLN);
             P->projStage = P->ZT >= 4 ? 1 : 0;
             P->nzc = (SLN + P->ZT - 1) / P->ZT;
             P->ZTb = P->ZTmax;

@@ -99,9 +99,7 @@ int choose_slice_chunk(const float* hangs, int PN, int SLN, int limit) {
     static const int cands[] = {16, 8, 4, 2};
     for (int ZT : cands) {
         if (limit > 0 && ZT > limit) continue;
-        const int nzc = (SLN + ZT - 1) / ZT;
-        const bool oneChunk = nzc == 1 && ZT < 2 * SLN;
-        if (!oneChunk && (long long)(nzc * ZT - SLN) * 8 > SLN) continue;
+        if (ZT >= 2 * SLN) continue;  // at most half of a chunk is padding
         bool ok = true;
         for (int v = 0; v < PN && ok; ++v) {
             const float* h = hangs + (size_t)v * SLN;
@@ -111,6 +109,14 @@ int choose_slice_chunk(const float* hangs, int PN, int SLN, int limit) {
             }
         }
         if (ok) return ZT;
+    }
+    return 1;
+}
+
+int default_proj_chunk(int ZTmax, int SLN) {
+    for (int ZT = ZTmax < 8 ? ZTmax : 8; ZT > 1; ZT >>= 1) {
+        const int nzc = (SLN + ZT - 1) / ZT;
+        if ((long long)(nzc * ZT - SLN) * 8 <= SLN) return ZT;
     }
     return 1;
 }

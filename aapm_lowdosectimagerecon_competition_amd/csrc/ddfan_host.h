@@ -48,13 +48,17 @@ int rebin_start(float z, float h, float deltaZ);
 void rebin_sm_table(int W, float SO, float DetCenterW, float PerDetA, float* SM);
 
 // Largest slice chunk ZT in {16,8,4,2,1} such that every chunk of ZT
-// consecutive slices has bit-identical angles in every view and the padding
-// (nzc*ZT - SLN) stays within an eighth of SLN, or, for a volume that fits one
-// chunk (ZT < 2*SLN), below half the chunk: the backprojector computes each
-// voxel's detector footprint (about 100 VALU operations) once per view and
-// chunk and then about 6 per slice, so 13 slices in one chunk of 16 take about
-// 196 operations per voxel and view against 7 x 112 in chunks of 2.  limit
-// caps ZT (0: none).
+// consecutive slices has bit-identical angles in every view and ZT < 2*SLN
+// (less than half of a chunk is padding).  The last chunk may be partly
+// padding: the backprojector computes each voxel's detector footprint (about
+// 100 VALU operations) once per view and chunk and then about 6 per slice, so
+// 13 slices in one chunk of 16 take about 196 operations per voxel and view
+// against 7 x 112 in chunks of 2, and 18 slices 2 x 196 in chunks of 16
+// against 5 x 124 in chunks of 4.  limit caps ZT (0: none).
 int choose_slice_chunk(const float* hangs, int PN, int SLN, int limit);
+// The projector's default chunk: the widest ZT <= min(ZTmax, 8) whose padding
+// (nzc*ZT - SLN) stays within an eighth of SLN (the projector's per-slice
+// sums are most of its work, so padding costs it more), else 1.
+int default_proj_chunk(int ZTmax, int SLN);
 
 }  // namespace ddf

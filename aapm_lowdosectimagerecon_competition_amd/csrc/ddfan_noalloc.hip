@@ -81,7 +81,7 @@ int ddfan_proj_noalloc(float x0, float y0, int DNU, const float* xds, const floa
                   "angle %zu is not finite or exceeds 1e6 rad", i);
         const int mode = default_sat_mode();
         // the launch shape of a plan's defaults (ddfan_plan_create)
-        const int ZT = std::min(choose_slice_chunk(hangs, PN, SLN, 0), 8);
+        const int ZT = default_proj_chunk(choose_slice_chunk(hangs, PN, SLN, 0), SLN);
         const int nzc = (SLN + ZT - 1) / ZT;
         const int stage = ZT >= 4 ? 1 : 0;
         // scratch layout

@@ -44,14 +44,15 @@ def test_zero_views():
         assert np.all(dv.to_numpy(vol.shape) == 0.0)
 
 
-@pytest.mark.parametrize("SLN", [1, 9, 17, 30])
-def test_slice_counts_and_nonsquare_without_mask(oracle, SLN):
+@pytest.mark.parametrize("SLN,limit", [(1, 0), (9, 0), (17, 0), (17, 2), (30, 0)])
+def test_slice_counts_and_nonsquare_without_mask(oracle, SLN, limit):
     """SLN 1 (one chunk of one slice), 9 (one chunk of 16, 7 slices of it
-    padding), 17 (chunks of 2, last one half empty), 30 (chunks of 16, last one
-    14 slices); 37x53 image, no mask."""
+    padding), 17 (two chunks of 16, the second one slice; or, with chunks
+    limited to 2, the last one half empty), 30 (chunks of 16, last one 14
+    slices); 37x53 image, no mask."""
     dd, conf, g, hangs, vol, _ = _geom(37, 53, SLN, 20, mask=False)
-    print(f"\n[edges] SLN {SLN}, 37x53, no mask", flush=True)
-    with dd.Plan(g, hangs, None, sat_mode="twofloat") as plan:
+    print(f"\n[edges] SLN {SLN} (chunk limit {limit}), 37x53, no mask", flush=True)
+    with dd.Plan(g, hangs, None, sat_mode="twofloat", slice_chunk=limit) as plan:
         zt = plan.slice_chunk
         prj = plan.proj_host(vol)
         bp = plan.back_host(prj)
@@ -60,7 +61,7 @@ def test_slice_counts_and_nonsquare_without_mask(oracle, SLN):
     ref_b = oracle.back(g.as_dict(), hangs, prj, ones, 37, 53)
     ep, eb = rel_l2(prj, ref_p), rel_l2(bp, ref_b)
     print(f"[edges] chunk {zt}: projector {ep:.2e}, backprojector {eb:.2e}", flush=True)
-    assert {1: 1, 9: 16, 17: 2, 30: 16}[SLN] == zt
+    assert {(1, 0): 1, (9, 0): 16, (17, 0): 16, (17, 2): 2, (30, 0): 16}[(SLN, limit)] == zt
     assert ep <= 5e-6 and eb <= 5e-6
 
 
