@@ -50,6 +50,10 @@ BATCH_BACK_FILL = 2
 BATCH_SOLO_BYTES = 3
 BATCH_STREAMS = 4
 SERVER_GRAPH = 1
+SERVER_POLICY = 2
+SERVER_MAX_WAIT_MS = 3
+POLICY_FIFO = 0
+POLICY_SHORTEST = 1
 
 
 class DDFanError(RuntimeError):

@@ -182,6 +182,11 @@ struct ddfan_server {
     bool stop = false, reaperStop = false;
     ddf::ServerHooks hooks;  // how the worker checks GPU completion (ddfan_serve_hooks.h)
     bool graphs_ = false;    // DDFAN_SERVER_GRAPH: replay each membership's iteration as a hipGraph
+    const Clock::time_point t0 = Clock::now();  // the admission policy's clock
+    double maxWaitS = 10.0;                      // DDFAN_SERVER_MAX_WAIT_MS
+    double clock_s(Clock::time_point t) const {
+        return std::chrono::duration<double>(t - t0).count();
+    }
     // With graphs on, the worker's stream capture and graph instantiation, the
     // reaper's frees (hipFree waits for all device work, the capturing stream
     // included) and the submit path's allocations and copies hold this, so no
@@ -389,7 +394,7 @@ void ddfan_server::loop() {
             for (;;) {
                 for (Request* R : ready) admitted.push_back(R);
                 ready.clear();
-                for (long long id : adm.admit()) {
+                for (long long id : adm.admit(clock_s(Clock::now()))) {
                     Request* R = reqs[id].get();
                     R->tAdmit = Clock::now();
                     admitted.push_back(R);
@@ -637,6 +642,16 @@ int ddfan_server_set_param(ddfan_server* S, int param, long long value) {
                 CHECK(value == 0 || value == 1, DDFAN_EINVAL, "DDFAN_SERVER_GRAPH takes 0 or 1");
                 S->graphs_ = value != 0;
                 return DDFAN_OK;
+            case DDFAN_SERVER_POLICY:
+                CHECK(value == DDFAN_POLICY_FIFO || value == DDFAN_POLICY_SHORTEST, DDFAN_EINVAL,
+                      "DDFAN_SERVER_POLICY takes DDFAN_POLICY_FIFO or DDFAN_POLICY_SHORTEST");
+                S->adm.set_policy((int)value, S->maxWaitS);
+                return DDFAN_OK;
+            case DDFAN_SERVER_MAX_WAIT_MS:
+                CHECK(value >= 0, DDFAN_EINVAL, "DDFAN_SERVER_MAX_WAIT_MS must be >= 0");
+                S->maxWaitS = (double)value * 1e-3;
+                S->adm.set_policy(S->adm.policy(), S->maxWaitS);
+                return DDFAN_OK;
             default:
                 return fail(DDFAN_EINVAL, "unknown server parameter %d", param);
         }
@@ -704,7 +719,10 @@ int ddfan_server_submit(ddfan_server* S, const ddfan_geom* G, const float* hangs
               "host copy of the request's inputs failed");
         std::lock_guard<std::mutex> lk(S->mu);
         CHECK(!S->stop, DDFAN_EINVAL, "server is shutting down");
-        const bool queued = S->adm.submit(R->id, bytes);  // bytes <= budget: always true
+        // work of the whole request (voxel-views of its iterations) orders
+        // shortest-first admission
+        const double work = request_work(*R) * std::max(numIter, 1);
+        const bool queued = S->adm.submit(R->id, bytes, work, S->clock_s(arrival));
         CHECK(queued, DDFAN_EBUSY, "request rejected by the HBM budget");
         const long long rid = R->id;
         *id = rid;

@@ -27,9 +27,11 @@ class Server:
     memory."""
 
     def __init__(self, hbm_budget: int = 0, max_slots: int = 16, sat_mode="twofloat",
-                 graphs: bool = False):
+                 graphs: bool = False, policy: str = "fifo", max_wait_ms=None):
         """``graphs``: replay each request set's batch iteration as a captured
-        hipGraph instead of launching its kernels directly (same results)."""
+        hipGraph instead of launching its kernels directly (same results).
+        ``policy``: "fifo" (arrival order) or "shortest" (least work first,
+        unless a request has waited ``max_wait_ms``, default 10 s)."""
         h = ctypes.c_void_p()
         N.call("ddfan_server_create", ctypes.c_size_t(int(hbm_budget)), int(max_slots),
                _sat_mode(sat_mode), ctypes.byref(h))
@@ -37,6 +39,10 @@ class Server:
         self._shapes = {}
         if graphs:
             N.call("ddfan_server_set_param", self.handle, N.SERVER_GRAPH, 1)
+        pol = {"fifo": N.POLICY_FIFO, "shortest": N.POLICY_SHORTEST}[policy]
+        if max_wait_ms is not None:
+            N.call("ddfan_server_set_param", self.handle, N.SERVER_MAX_WAIT_MS, int(max_wait_ms))
+        N.call("ddfan_server_set_param", self.handle, N.SERVER_POLICY, pol)
 
     def submit(self, geom: DetectorGeometry, hangs, meas, num_iter, num_os, use_fista=1,
                mask=None, x0=None) -> int:

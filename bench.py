@@ -340,7 +340,7 @@ def big_request(dd, sat):
 
 
 def bench_serve(dd, jobs, alone_s, sat, load=0.8, nreq=40, iters=10, seed=4, slots=4, big=None,
-                graphs=False):
+                graphs=False, policy="fifo", max_wait_ms=None):
     """Poisson arrivals of `nreq` requests drawn from the batch mix plus one
     768x768x256, 2304-view request (`big`), each `iters` OS-SART iterations,
     at `load` times the GPU's capacity for mix requests (capacity = 1 / the
@@ -360,7 +360,8 @@ def bench_serve(dd, jobs, alone_s, sat, load=0.8, nreq=40, iters=10, seed=4, slo
         solo.insert(ibig, big.alone_s)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, len(reqs)))
     log(f"[bench] serve: {len(reqs)} requests at {rate:.2f}/s (load {load}), graphs {graphs}")
-    with dd.Server(hbm_budget=0, max_slots=slots, sat_mode=sat, graphs=graphs) as S:
+    with dd.Server(hbm_budget=0, max_slots=slots, sat_mode=sat, graphs=graphs, policy=policy,
+                   max_wait_ms=max_wait_ms) as S:
         t0 = time.perf_counter()
         ids = []
         for i, j in enumerate(reqs):
@@ -377,6 +378,7 @@ def bench_serve(dd, jobs, alone_s, sat, load=0.8, nreq=40, iters=10, seed=4, slo
     out = {"requests": len(reqs), "load": load, "rate_per_s": rate,
            "capacity_req_per_s": 1.0 / mean_alone,
            "offered_load": float(sum(solo) / arrivals[-1]), "iters": iters, "slots": slots,
+           "policy": policy, "max_wait_ms": max_wait_ms,
            "graphs": graphs, "p50_ms": float(np.percentile(lat, 50)),
            "p99_ms": float(np.percentile(lat, 99)),
            "mix_p50_ms": float(np.percentile(lat[mix], 50)),
@@ -534,6 +536,11 @@ def main():
                     help="offered load of the served stream, as a fraction of the GPU's capacity "
                          "(1 / the mean time of a mix request run alone)")
     ap.add_argument("--serve-requests", type=int, default=40)
+    ap.add_argument("--serve-policy", default="shortest", choices=["shortest", "fifo"],
+                    help="admission order of the served stream (the other policy is measured "
+                         "on the same arrivals beside it)")
+    ap.add_argument("--serve-max-wait-ms", type=int, default=4000,
+                    help="age limit of shortest-first admission")
     ap.add_argument("--serve-graphs", action="store_true",
                     help="the request loop replays each request set's iteration as a hipGraph")
     ap.add_argument("--serve-sweep", action="store_true",
@@ -780,13 +787,26 @@ def main():
         jobs, alone_s, out["batch"] = bench_batch(dd, args.sat, untuned=False)
         log(f"[bench] batch {out['batch']}")
         big = big_request(dd, args.sat)
+        mw = args.serve_max_wait_ms
         out["serve"] = bench_serve(dd, jobs, alone_s, args.sat, load=args.serve_load,
-                                   nreq=args.serve_requests, big=big, graphs=args.serve_graphs)
+                                   nreq=args.serve_requests, big=big, graphs=args.serve_graphs,
+                                   policy=args.serve_policy,
+                                   max_wait_ms=mw if args.serve_policy == "shortest" else None)
         log(f"[bench] serve {out['serve']}")
+        other = "fifo" if args.serve_policy == "shortest" else "shortest"
+        o = bench_serve(dd, jobs, alone_s, args.sat, load=args.serve_load,
+                        nreq=args.serve_requests, big=big, graphs=args.serve_graphs, policy=other,
+                        max_wait_ms=mw if other == "shortest" else None)
+        out["serve"]["other_policy"] = {k: o[k] for k in ("policy", "max_wait_ms", "p50_ms", "p99_ms",
+                                                          "mean_queue_ms", "voxel_views_per_s")}
+        out["serve"]["other_policy"]["large_request_ms"] = o["large_request"]["latency_ms"]
+        log(f"[bench] serve, {other} admission {out['serve']['other_policy']}")
         if args.serve_sweep:
             out["serve_sweep"] = [bench_serve(dd, jobs, alone_s, args.sat, load=ld, nreq=100,
-                                              big=big, graphs=args.serve_graphs, seed=10 + i)
-                                  for i, ld in enumerate((0.5, 0.8, 1.0))]
+                                              big=big, graphs=args.serve_graphs, seed=10 + i,
+                                              policy=pol, max_wait_ms=mw if pol == "shortest" else None)
+                                  for i, ld in enumerate((0.5, 0.8, 1.0))
+                                  for pol in ("shortest", "fifo")]
             log(f"[bench] serve sweep {out['serve_sweep']}")
         big.close()
         for j in jobs:

@@ -306,8 +306,9 @@ int ddfan_batch_destroy(ddfan_batch* batch);
 /* ------------------------------------------------------------ request loop
  *
  * A worker thread serves reconstruction requests on the current device.
- * Requests are admitted in arrival order while one of max_slots is free and
- * their device footprint fits in what is left of hbm_budget bytes (0: 80% of
+ * Requests are admitted (in arrival order by default, see DDFAN_SERVER_POLICY)
+ * while one of max_slots is free and their device footprint fits in what is
+ * left of hbm_budget bytes (0: 80% of
  * the free device memory); a request larger than the whole budget is
  * rejected at submission (DDFAN_EBUSY).  Admitted requests keep their
  * sinogram and iterate in device memory and run together as a batch
@@ -336,6 +337,18 @@ int ddfan_server_poll(ddfan_server* server, long long id, int* done);
  * keeps at most two steps in flight either way.  Results are the same bit
  * for bit. */
 #define DDFAN_SERVER_GRAPH 1
+/* DDFAN_SERVER_POLICY: the order in which queued requests are admitted.
+ * DDFAN_POLICY_FIFO (default): arrival order, a request that does not fit is
+ * not overtaken.  DDFAN_POLICY_SHORTEST: the queued request with the least
+ * work (voxels x views x iterations) that fits goes first, unless a request
+ * has waited DDFAN_SERVER_MAX_WAIT_MS (default 10000) or longer: then the
+ * longest-waiting of those goes first and is not overtaken.  A submission that
+ * finds nobody queued is admitted at once under both.  Results do not depend
+ * on the policy. */
+#define DDFAN_SERVER_POLICY 2
+#define DDFAN_SERVER_MAX_WAIT_MS 3
+#define DDFAN_POLICY_FIFO 0
+#define DDFAN_POLICY_SHORTEST 1
 int ddfan_server_set_param(ddfan_server* server, int param, long long value);
 /* Blocks until the request is done, copies its reconstruction to x (may be
  * NULL) and returns the milliseconds from submission to admission and to

@@ -1,5 +1,6 @@
 // Admission policy checks (host only): FIFO order, no overtaking of a large
-// head request, budget and slot limits, rejection of impossible requests.
+// head request, budget and slot limits, rejection of impossible requests;
+// shortest-first order, ties, the age limit and an aged request that blocks.
 #include <cassert>
 #include <cstdio>
 
@@ -45,6 +46,55 @@ int main() {
         assert(v.size() == 1 && v[0] == 7);
         assert(a.admit_now(40) && a.active() == 2);
         assert(!a.admit_now(1));                      // both slots taken
+    }
+    {
+        // shortest first: least work that fits, earliest on ties
+        Admission a(100, 2);
+        a.set_policy(Admission::SHORTEST, 10.0);
+        assert(a.submit(1, 30, 500.0, 0.0) && a.submit(2, 30, 100.0, 1.0) &&
+               a.submit(3, 30, 100.0, 2.0) && a.submit(4, 90, 10.0, 3.0));
+        auto v = a.admit(4.0);
+        // 4 has the least work but 90 + 30 > 100 once 2 runs: 4 first (fits), then nothing
+        assert(v.size() == 1 && v[0] == 4 && a.used() == 90);
+        a.release(90);
+        v = a.admit(5.0);
+        assert(v.size() == 2 && v[0] == 2 && v[1] == 3);  // tie on work: earlier first
+        a.release(30);
+        a.release(30);
+        v = a.admit(6.0);
+        assert(v.size() == 1 && v[0] == 1);
+        a.release(30);
+        assert(a.used() == 0 && a.active() == 0 && a.waiting() == 0);
+    }
+    {
+        // age limit: a request waiting max_wait or longer goes first and is not overtaken
+        Admission a(100, 1);
+        a.set_policy(Admission::SHORTEST, 5.0);
+        assert(a.submit(1, 60, 900.0, 0.0) && a.submit(2, 10, 1.0, 1.0));
+        auto v = a.admit(2.0);
+        assert(v.size() == 1 && v[0] == 2);  // 1 has waited 2 s < 5 s: shortest first
+        assert(a.submit(3, 10, 1.0, 2.5));
+        a.release(10);
+        v = a.admit(6.0);
+        assert(v.size() == 1 && v[0] == 1);  // 1 has waited 6 s: it goes before 3
+        a.release(60);
+        v = a.admit(6.5);
+        assert(v.size() == 1 && v[0] == 3);
+        a.release(10);
+        // an aged request that does not fit blocks the others until it does
+        Admission b(100, 4);
+        b.set_policy(Admission::SHORTEST, 1.0);
+        assert(b.admit_now(50));
+        assert(b.submit(5, 80, 900.0, 0.0) && b.submit(6, 10, 1.0, 0.5));
+        v = b.admit(0.6);
+        assert(v.size() == 1 && v[0] == 6);  // 5 not yet aged, does not fit; 6 fits
+        assert(b.submit(7, 10, 1.0, 0.7));
+        v = b.admit(2.0);
+        assert(v.empty());  // 5 aged: 7 must not overtake it
+        b.release(50);
+        b.release(10);
+        v = b.admit(2.1);
+        assert(v.size() == 2 && v[0] == 5 && v[1] == 7);
     }
     std::printf("sched ok\n");
     return 0;
