@@ -1263,7 +1263,7 @@ void k_proj_rows(const ProjLaunch L) {
     if (nv == 0) return;  // past the subset's blocks (whole block)
 
     // The wave's tasks: boundary rays of this lane; bit s of `on`: slot s holds
-    // a task (wave-uniform).
+    // a task.
     float pa[T], pf[T], pbb[T];
     unsigned on = 0;
 #pragma unroll
@@ -1284,7 +1284,10 @@ void k_proj_rows(const ProjLaunch L) {
             pbb[s] = pb.b;
         }
     }
-    on = __builtin_amdgcn_readfirstlane(on);
+    // Slots holding a task are a prefix (slot s holds task s * NW + wave, whose
+    // view index grows with s): their count, wave-uniform, bounds the slot
+    // loops with a scalar compare.
+    const int non = __builtin_amdgcn_readfirstlane(__builtin_popcount(on));
 
     float acc[T][ZS];
 #pragma unroll
@@ -1313,7 +1316,7 @@ void k_proj_rows(const ProjLaunch L) {
         const float obj = slab_obj(ii, ocS, g.dx);
 #pragma unroll
         for (int s = 0; s < T; ++s) {
-            if (!((on >> s) & 1u)) continue;  // uniform across the wave
+            if (s >= non) continue;  // uniform across the wave
             const float r = slab_r(obj, pa[s], pf[s], pbb[s]);
             const Fetch f = fetch_pos<E>(slab_coord(r, g.invdx, ocK), Nk);
             const float invlen = __builtin_amdgcn_rcpf(r - lane_prev(r));
@@ -1343,7 +1346,7 @@ void k_proj_rows(const ProjLaunch L) {
 
 #pragma unroll
     for (int s = 0; s < T; ++s) {
-        if (!((on >> s) & 1u)) continue;
+        if (s >= non) continue;
         const int t = s * NW + wave;
         const int tv = t / nug, tug = t - tv * nug;
         const int j = jv0 + tv;
