@@ -49,6 +49,38 @@ DDF_HD float asinf_cephes(float xx) {
     return xx < 0.0f ? -z : z;
 }
 
+#if defined(__HIPCC__)
+// Correctly rounded square root and quotient for operands in a middle range.
+// The library is compiled with -fhip-fp32-correctly-rounded-divide-sqrt, whose
+// expansions of sqrtf and '/' rescale operands near the ends of the exponent
+// range and fix up zeros, infinities and NaNs around the same core: the
+// hardware estimate (v_sqrt_f32, v_rcp_f32) and its residual corrections.
+// For a positive normal x in [2^-20, 2^40], and for a quotient whose divisor
+// lies there and whose dividend is 0 or of magnitude in [2^-40, 2^20], neither
+// the rescaling nor the fix-up changes anything, so the core alone gives the
+// same bits with 7 (square root) and 3 (quotient) fewer instructions.
+__device__ __forceinline__ float sqrt_rn_mid(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sdn = __int_as_float(__float_as_int(s) - 1);
+    const float sup = __int_as_float(__float_as_int(s) + 1);
+    float r = fmaf(-sdn, s, x) <= 0.0f ? sdn : s;
+    r = fmaf(-sup, s, x) > 0.0f ? sup : r;
+    return r;
+}
+__device__ __forceinline__ float div_rn_mid(float a, float b) {
+    float r = __builtin_amdgcn_rcpf(b);
+    r = fmaf(fmaf(-b, r, 1.0f), r, r);
+    float q = a * r;
+    q = fmaf(fmaf(-b, q, a), r, q);
+    return fmaf(fmaf(-b, q, a), r, q);
+}
+__device__ __forceinline__ bool rn_mid_divisor(float b) { return b >= 0x1p-20f && b <= 0x1p40f; }
+__device__ __forceinline__ bool rn_mid_dividend(float a) {
+    const float m = fabsf(a);
+    return a == 0.0f || (m >= 0x1p-40f && m <= 0x1p20f);
+}
+#endif
+
 // Texel index of an unnormalised, texel-centred linear fetch at coordinate c:
 // floor(c - 0.5), limited to [-1, N] (anything outside reads the clamped edge).
 DDF_HD int texel(float c, int N) {
@@ -126,10 +158,24 @@ DDF_HD BackSample back_sample(float vx, float vy, float c, float s, float x0, fl
     const float ddx = vx - srcx, ddy = vy - srcy;
     BackSample b;
     b.l2 = ddx * ddx + ddy * ddy;
-    b.l = 1.0f / sqrtf(b.l2);
-    const float alpha = asinf_cephes(((srcy * ddx - srcx * ddy) * inv_sid) * b.l);
     b.ddv = (fabsf(srcx) > fabsf(srcy)) ? ddx : ddy;
-    const float dA = ((b.ddv / b.l2) * dx) * 0.5f;
+    float q;  // ddv / l2
+#if defined(__HIP_DEVICE_COMPILE__)
+    // the short correctly rounded sequences when every lane of the wave is in
+    // their range (a wave-uniform branch), else the compiler's full ones
+    if (__builtin_amdgcn_ballot_w64(!(rn_mid_divisor(b.l2) && rn_mid_dividend(b.ddv))) == 0) {
+        b.l = div_rn_mid(1.0f, sqrt_rn_mid(b.l2));
+        q = div_rn_mid(b.ddv, b.l2);
+    } else {
+        b.l = 1.0f / sqrtf(b.l2);
+        q = b.ddv / b.l2;
+    }
+#else
+    b.l = 1.0f / sqrtf(b.l2);
+    q = b.ddv / b.l2;
+#endif
+    const float alpha = asinf_cephes(((srcy * ddx - srcx * ddy) * inv_sid) * b.l);
+    const float dA = (q * dx) * 0.5f;
     b.idxL = ((alpha - dA) * invdb + detCntIdx) + 1.0f;
     b.idxR = ((alpha + dA) * invdb + detCntIdx) + 1.0f;
     return b;

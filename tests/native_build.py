@@ -21,3 +21,17 @@ def serve_fault_test(rebuild=False) -> str:
                         "-Wl,-rpath,$ORIGIN/../../../aapm_lowdosectimagerecon_competition_amd",
                         "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"], check=True)
     return exe
+
+
+def cr_math_test(rebuild=False) -> str:
+    """tests/native/cr_math_test.hip, built with the library's HIP flags."""
+    exe = os.path.join(OUT, "cr_math_test")
+    if rebuild or not os.path.exists(exe):
+        os.makedirs(OUT, exist_ok=True)
+        subprocess.run(["/opt/rocm/bin/hipcc", "--offload-arch=gfx950", "-O3", "-std=c++17",
+                        "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt",
+                        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                        "-I" + os.path.join(PKG, "csrc"),
+                        os.path.join(ROOT, "tests", "native", "cr_math_test.hip"), "-o", exe],
+                       check=True)
+    return exe

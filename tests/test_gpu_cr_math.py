@@ -1,0 +1,20 @@
+"""The backprojector's short correctly rounded square root and quotient
+(ddfan_index.h) give the bits of the compiler's correctly rounded sqrtf and
+'/' over their whole operand range (tests/native/cr_math_test.hip: every float
+in [2^-20, 2^40] for the square root and its reciprocal, 2e11 quotients over
+three full divisor binades and 2^26 random pairs)."""
+import subprocess
+
+import pytest
+
+import native_build
+
+pytestmark = pytest.mark.gpu
+
+
+def test_short_correctly_rounded_sequences_match():
+    exe = native_build.cr_math_test()
+    print("\n[cr-math] running", exe, flush=True)
+    out = subprocess.run([exe], check=True, capture_output=True, text=True, timeout=120).stdout
+    print(out, flush=True)
+    assert "cr math ok" in out
