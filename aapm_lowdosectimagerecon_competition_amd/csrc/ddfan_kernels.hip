@@ -738,15 +738,16 @@ void k_back(const BackLaunch L) {
 
 // Staged backprojector: the sums of k_back, with the table reads served from
 // LDS.  For each group of NV consecutive views the block copies, per (view,
-// slice), the 1 KiB run of detector entries around the tile's footprint
-// (64 two-float / f64 entries, 128 f32 entries) into LDS with one 64-lane
-// global-to-LDS load, and the lanes gather from LDS instead of through the
-// texture-address path (one address per lane per slice, the bound of k_back).
-// Two buffers alternate: the copy of the next group is in flight while the
-// current group is summed.  A view's window is centred on the footprint of
-// the tile's four corner voxels, which every wave computes the same way (no
-// exchange needed); a lane whose entries fall outside it reads the table in
-// global memory, so the window decides speed only.  Arithmetic and summation
+// slice), the run of detector entries the tile's footprint covers (at most
+// 64 entries) into a 64-entry LDS row with one 64-lane global-to-LDS load,
+// and the lanes gather from LDS instead of through the texture-address path
+// (one address per lane per slice, the bound of k_back).  Two buffers
+// alternate: the copy of the next group is in flight while the current group
+// is summed.  A view's run spans the footprints of the tile's four corner
+// voxels, one entry wider on each side (a wider footprint gets the 64 entries
+// centred on it), which every wave computes the same way (no exchange
+// needed); a lane whose entries fall outside the run reads the table in
+// global memory, so the run decides speed only.  Arithmetic and summation
 // order are those of k_back, so the results are bit-identical.
 // Staging: one 64-lane global-to-LDS copy of 16 bytes per lane (one two-float
 // or f64 entry, two f32 entries) from a table row into LDS.  LDS operands are
@@ -837,8 +838,14 @@ void k_back_lds(const BackLaunch L) {
     const int tile = local % (J.ntx * J.nty);
     const int zc = local / (J.ntx * J.nty);
     const int tx0 = (tile % J.ntx) * 16, ty0 = (tile / J.ntx) * 16;
-    const int x = tx0 + (threadIdx.x & 15);
-    const int y = ty0 + (threadIdx.x >> 4);
+    // Lane -> voxel: a wave is an 8x8 voxel patch and each 16 lanes a 4x4 one,
+    // so neighbouring lanes read nearby detector entries in every view
+    // direction (a row of 16 voxels spans ~22 entries when rays run along y).
+    // Measured on MI355X at 512x512x64, 2304 views, against 16x4 row-major
+    // waves: two-float 16.9 -> 15.2 ms, f32 14.7 -> 11.8, f64 19.9 -> 18.9
+    // (fewer LDS bank conflicts; 4x4 patches in 16x4 waves do the same).
+    const int x = tx0 + 8 * ((threadIdx.x >> 6) & 1) + 4 * ((threadIdx.x >> 4) & 1) + (threadIdx.x & 3);
+    const int y = ty0 + 8 * (threadIdx.x >> 7) + 4 * ((threadIdx.x >> 5) & 1) + ((threadIdx.x >> 2) & 3);
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const bool inb = x < g.XN && y < g.YN;
@@ -872,7 +879,7 @@ void k_back_lds(const BackLaunch L) {
         const int cx = (lane & 1) ? min(tx0 + 15, g.XN - 1) : tx0;
         const int cy = (lane & 2) ? min(ty0 + 15, g.YN - 1) : ty0;
         const float cvx = ((float)cx - g.ocxB) * g.dx, cvy = ((float)cy - g.ocyB) * g.dx;
-        int wl = 0;
+        int wl = 0, wn = 0;
         auto windows16 = [&](int v0) __attribute__((always_inline)) {
             const int j = min(v0 + (lane >> 2), count - 1);
             const int vg = J.sub.first + j * J.sub.stride;
@@ -881,23 +888,33 @@ void k_back_lds(const BackLaunch L) {
                                               g.detCntIdx, g.dx);
             const int lo = quad_min(texel(fminf(bs.idxL, bs.idxR), DNU));
             const int hi = quad_max(texel(fmaxf(bs.idxL, bs.idxR), DNU));
-            wl = max(0, min(((lo + hi) >> 1) - WENT / 2, K - WENT));
+            const int s = max(lo - 1, 0), e = min(hi + 1, DNU);
+            if (e - s < WENT) {
+                wl = s;
+                wn = e - s + 1;
+            } else {
+                wl = max(0, min(((lo + hi) >> 1) - WENT / 2, K - WENT));
+                wn = min(K - wl, WENT);
+            }
         };
-        auto window = [&](int grp, int* w0) __attribute__((always_inline)) {
+        auto window = [&](int grp, int* w0, int* n0) __attribute__((always_inline)) {
             const int j0 = grp * NV;  // NV divides 16: a group never straddles two sets
             if ((j0 & 15) == 0) windows16(j0);
 #pragma unroll
-            for (int v = 0; v < NV; ++v) w0[v] = __builtin_amdgcn_readlane(wl, 4 * ((j0 + v) & 15));
+            for (int v = 0; v < NV; ++v) {
+                w0[v] = __builtin_amdgcn_readlane(wl, 4 * ((j0 + v) & 15));
+                n0[v] = __builtin_amdgcn_readlane(wn, 4 * ((j0 + v) & 15));
+            }
         };
         // Row (view v, slice q) is copied by wave (v*ZT + q) mod 4: its WENT
         // entries, or the rest of the table row if shorter.
-        auto stage = [&](int grp, LE* buf, const int* w0) __attribute__((always_inline)) {
+        auto stage = [&](int grp, LE* buf, const int* w0, const int* n0) __attribute__((always_inline)) {
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 const int j = grp * NV + v;
                 if (j >= count) break;
                 const E* src = Pc + (size_t)j * viewStride + w0[v];
-                const int nrow = min(K - w0[v], WENT);
+                const int nrow = n0[v];
                 if constexpr (ZT % 4 == 0) {
 #pragma unroll
                     for (int i = 0; i < ZT / 4; ++i) {
@@ -917,7 +934,7 @@ void k_back_lds(const BackLaunch L) {
         };
         const float vx = ((float)x - g.ocxB) * g.dx;
         const float vy = ((float)y - g.ocyB) * g.dx;
-        auto sum = [&](int grp, const LE* buf, const int* w0) __attribute__((always_inline)) {
+        auto sum = [&](int grp, const LE* buf, const int* w0, const int* n0) __attribute__((always_inline)) {
             if (m == 0) return;
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
@@ -931,7 +948,7 @@ void k_back_lds(const BackLaunch L) {
                 const float cosVal = g.dx * __builtin_amdgcn_rcpf(bs.ddv * bs.l);
                 const int iL = fL.i0 - w0[v];
                 const int iR = fR.i0 - w0[v];
-                if ((unsigned)iL < (unsigned)WENT && (unsigned)iR < (unsigned)WENT) {
+                if ((unsigned)iL < (unsigned)n0[v] && (unsigned)iR < (unsigned)n0[v]) {
                     const LE* lrow = buf + v * ZT * WENT;
 #pragma unroll
                     for (int q = 0; q < ZT; ++q) {
@@ -950,26 +967,26 @@ void k_back_lds(const BackLaunch L) {
                 }
             }
         };
-        int wA[NV], wB[NV];
+        int wA[NV], wB[NV], nA[NV], nB[NV];
         const int ng = (count + NV - 1) / NV;
-        window(0, wA);
-        stage(0, ldsA, wA);
+        window(0, wA, nA);
+        stage(0, ldsA, wA, nA);
         wait_vmem();
         __syncthreads();
         for (int grp = 0; grp < ng; grp += 2) {
             if (grp + 1 < ng) {
-                window(grp + 1, wB);
-                stage(grp + 1, ldsB, wB);
+                window(grp + 1, wB, nB);
+                stage(grp + 1, ldsB, wB, nB);
             }
-            sum(grp, ldsA, wA);
+            sum(grp, ldsA, wA, nA);
             wait_vmem();
             __syncthreads();
             if (grp + 1 >= ng) break;
             if (grp + 2 < ng) {
-                window(grp + 2, wA);
-                stage(grp + 2, ldsA, wA);
+                window(grp + 2, wA, nA);
+                stage(grp + 2, ldsA, wA, nA);
             }
-            sum(grp + 1, ldsB, wB);
+            sum(grp + 1, ldsB, wB, nB);
             wait_vmem();
             __syncthreads();
         }
