@@ -1126,7 +1126,11 @@ void k_proj_lds(const ProjLaunch L) {
                     const int K = Nk + 1;
                     const float r = slab_r(slab_obj(ii, ocS, g.dx), pa[v], pf[v], pbb[v]);
                     const Fetch f = fetch_pos<E>(slab_coord(r, g.invdx, ocK), Nk);
-                    const float invlen = __builtin_amdgcn_rcpf(r - lane_prev(r));
+                    // The neighbour difference is pinned ahead of the fallback
+                    // branch, where the DPP read fuses into the subtraction.
+                    float dr = r - lane_prev(r);
+                    asm volatile("" : "+v"(dr));
+                    const float invlen = __builtin_amdgcn_rcpf(dr);
                     const int li = f.i0 - w0[s];
                     // Every lane reads the run (index clamped into it); lanes outside
                     // the run, and views of the other slab orientation, then replace
