@@ -44,6 +44,7 @@ PARAM_BACK_CHUNK = 4
 PARAM_BACK_STAGE = 5
 PARAM_PROJ_STAGE = 6
 PARAM_BACK_PASS = 7
+PARAM_BACK_MID_RANGE = 8
 TUNE_USE_CACHE = 1
 BATCH_PROJ_FILL = 1
 BATCH_BACK_FILL = 2

@@ -104,6 +104,7 @@ void fill_geom(GeomDev& g, const ddfan_geom* G, int SLN) {
     g.YN = G->YN;
     g.SLN = SLN;
     g.DNU = G->DNU;
+    g.backMid = back_mid_range(G->XN, G->YN, g.dx, g.ocxB, g.ocyB, g.x0, g.y0);
 }
 
 int default_sat_mode() { return g_default_sat.load(); }
@@ -258,7 +259,7 @@ int ddfan_plan_footprint(const ddfan_geom* G, int PN, int sat_mode, int slice_ch
         CHECK(bytes && PN >= 1 && sat_mode >= 0 && sat_mode <= 2, DDFAN_EINVAL, "bad arguments");
         const int ZT = slice_chunk > 0 ? slice_chunk : 16;
         const int nzc = (G->SLN + ZT - 1) / ZT;
-        size_t b = sizeof(float) * 2 * (size_t)PN * G->SLN + sizeof(float) * 4 * (G->DNU + 1) +
+        size_t b = sizeof(float) * 4 * (size_t)PN * G->SLN + sizeof(float) * 4 * (G->DNU + 1) +
                    (size_t)G->XN * G->YN;
         b += 2 * sat_vol_bytes(sat_mode, nzc, ZT, G->XN > G->YN ? G->XN : G->YN,
                                G->XN > G->YN ? G->XN : G->YN);
@@ -269,6 +270,7 @@ int ddfan_plan_footprint(const ddfan_geom* G, int PN, int sat_mode, int slice_ch
             b += sizeof(float) * ((size_t)cnt * G->DNU * G->SLN + nvox + (size_t)numOS * cnt * G->DNU +
                                   (size_t)numOS * G->XN * G->YN);
             b += sat_sino_bytes(sat_mode, 1, 1, G->DNU, cnt) + 2 * sat_vol_bytes(sat_mode, 1, 1, G->XN, G->YN);
+            b += sizeof(float) * 4 * (size_t)PN;  // slice 0's (cos, sin) and source tables
         } else {
             b += sat_sino_bytes(sat_mode, nzc, ZT, G->DNU, PN);
         }
@@ -331,7 +333,10 @@ int ddfan_plan_create(const ddfan_geom* G, const float* hangs, int PN, const uns
             dd_boundaries(DNU + 1, G->yds, hby.data());
             std::vector<uint8_t> hm;
             if (!mask) hm.assign((size_t)G->XN * G->YN, 1);
+            std::vector<float> hsrc(P->h_cs.size());
+            source_table(P->h_cs.data(), (size_t)PN * SLN, G->x0, G->y0, hsrc.data());
             TRY(upload(P->cs, P->h_cs.data(), sizeof(float) * P->h_cs.size(), P->stream));
+            TRY(upload(P->src, hsrc.data(), sizeof(float) * hsrc.size(), P->stream));
             TRY(upload(P->bx, hbx.data(), sizeof(float) * (DNU + 1), P->stream));
             TRY(upload(P->by, hby.data(), sizeof(float) * (DNU + 1), P->stream));
             TRY(upload(P->xds, G->xds, sizeof(float) * DNU, P->stream));
@@ -421,6 +426,8 @@ int ddfan_plan_set_param(ddfan_plan* P, int param, int value) {
                 HIPCALL(hipStreamSynchronize(P->stream));
                 P->backStage = value;
                 return DDFAN_OK;
+            case DDFAN_PARAM_BACK_MID_RANGE:
+                return fail(DDFAN_EINVAL, "DDFAN_PARAM_BACK_MID_RANGE is read-only");
             default:
                 return fail(DDFAN_EINVAL, "unknown plan parameter %d", param);
         }
@@ -438,6 +445,7 @@ int ddfan_plan_get_param(const ddfan_plan* P, int param, int* value) {
             case DDFAN_PARAM_BACK_STAGE: *value = P->backStage; return DDFAN_OK;
             case DDFAN_PARAM_PROJ_STAGE: *value = P->projStage; return DDFAN_OK;
             case DDFAN_PARAM_BACK_PASS: *value = P->backPass; return DDFAN_OK;
+            case DDFAN_PARAM_BACK_MID_RANGE: *value = P->g.backMid; return DDFAN_OK;
             default: return fail(DDFAN_EINVAL, "unknown plan parameter %d", param);
         }
     });
@@ -538,7 +546,7 @@ int ddfan_plan_back(ddfan_plan* P, const float* d_prj, float* d_vol, int first, 
                                    P->stream);
         }));
         HIPCALL(P->timed(3, [&] {
-            return launch_back(P->mode, P->ZTb, BACK_PLAIN, P->g, P->nzcb, P->cs.as<float2>(),
+            return launch_back(P->mode, P->ZTb, BACK_PLAIN, P->g, P->nzcb, P->src.as<float2>(),
                                P->mask.as<uint8_t>(), P->satP.p, sub, d_vol, nullptr, nullptr, nullptr,
                                nullptr, P->backStage, P->stream, P->backPass);
         }));
@@ -594,7 +602,7 @@ int ddfan_plan_trace_back(ddfan_plan* P, int first, int stride, int count, int32
         CHECK(P && d_trace, DDFAN_EINVAL, "NULL argument");
         TRY(check_subset(P->PN, first, stride, count));
         HIPCALL(hipSetDevice(P->device));
-        HIPCALL(launch_back_trace(P->g, P->cs.as<float2>(), Subset{first, stride, count}, d_trace,
+        HIPCALL(launch_back_trace(P->g, P->src.as<float2>(), Subset{first, stride, count}, d_trace,
                                   P->stream));
         HIPCALL(hipStreamSynchronize(P->stream));
         return DDFAN_OK;

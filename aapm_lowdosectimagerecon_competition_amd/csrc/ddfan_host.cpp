@@ -34,6 +34,31 @@ void cossin_table(const float* hangs, size_t n, float* cs) {
     }
 }
 
+void source_table(const float* cs, size_t n, float x0, float y0, float* src) {
+    for (size_t i = 0; i < n; ++i) {
+        const float c = cs[2 * i], s = cs[2 * i + 1];
+        src[2 * i] = x0 * c - y0 * s;
+        src[2 * i + 1] = x0 * s + y0 * c;
+    }
+}
+
+int back_mid_range(int XN, int YN, float dx, float ocxB, float ocyB, float x0, float y0) {
+    // voxel centres (i - oc) * dx, i = 0..N-1: largest magnitude per axis
+    const double ax = fmax(fabs((0.0 - ocxB) * dx), fabs((XN - 1.0 - ocxB) * dx));
+    const double ay = fmax(fabs((0.0 - ocyB) * dx), fabs((YN - 1.0 - ocyB) * dx));
+    const double rv = sqrt(ax * ax + ay * ay);      // bounding radius of the voxel centres
+    const double rs = sqrt((double)x0 * x0 + (double)y0 * y0);  // source radius
+    if (!(rs > 0.0) || !isfinite(rs) || !isfinite(rv)) return 0;
+    const double dmin = rs - rv, dmax = rs + rv;  // voxel-source distance
+    // ddv is the offset along the axis where the source's coordinate is the
+    // larger in magnitude (>= rs / sqrt(2)), so |ddv| >= rs / sqrt(2) - max(ax, ay).
+    const double vmin = rs * 0.7071067811865476 - fmax(ax, ay);
+    const double lo2 = 0x1p-18, hi2 = 0x1p38, lo1 = 0x1p-38, hi1 = 0x1p18;
+    // 1e-3 relative slack covers the FP32 rounding of the positions and sums
+    return dmin > 0.0 && 0.999 * dmin * dmin >= lo2 && 1.001 * dmax * dmax <= hi2 &&
+           0.999 * vmin >= lo1 && 1.001 * dmax <= hi1;
+}
+
 void dd_boundaries(int nb, const float* c, float* b) {
     if (nb >= 3) {
         for (int i = 0; i < nb; ++i) {

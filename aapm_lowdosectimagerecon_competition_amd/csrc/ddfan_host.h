@@ -23,6 +23,21 @@ float regularize_angle(float c);
 // utilities.cuh:218-227: (cosf, sinf) of the regularised angle, interleaved.
 void cossin_table(const float* hangs, size_t n, float* cs);
 
+// MultiSlicesBackGPU.cu:154-157: the source position of each (view, slice),
+// (x0, y0) rotated by its (cos, sin) pair from cossin_table, as interleaved
+// (x, y) pairs.  The backprojectors read it instead of the angle.
+void source_table(const float* cs, size_t n, float x0, float y0, float* src);
+
+// 1 when, for every voxel centre of the XN x YN grid and every source on the
+// circle through (x0, y0), the backprojector's squared voxel-source distance
+// lies in [2^-18, 2^38] and its view-axis offset ddv (MultiSlicesBackGPU.cu:
+// 166-170) in [2^-38, 2^18] in magnitude: inside the range where the short
+// correctly rounded square root and quotients need no per-lane check, with a
+// factor 4 to spare.  It holds when the source is farther from the centre
+// than sqrt(2) times the grid's largest half-extent (any CT geometry whose
+// source lies outside the grid) at millimetre-like scales; 0 otherwise.
+int back_mid_range(int XN, int YN, float dx, float ocxB, float ocyB, float x0, float y0);
+
 // utilities.cuh:268-287 DD3Boundaries<float>.
 void dd_boundaries(int nb, const float* centres, float* bnd);
 

@@ -145,25 +145,35 @@ DDF_HD float slab_r(float obj, float a, float f, float b) { return (obj - a) * f
 DDF_HD float slab_coord(float r, float invdx, float ocK) { return (r * invdx + ocK) + 1.0f; }
 
 // Backprojector per (voxel, view) detector coordinates
-// (MultiSlicesBackGPU.cu:155-176).  vx, vy: voxel centre in mm.
+// (MultiSlicesBackGPU.cu:159-176).  vx, vy: voxel centre in mm; srcx, srcy:
+// the view's source position (MultiSlicesBackGPU.cu:155-157, source_table).
+// mid: the geometry keeps every operand of the footprint's square root and
+// quotients in the short sequences' range (back_mid_range), so no lane needs
+// checking.
 struct BackSample {
     float idxL, idxR;  // texture coordinates along the detector SAT
     float ddv, l2, l;  // for the continuous weight dx/ddv*sqrt(l2)
 };
 
-DDF_HD BackSample back_sample(float vx, float vy, float c, float s, float x0, float y0,
-                              float inv_sid, float invdb, float detCntIdx, float dx) {
-    const float srcx = x0 * c - y0 * s;
-    const float srcy = x0 * s + y0 * c;
+DDF_HD BackSample back_sample(float vx, float vy, float srcx, float srcy, float inv_sid,
+                              float invdb, float detCntIdx, float dx, int mid) {
     const float ddx = vx - srcx, ddy = vy - srcy;
     BackSample b;
     b.l2 = ddx * ddx + ddy * ddy;
-    b.ddv = (fabsf(srcx) > fabsf(srcy)) ? ddx : ddy;
+#if defined(__HIP_DEVICE_COMPILE__)
+    // |srcx| > |srcy| on the magnitude bits (the same order for finite
+    // values), so a view-uniform source stays on the scalar unit
+    const bool xmaj = (__float_as_uint(srcx) & 0x7fffffffu) > (__float_as_uint(srcy) & 0x7fffffffu);
+#else
+    const bool xmaj = fabsf(srcx) > fabsf(srcy);
+#endif
+    b.ddv = xmaj ? ddx : ddy;
     float q;  // ddv / l2
 #if defined(__HIP_DEVICE_COMPILE__)
-    // the short correctly rounded sequences when every lane of the wave is in
-    // their range (a wave-uniform branch), else the compiler's full ones
-    if (__builtin_amdgcn_ballot_w64(!(rn_mid_divisor(b.l2) && rn_mid_dividend(b.ddv))) == 0) {
+    // the short correctly rounded sequences when the geometry, or else every
+    // lane of the wave (a wave-uniform branch), is in their range; else the
+    // compiler's full ones
+    if (mid || __builtin_amdgcn_ballot_w64(!(rn_mid_divisor(b.l2) && rn_mid_dividend(b.ddv))) == 0) {
         b.l = div_rn_mid(1.0f, sqrt_rn_mid(b.l2));
         q = div_rn_mid(b.ddv, b.l2);
     } else {
@@ -171,6 +181,7 @@ DDF_HD BackSample back_sample(float vx, float vy, float c, float s, float x0, fl
         q = b.ddv / b.l2;
     }
 #else
+    (void)mid;
     b.l = 1.0f / sqrtf(b.l2);
     q = b.ddv / b.l2;
 #endif

@@ -26,6 +26,7 @@ struct GeomDev {
     float invdb;       // 1/dbeta (MultiSlicesBackGPU.cu:150)
     float detCntIdx;
     int XN, YN, SLN, DNU;
+    int backMid;  // back_mid_range(): the backprojector's footprint needs no range check
 };
 
 // Views first, first+stride, ... (count of them) of the plan's angle table.
@@ -86,7 +87,7 @@ struct BackJob {
     GeomDev g;
     Subset sub;
     int nzc, ZT, epi;
-    const float2* cossin;
+    const float2* src;  // source positions [view][slice] (source_table)
     const uint8_t* mask;
     const void* P;
     float* vol;
@@ -122,7 +123,7 @@ hipError_t launch_proj(int mode, int ZT, int epi, const GeomDev& g, int nzc, con
                        int stage, hipStream_t st, const float* hcs = nullptr);
 // viewPass > 0 (plain epilogue only): launches of at most viewPass views each,
 // every one continuing the sums of the previous (same results as one launch).
-hipError_t launch_back(int mode, int ZT, int epi, const GeomDev& g, int nzc, const float2* cossin,
+hipError_t launch_back(int mode, int ZT, int epi, const GeomDev& g, int nzc, const float2* src,
                        const uint8_t* mask, const void* P, Subset sub, float* vol,
                        const float* colSum, float* xprev, const float* momTab,
                        const int* iterCounter, int stageViews, hipStream_t st, int viewPass = 0);
@@ -139,7 +140,7 @@ hipError_t launch_fill(float* p, float v, size_t n, hipStream_t st);
 // kernels above): projector [j][u][z][slab][2], backprojector [y][x][z][j][2].
 hipError_t launch_proj_trace(const GeomDev& g, const float2* cossin, const float* bx,
                              const float* by, Subset sub, int32_t* trace, hipStream_t st);
-hipError_t launch_back_trace(const GeomDev& g, const float2* cossin, Subset sub, int32_t* trace,
+hipError_t launch_back_trace(const GeomDev& g, const float2* src, Subset sub, int32_t* trace,
                              hipStream_t st);
 
 // Helical-to-fan rebin parameters (HelicalToFanFunc_mex.cpp:160-164).

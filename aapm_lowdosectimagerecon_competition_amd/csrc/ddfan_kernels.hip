@@ -637,7 +637,7 @@ struct BackDev {
     int nzc, ntx, nty, blk0, epi;
     int tabViews;  // views per slice chunk of the table P (>= sub.count)
     int accum;     // 1: the sums continue the values in vol (a later view pass)
-    const float2* cossin;
+    const float2* src;  // source positions [view][slice]
     const uint8_t* mask;
     const void* P;
     float* vol;
@@ -717,9 +717,9 @@ void k_back(const BackLaunch L) {
         const float vy = ((float)y - g.ocyB) * g.dx;
         for (int j = 0; j < J.sub.count; ++j) {
             const int vg = J.sub.first + j * J.sub.stride;
-            const float2 cs = J.cossin[(size_t)vg * g.SLN + zg];
+            const float2 sp = J.src[(size_t)vg * g.SLN + zg];
             const BackSample bs =
-                back_sample(vx, vy, cs.x, cs.y, g.x0, g.y0, g.inv_sid, g.invdb, g.detCntIdx, g.dx);
+                back_sample(vx, vy, sp.x, sp.y, g.inv_sid, g.invdb, g.detCntIdx, g.dx, g.backMid);
             const Fetch fL = fetch_pos<E>(bs.idxL, DNU), fR = fetch_pos<E>(bs.idxR, DNU);
             // dx / ddv * sqrt(l2) with sqrt(l2) = 1 / l  (MultiSlicesBackGPU.cu:173)
             const float cosVal = g.dx * __builtin_amdgcn_rcpf(bs.ddv * bs.l);
@@ -883,9 +883,9 @@ void k_back_lds(const BackLaunch L) {
         auto windows16 = [&](int v0) __attribute__((always_inline)) {
             const int j = min(v0 + (lane >> 2), count - 1);
             const int vg = J.sub.first + j * J.sub.stride;
-            const float2 cs = J.cossin[(size_t)vg * g.SLN + zg];
-            const BackSample bs = back_sample(cvx, cvy, cs.x, cs.y, g.x0, g.y0, g.inv_sid, g.invdb,
-                                              g.detCntIdx, g.dx);
+            const float2 sp = J.src[(size_t)vg * g.SLN + zg];
+            const BackSample bs = back_sample(cvx, cvy, sp.x, sp.y, g.inv_sid, g.invdb,
+                                              g.detCntIdx, g.dx, g.backMid);
             const int lo = quad_min(texel(fminf(bs.idxL, bs.idxR), DNU));
             const int hi = quad_max(texel(fmaxf(bs.idxL, bs.idxR), DNU));
             const int s = max(lo - 1, 0), e = min(hi + 1, DNU);
@@ -941,9 +941,9 @@ void k_back_lds(const BackLaunch L) {
                 const int j = grp * NV + v;
                 if (j >= count) break;
                 const int vg = J.sub.first + j * J.sub.stride;
-                const float2 cs = uniform_ld(J.cossin, (size_t)vg * g.SLN + zg);
-                const BackSample bs = back_sample(vx, vy, cs.x, cs.y, g.x0, g.y0, g.inv_sid,
-                                                  g.invdb, g.detCntIdx, g.dx);
+                const float2 sp = uniform_ld(J.src, (size_t)vg * g.SLN + zg);
+                const BackSample bs = back_sample(vx, vy, sp.x, sp.y, g.inv_sid, g.invdb,
+                                                  g.detCntIdx, g.dx, g.backMid);
                 const Fetch fL = fetch_pos<E>(bs.idxL, DNU), fR = fetch_pos<E>(bs.idxR, DNU);
                 const float cosVal = g.dx * __builtin_amdgcn_rcpf(bs.ddv * bs.l);
                 const int iL = fL.i0 - w0[v];
@@ -1406,7 +1406,7 @@ __global__ __launch_bounds__(256) void k_proj_trace(const GeomDev g, const float
     }
 }
 
-__global__ __launch_bounds__(256) void k_back_trace(const GeomDev g, const float2* __restrict__ cossin,
+__global__ __launch_bounds__(256) void k_back_trace(const GeomDev g, const float2* __restrict__ src,
                                                     const Subset sub, int32_t* __restrict__ trace) {
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long n = (long long)g.XN * g.YN * g.SLN;
@@ -1419,9 +1419,9 @@ __global__ __launch_bounds__(256) void k_back_trace(const GeomDev g, const float
     int32_t* tr = trace + (size_t)idx * sub.count * 2;
     for (int j = 0; j < sub.count; ++j) {
         const int vg = sub.first + j * sub.stride;
-        const float2 cs = cossin[(size_t)vg * g.SLN + z];
+        const float2 sp = src[(size_t)vg * g.SLN + z];
         const BackSample bs =
-            back_sample(vx, vy, cs.x, cs.y, g.x0, g.y0, g.inv_sid, g.invdb, g.detCntIdx, g.dx);
+            back_sample(vx, vy, sp.x, sp.y, g.inv_sid, g.invdb, g.detCntIdx, g.dx, g.backMid);
         tr[2 * j] = texel(bs.idxL, g.DNU);
         tr[2 * j + 1] = texel(bs.idxR, g.DNU);
     }
@@ -1998,7 +1998,7 @@ static hipError_t launch_back_group(int mode, int ZT, int NV, const BackJob* con
         d.epi = J.epi;
         d.tabViews = J.tabViews > 0 ? J.tabViews : J.sub.count;
         d.accum = J.accum;
-        d.cossin = J.cossin;
+        d.src = J.src;
         d.mask = J.mask;
         d.P = J.P;
         d.vol = J.vol;
@@ -2050,12 +2050,12 @@ hipError_t launch_back_batch(int mode, const BackJob* jobs, int n, hipStream_t s
     return hipSuccess;
 }
 
-hipError_t launch_back(int mode, int ZT, int epi, const GeomDev& g, int nzc, const float2* cossin,
+hipError_t launch_back(int mode, int ZT, int epi, const GeomDev& g, int nzc, const float2* src,
                        const uint8_t* mask, const void* P, Subset sub, float* vol,
                        const float* colSum, float* xprev, const float* momTab,
                        const int* iterCounter, int stageViews, hipStream_t st, int viewPass) {
     if (viewPass <= 0 || viewPass >= sub.count || epi != BACK_PLAIN) {
-        const BackJob j{g,      sub,   nzc,    ZT,          epi,       cossin, mask, P,
+        const BackJob j{g,      sub,   nzc,    ZT,          epi,       src, mask, P,
                         vol,    colSum, xprev, momTab,      iterCounter, stageViews};
         return launch_back_batch(mode, &j, 1, st);
     }
@@ -2066,7 +2066,7 @@ hipError_t launch_back(int mode, int ZT, int epi, const GeomDev& g, int nzc, con
     for (int j0 = 0; j0 < sub.count; j0 += viewPass) {
         const Subset sp{sub.first + j0 * sub.stride, sub.stride, std::min(viewPass, sub.count - j0)};
         const void* Pp = static_cast<const char*>(P) + (size_t)j0 * ZT * (g.DNU + 1) * eb;
-        BackJob j{g,   sp,     nzc,   ZT,     epi,         cossin,     mask, Pp,
+        BackJob j{g,   sp,     nzc,   ZT,     epi,         src,        mask, Pp,
                   vol, colSum, xprev, momTab, iterCounter, stageViews};
         j.tabViews = sub.count;
         j.accum = j0 > 0;
@@ -2095,11 +2095,11 @@ hipError_t launch_proj_trace(const GeomDev& g, const float2* cossin, const float
     return hipGetLastError();
 }
 
-hipError_t launch_back_trace(const GeomDev& g, const float2* cossin, Subset sub, int32_t* trace,
+hipError_t launch_back_trace(const GeomDev& g, const float2* src, Subset sub, int32_t* trace,
                              hipStream_t st) {
     const long long n = (long long)g.XN * g.YN * g.SLN;
     if (n == 0) return hipSuccess;
-    k_back_trace<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(g, cossin, sub, trace);
+    k_back_trace<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(g, src, sub, trace);
     return hipGetLastError();
 }
 

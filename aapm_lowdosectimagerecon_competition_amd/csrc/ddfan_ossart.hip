@@ -47,12 +47,14 @@ int ossart_prepare(OsJob& J, hipStream_t st) {
     // with the slice-0 angles.
     GeomDev g0 = g;
     g0.SLN = 1;
-    J.h_cs0.resize((size_t)P->PN * 2);
+    J.h_cs0.resize((size_t)P->PN * 4);
     for (int v = 0; v < P->PN; ++v) {
         J.h_cs0[2 * v] = P->h_cs[2 * ((size_t)v * SLN)];
         J.h_cs0[2 * v + 1] = P->h_cs[2 * ((size_t)v * SLN) + 1];
     }
+    source_table(J.h_cs0.data(), P->PN, P->g.x0, P->g.y0, J.h_cs0.data() + 2 * (size_t)P->PN);
     TRY(upload(P->cs0, J.h_cs0.data(), sizeof(float) * J.h_cs0.size(), st));
+    const float2* src0 = P->cs0.as<float2>() + P->PN;
     HIPCALL(P->onesV.reserve(sizeof(float) * npix));
     HIPCALL(P->onesP.reserve(sizeof(float) * (size_t)maxc * DNU));
     HIPCALL(P->satX0.reserve(sat_vol_bytes(mode, 1, 1, XN, YN)));
@@ -77,7 +79,7 @@ int ossart_prepare(OsJob& J, hipStream_t st) {
                              P->bx.as<float>(), P->by.as<float>(), P->xds.as<float>(),
                              P->yds.as<float>(), P->satX0.p, P->satY0.p,
                              P->rowSum.as<float>() + (size_t)k * maxc * DNU, nullptr, nullptr, 0});
-        bj.push_back(BackJob{g0, sub, 1, 1, BACK_PLAIN, P->cs0.as<float2>(),
+        bj.push_back(BackJob{g0, sub, 1, 1, BACK_PLAIN, src0,
                              P->mask.as<uint8_t>(), P->satP0.p,
                              P->colSum.as<float>() + (size_t)k * npix, nullptr, nullptr, nullptr,
                              nullptr, 0});
@@ -144,7 +146,7 @@ hipError_t enqueue_batch_iteration(OsJob* const* jobs, int n, int mode, const Ba
                                  P->projStage, P->h_cs.data()});
             ss.push_back(SatSinoJob{g.DNU, g.SLN, P->nzcb, P->ZTb, J.cnt[k], P->rbuf.as<float>(),
                                     P->satP.p});
-            bj.push_back(BackJob{g, sub, P->nzcb, P->ZTb, epi, P->cs.as<float2>(),
+            bj.push_back(BackJob{g, sub, P->nzcb, P->ZTb, epi, P->src.as<float2>(),
                                  P->mask.as<uint8_t>(), P->satP.p, J.x,
                                  P->colSum.as<float>() + (size_t)k * npix, P->xprev.as<float>(),
                                  P->momTab.as<float>(), P->counter.as<int>(), P->backStage});
@@ -488,7 +490,7 @@ int batch_tune(ddfan_batch* B, int reps, double* out_ms) {
                 const Subset sub{ks[i][s], sh[i]->numOS, cnt[i][s]};
                 ss.push_back(SatSinoJob{P->g.DNU, P->g.SLN, P->nzcb, P->ZTb, cnt[i][s],
                                         sino[i].as<float>(), P->satP.p});
-                bj.push_back(BackJob{P->g, sub, P->nzcb, P->ZTb, BACK_PLAIN, P->cs.as<float2>(),
+                bj.push_back(BackJob{P->g, sub, P->nzcb, P->ZTb, BACK_PLAIN, P->src.as<float2>(),
                                      P->mask.as<uint8_t>(), P->satP.p, vol[i].as<float>(), nullptr,
                                      nullptr, nullptr, nullptr, P->backStage});
             }

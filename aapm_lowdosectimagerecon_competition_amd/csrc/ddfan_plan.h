@@ -110,9 +110,10 @@ struct ddfan_plan {
     int projStage = 0;     // 0: per-lane gathers, 1: staged runs, 2/3: whole rows (4/2 slices)
     int backPass = 0;      // backprojector views per pass of a plain call (0: one pass)
     std::vector<float> h_hangs, h_cs;
-    ddf::DBuf cs, bx, by, xds, yds, mask;
+    ddf::DBuf cs, src, bx, by, xds, yds, mask;  // src: source positions (source_table)
     ddf::DBuf satX, satY, satP;
     // OS-SART state
+    // cs0: slice 0's (cos, sin) pairs of every view, then its source positions
     ddf::DBuf rbuf, rowSum, colSum, xprev, momTab, counter, cs0, onesV, onesP, satX0, satY0, satP0;
     bool timing = false;
     struct Ev {

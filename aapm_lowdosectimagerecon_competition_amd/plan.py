@@ -209,6 +209,12 @@ class Plan:
     def back_pass(self, v):
         self.set_param(N.PARAM_BACK_PASS, v)
 
+    @property
+    def back_mid_range(self):
+        """1 when the geometry lets the backprojector's footprint skip its
+        per-lane operand-range check (read-only)."""
+        return self.get_param(N.PARAM_BACK_MID_RANGE)
+
     def autotune(self, views=None, reps=3, use_cache=True) -> dict:
         """Times launch shapes for calls of `views` views (default PN), sets
         the fastest and records it (see tune_save / tune_load)."""

@@ -190,6 +190,13 @@ int ddfan_plan_query(const ddfan_plan* plan, int* slice_chunk, int* n_chunks, in
  * as launches of at most this many views, each continuing the sums the
  * previous one stored (0 = one launch).  Results are bit-identical. */
 #define DDFAN_PARAM_BACK_PASS 7
+/* DDFAN_PARAM_BACK_MID_RANGE (read-only): 1 when the geometry keeps every
+ * voxel-source distance and view-axis offset of the backprojector's footprint
+ * inside the range where its correctly rounded square root and quotients run
+ * as their short sequences with no per-lane check (any source farther from the
+ * grid centre than sqrt(2) times its largest half-extent); 0: each wave checks
+ * its lanes.  Results are the same either way. */
+#define DDFAN_PARAM_BACK_MID_RANGE 8
 int ddfan_plan_set_param(ddfan_plan* plan, int param, int value);
 int ddfan_plan_get_param(const ddfan_plan* plan, int param, int* value);
 /* Autotuner: times the projector over slice chunks x views per wave x LDS

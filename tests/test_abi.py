@@ -21,6 +21,19 @@ def _header_functions():
     return sorted(set(re.findall(r"\b(ddfan_\w+)\s*\(", text)))
 
 
+def test_binding_constants_match_header():
+    """Every integer #define of include/ddfan.h has the same value in the
+    Python binding (status codes keep their DDFAN_ prefix, the rest drop it)."""
+    from aapm_lowdosectimagerecon_competition_amd import _native as N
+    text = open(os.path.join(ROOT, "include", "ddfan.h")).read()
+    defs = re.findall(r"^#define (DDFAN_\w+) \(?(-?\d+)\)?", text, flags=re.M)
+    assert len(defs) >= 30
+    for name, val in defs:
+        attr = name if hasattr(N, name) else name[len("DDFAN_"):]
+        assert hasattr(N, attr), name
+        assert getattr(N, attr) == int(val), name
+
+
 def test_library_exports_every_declared_symbol():
     from aapm_lowdosectimagerecon_competition_amd import _native as N
     L = N.lib()
