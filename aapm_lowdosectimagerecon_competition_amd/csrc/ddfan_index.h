@@ -50,6 +50,24 @@ DDF_HD float asinf_cephes(float xx) {
 }
 
 #if defined(__HIPCC__)
+// asinf_cephes for a wave: when no lane's |x| exceeds 0.5 (a NaN fails the
+// test as well), the polynomial branch alone, with the sign copied onto the
+// positive result; the same bits as asinf_cephes, without the select and
+// checks of the other branches.  Else every lane takes asinf_cephes.
+__device__ __forceinline__ float asinf_cephes_wave(float xx) {
+    const float a = fabsf(xx);
+    if (__builtin_amdgcn_ballot_w64(!(a <= 0.5f)) == 0) {
+        const float z = a * a;
+        float p = fmaf(4.2163199048e-2f, z, 2.4181311049e-2f);
+        p = fmaf(p, z, 4.5470025998e-2f);
+        p = fmaf(p, z, 7.4953002686e-2f);
+        p = fmaf(p, z, 1.6666752422e-1f);
+        const float r = fmaf(p * z, a, a);  // > 0 for a >= 1e-4
+        return a < 1.0e-4f ? xx : copysignf(r, xx);
+    }
+    return asinf_cephes(xx);
+}
+
 // Correctly rounded square root and quotient for operands in a middle range.
 // The library is compiled with -fhip-fp32-correctly-rounded-divide-sqrt, whose
 // expansions of sqrtf and '/' rescale operands near the ends of the exponent
@@ -185,7 +203,11 @@ DDF_HD BackSample back_sample(float vx, float vy, float srcx, float srcy, float 
     b.l = 1.0f / sqrtf(b.l2);
     q = b.ddv / b.l2;
 #endif
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float alpha = asinf_cephes_wave(((srcy * ddx - srcx * ddy) * inv_sid) * b.l);
+#else
     const float alpha = asinf_cephes(((srcy * ddx - srcx * ddy) * inv_sid) * b.l);
+#endif
     const float dA = (q * dx) * 0.5f;
     b.idxL = ((alpha - dA) * invdb + detCntIdx) + 1.0f;
     b.idxR = ((alpha + dA) * invdb + detCntIdx) + 1.0f;

@@ -4,7 +4,10 @@
 // whole operand range: every float x in [2^-20, 2^40] for the square root and
 // the reciprocal 1/sqrt(x), and for the quotient every divisor of two full
 // binades at the range ends and the middle against 4096 dividends spread over
-// [2^-40, 2^20] (both signs, and 0), plus 2^26 pseudo-random pairs.
+// [2^-40, 2^20] (both signs, and 0), plus 2^26 pseudo-random pairs.  And the
+// wave form of the Cephes arcsine against asinf_cephes on every float bit
+// pattern (consecutive patterns per wave, so waves run both of its paths and
+// the small-argument select inside the fast one).
 // Prints the number of inputs whose results differ in any bit.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -71,6 +74,18 @@ __global__ void k_div_rand(uint32_t n, unsigned long long* bad) {
     if (nb) atomicAdd(bad, nb);
 }
 
+// every 32-bit pattern: u = base + i for i over the grid-stride range
+__global__ void k_asin(unsigned long long* bad) {
+    unsigned long long nb = 0;
+    const unsigned long long n = 1ull << 32;
+    for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (unsigned long long)gridDim.x * blockDim.x) {
+        const float x = __uint_as_float((uint32_t)i);
+        nb += __float_as_uint(asinf_cephes_wave(x)) != __float_as_uint(asinf_cephes(x));
+    }
+    if (nb) atomicAdd(bad, nb);
+}
+
 #define CK(x)                                                                   \
     do {                                                                        \
         hipError_t e_ = (x);                                                    \
@@ -96,13 +111,16 @@ int main() {
     CK(hipGetLastError());
     k_div_rand<<<4096, 256>>>(1u << 26, d_bad + 2);
     CK(hipGetLastError());
+    k_asin<<<8192, 256>>>(d_bad + 3);
+    CK(hipGetLastError());
     unsigned long long bad[4];
     CK(hipMemcpy(bad, d_bad, sizeof(bad), hipMemcpyDeviceToHost));
     printf("sqrt and reciprocal mismatches: %llu of %llu\n", bad[0], 2ull * (hi + 1 - lo));
     printf("quotient grid mismatches: %llu of %llu\n", bad[1],
            4096ull * ((b1 - b0) + (m1 - m0) + (e1 + 1 - e0)));
     printf("quotient random mismatches: %llu of %llu\n", bad[2], 1ull << 26);
+    printf("arcsine wave-form mismatches: %llu of %llu\n", bad[3], 1ull << 32);
     CK(hipFree(d_bad));
-    printf("%s\n", bad[0] + bad[1] + bad[2] == 0 ? "cr math ok" : "cr math MISMATCH");
+    printf("%s\n", bad[0] + bad[1] + bad[2] + bad[3] == 0 ? "cr math ok" : "cr math MISMATCH");
     return 0;
 }

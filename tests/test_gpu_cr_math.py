@@ -2,7 +2,8 @@
 (ddfan_index.h) give the bits of the compiler's correctly rounded sqrtf and
 '/' over their whole operand range (tests/native/cr_math_test.hip: every float
 in [2^-20, 2^40] for the square root and its reciprocal, 2e11 quotients over
-three full divisor binades and 2^26 random pairs)."""
+three full divisor binades and 2^26 random pairs), and the wave form of the
+Cephes arcsine gives asinf_cephes's bits on every float bit pattern."""
 import subprocess
 
 import pytest
