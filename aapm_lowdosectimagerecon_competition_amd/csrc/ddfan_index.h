@@ -68,15 +68,20 @@ __device__ __forceinline__ float asinf_cephes_wave(float xx) {
     return asinf_cephes(xx);
 }
 
-// Correctly rounded square root and quotient for operands in a middle range.
-// The library is compiled with -fhip-fp32-correctly-rounded-divide-sqrt, whose
-// expansions of sqrtf and '/' rescale operands near the ends of the exponent
-// range and fix up zeros, infinities and NaNs around the same core: the
-// hardware estimate (v_sqrt_f32, v_rcp_f32) and its residual corrections.
-// For a positive normal x in [2^-20, 2^40], and for a quotient whose divisor
-// lies there and whose dividend is 0 or of magnitude in [2^-40, 2^20], neither
-// the rescaling nor the fix-up changes anything, so the core alone gives the
-// same bits with 7 (square root) and 3 (quotient) fewer instructions.
+// Correctly rounded square root, reciprocal and quotient for operands in a
+// middle range.  The library is compiled with
+// -fhip-fp32-correctly-rounded-divide-sqrt, whose expansions of sqrtf and '/'
+// rescale operands near the ends of the exponent range and fix up zeros,
+// infinities and NaNs around the same core: the hardware estimate (v_sqrt_f32,
+// v_rcp_f32) and its residual corrections.  For a positive normal x in
+// [2^-20, 2^40], and for a quotient whose divisor lies there and whose dividend
+// is 0 or of magnitude in [2^-40, 2^20], neither the rescaling nor the fix-up
+// changes anything, so the core alone gives the same bits.  The reciprocal
+// needs one Newton step from the estimate (exhaustively checked: the correctly
+// rounded 1/x for every float x of that range), and the quotient then one
+// residual correction of q = a * (1/b) (P. Markstein, IA-64 and Elementary
+// Functions, 2000: with the correctly rounded reciprocal and q within an ulp,
+// q + (a - b q)(1/b) rounds to the correctly rounded quotient).
 __device__ __forceinline__ float sqrt_rn_mid(float x) {
     const float s = __builtin_amdgcn_sqrtf(x);
     const float sdn = __int_as_float(__float_as_int(s) - 1);
@@ -85,11 +90,13 @@ __device__ __forceinline__ float sqrt_rn_mid(float x) {
     r = fmaf(-sup, s, x) > 0.0f ? sup : r;
     return r;
 }
+__device__ __forceinline__ float rcp_rn_mid(float b) {
+    const float r = __builtin_amdgcn_rcpf(b);
+    return fmaf(fmaf(-b, r, 1.0f), r, r);
+}
 __device__ __forceinline__ float div_rn_mid(float a, float b) {
-    float r = __builtin_amdgcn_rcpf(b);
-    r = fmaf(fmaf(-b, r, 1.0f), r, r);
-    float q = a * r;
-    q = fmaf(fmaf(-b, q, a), r, q);
+    const float r = rcp_rn_mid(b);
+    const float q = a * r;
     return fmaf(fmaf(-b, q, a), r, q);
 }
 __device__ __forceinline__ bool rn_mid_divisor(float b) { return b >= 0x1p-20f && b <= 0x1p40f; }
@@ -192,7 +199,7 @@ DDF_HD BackSample back_sample(float vx, float vy, float srcx, float srcy, float 
     // lane of the wave (a wave-uniform branch), is in their range; else the
     // compiler's full ones
     if (mid || __builtin_amdgcn_ballot_w64(!(rn_mid_divisor(b.l2) && rn_mid_dividend(b.ddv))) == 0) {
-        b.l = div_rn_mid(1.0f, sqrt_rn_mid(b.l2));
+        b.l = rcp_rn_mid(sqrt_rn_mid(b.l2));
         q = div_rn_mid(b.ddv, b.l2);
     } else {
         b.l = 1.0f / sqrtf(b.l2);

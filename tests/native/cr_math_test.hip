@@ -1,8 +1,9 @@
-// The short correctly rounded square root and quotient of ddfan_index.h
-// against the compiler's correctly rounded sqrtf and '/' (this file is built
-// with -fhip-fp32-correctly-rounded-divide-sqrt, as the library is) over their
-// whole operand range: every float x in [2^-20, 2^40] for the square root and
-// the reciprocal 1/sqrt(x), and for the quotient every divisor of two full
+// The short correctly rounded square root, reciprocal and quotient of
+// ddfan_index.h against the compiler's correctly rounded sqrtf and '/' (this
+// file is built with -fhip-fp32-correctly-rounded-divide-sqrt, as the library
+// is) over their whole operand range: every float x in [2^-20, 2^40] for the
+// square root, the reciprocal 1/sqrt(x) and the reciprocal 1/x, and for the
+// quotient every divisor of two full
 // binades at the range ends and the middle against 4096 dividends spread over
 // [2^-40, 2^20] (both signs, and 0), plus 2^26 pseudo-random pairs.  And the
 // wave form of the Cephes arcsine against asinf_cephes on every float bit
@@ -32,8 +33,9 @@ __global__ void k_sqrt(uint32_t lo, uint32_t hi, unsigned long long* bad) {
     for (uint32_t u = lo + blockIdx.x * blockDim.x + threadIdx.x; u < hi; u += gridDim.x * blockDim.x) {
         const float x = __uint_as_float(u);
         const float a = sqrt_rn_mid(x), b = sqrtf(x);
-        const float ra = div_rn_mid(1.0f, a), rb = 1.0f / b;
-        nb += (__float_as_uint(a) != __float_as_uint(b)) + (__float_as_uint(ra) != __float_as_uint(rb));
+        const float ra = rcp_rn_mid(a), rb = 1.0f / b;
+        nb += (__float_as_uint(a) != __float_as_uint(b)) + (__float_as_uint(ra) != __float_as_uint(rb)) +
+              (__float_as_uint(rcp_rn_mid(x)) != __float_as_uint(1.0f / x));
     }
     if (nb) atomicAdd(bad, nb);
 }
@@ -115,7 +117,7 @@ int main() {
     CK(hipGetLastError());
     unsigned long long bad[4];
     CK(hipMemcpy(bad, d_bad, sizeof(bad), hipMemcpyDeviceToHost));
-    printf("sqrt and reciprocal mismatches: %llu of %llu\n", bad[0], 2ull * (hi + 1 - lo));
+    printf("sqrt and reciprocal mismatches: %llu of %llu\n", bad[0], 3ull * (hi + 1 - lo));
     printf("quotient grid mismatches: %llu of %llu\n", bad[1],
            4096ull * ((b1 - b0) + (m1 - m0) + (e1 + 1 - e0)));
     printf("quotient random mismatches: %llu of %llu\n", bad[2], 1ull << 26);

@@ -1,7 +1,8 @@
-"""The backprojector's short correctly rounded square root and quotient
-(ddfan_index.h) give the bits of the compiler's correctly rounded sqrtf and
-'/' over their whole operand range (tests/native/cr_math_test.hip: every float
-in [2^-20, 2^40] for the square root and its reciprocal, 2e11 quotients over
+"""The backprojector's short correctly rounded square root, reciprocal and
+quotient (ddfan_index.h) give the bits of the compiler's correctly rounded sqrtf
+and '/' over their whole operand range (tests/native/cr_math_test.hip: every
+float in [2^-20, 2^40] for the square root, its reciprocal and the reciprocal
+of the float itself, 2e11 quotients over
 three full divisor binades and 2^26 random pairs), and the wave form of the
 Cephes arcsine gives asinf_cephes's bits on every float bit pattern."""
 import subprocess
