@@ -5,9 +5,11 @@
 // single operations that round identically on gfx950 and on the host:
 // + - * / sqrtf fmaf floorf only, no FMA contraction (the library is compiled
 // with -ffp-contract=off; fmaf appears only where it is written out), and
-// correctly rounded division and square root.  Where the reference relied on
-// CUDA fast-math intrinsics (rsqrtf, asinf, approximate divide) the
-// correctly rounded operation or the Cephes arcsine below is used instead.
+// correctly rounded division and square root (on the device, where operands
+// allow, as short sequences from the hardware estimates that give the same
+// bits).  Where the reference relied on CUDA fast-math intrinsics (rsqrtf,
+// asinf, approximate divide) the correctly rounded operation or the Cephes
+// arcsine below is used instead.
 #pragma once
 
 #include <math.h>
@@ -76,19 +78,19 @@ __device__ __forceinline__ float asinf_cephes_wave(float xx) {
 // v_rcp_f32) and its residual corrections.  For a positive normal x in
 // [2^-20, 2^40], and for a quotient whose divisor lies there and whose dividend
 // is 0 or of magnitude in [2^-40, 2^20], neither the rescaling nor the fix-up
-// changes anything, so the core alone gives the same bits.  The reciprocal
-// needs one Newton step from the estimate (exhaustively checked: the correctly
-// rounded 1/x for every float x of that range), and the quotient then one
-// residual correction of q = a * (1/b) (P. Markstein, IA-64 and Elementary
-// Functions, 2000: with the correctly rounded reciprocal and q within an ulp,
-// q + (a - b q)(1/b) rounds to the correctly rounded quotient).
+// changes anything, so a short sequence from the hardware estimate gives the
+// same bits (P. Markstein, IA-64 and Elementary Functions, 2000, residual
+// corrections with a fused multiply-add): the square root as
+// g = x y, g + (x - g g)(y / 2) from y = v_rsq_f32(x), the reciprocal as one
+// Newton step from v_rcp_f32 (both exhaustively checked: the correctly rounded
+// sqrt(x) and 1/x for every float x of that range), and the quotient as one
+// residual correction of q = a * (1/b) (with the correctly rounded reciprocal
+// and q within an ulp, q + (a - b q)(1/b) rounds to the correctly rounded
+// quotient).
 __device__ __forceinline__ float sqrt_rn_mid(float x) {
-    const float s = __builtin_amdgcn_sqrtf(x);
-    const float sdn = __int_as_float(__float_as_int(s) - 1);
-    const float sup = __int_as_float(__float_as_int(s) + 1);
-    float r = fmaf(-sdn, s, x) <= 0.0f ? sdn : s;
-    r = fmaf(-sup, s, x) > 0.0f ? sup : r;
-    return r;
+    const float y = __builtin_amdgcn_rsqf(x);
+    const float g = x * y, h = 0.5f * y;
+    return fmaf(fmaf(-g, g, x), h, g);
 }
 __device__ __forceinline__ float rcp_rn_mid(float b) {
     const float r = __builtin_amdgcn_rcpf(b);
