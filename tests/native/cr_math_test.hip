@@ -8,7 +8,10 @@
 // [2^-40, 2^20] (both signs, and 0), plus 2^26 pseudo-random pairs.  And the
 // wave form of the Cephes arcsine against asinf_cephes on every float bit
 // pattern (consecutive patterns per wave, so waves run both of its paths and
-// the small-argument select inside the fast one).
+// the small-argument select inside the fast one).  And the clamp of the DD
+// kernels' fetch position, floor(c - 0.5) into [0, N] as one v_med3_f32,
+// against fminf(fmaxf()) with the weight it leaves, on every float c for
+// table sizes N of 1, 512, 736 and 776.
 // Prints the number of inputs whose results differ in any bit.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -88,6 +91,20 @@ __global__ void k_asin(unsigned long long* bad) {
     if (nb) atomicAdd(bad, nb);
 }
 
+// fetch position (texel index and weight) with both clamps, every pattern c
+__global__ void k_clamp(int N, unsigned long long* bad) {
+    unsigned long long nb = 0;
+    for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+         i < (1ull << 32); i += (unsigned long long)gridDim.x * blockDim.x) {
+        const float cm = __uint_as_float((uint32_t)i) - 0.5f;
+        const float fa = fminf(fmaxf(floorf(cm), 0.0f), (float)N);
+        const float fb = __builtin_amdgcn_fmed3f(floorf(cm), 0.0f, (float)N);
+        const float ta = fminf(fmaxf(cm - fa, 0.0f), 1.0f), tb = fminf(fmaxf(cm - fb, 0.0f), 1.0f);
+        nb += (int)fa != (int)fb || __float_as_uint(ta) != __float_as_uint(tb);
+    }
+    if (nb) atomicAdd(bad, nb);
+}
+
 #define CK(x)                                                                   \
     do {                                                                        \
         hipError_t e_ = (x);                                                    \
@@ -99,8 +116,8 @@ __global__ void k_asin(unsigned long long* bad) {
 
 int main() {
     unsigned long long* d_bad = nullptr;
-    CK(hipMalloc(&d_bad, 4 * sizeof(unsigned long long)));
-    CK(hipMemset(d_bad, 0, 4 * sizeof(unsigned long long)));
+    CK(hipMalloc(&d_bad, 5 * sizeof(unsigned long long)));
+    CK(hipMemset(d_bad, 0, 5 * sizeof(unsigned long long)));
     const uint32_t lo = __builtin_bit_cast(uint32_t, 0x1p-20f), hi = __builtin_bit_cast(uint32_t, 0x1p40f);
     k_sqrt<<<4096, 256>>>(lo, hi + 1, d_bad);
     CK(hipGetLastError());
@@ -115,14 +132,17 @@ int main() {
     CK(hipGetLastError());
     k_asin<<<8192, 256>>>(d_bad + 3);
     CK(hipGetLastError());
-    unsigned long long bad[4];
+    for (int N : {1, 512, 736, 776}) k_clamp<<<8192, 256>>>(N, d_bad + 4);
+    CK(hipGetLastError());
+    unsigned long long bad[5];
     CK(hipMemcpy(bad, d_bad, sizeof(bad), hipMemcpyDeviceToHost));
     printf("sqrt and reciprocal mismatches: %llu of %llu\n", bad[0], 3ull * (hi + 1 - lo));
     printf("quotient grid mismatches: %llu of %llu\n", bad[1],
            4096ull * ((b1 - b0) + (m1 - m0) + (e1 + 1 - e0)));
     printf("quotient random mismatches: %llu of %llu\n", bad[2], 1ull << 26);
     printf("arcsine wave-form mismatches: %llu of %llu\n", bad[3], 1ull << 32);
+    printf("fetch clamp mismatches: %llu of %llu\n", bad[4], 4ull << 32);
     CK(hipFree(d_bad));
-    printf("%s\n", bad[0] + bad[1] + bad[2] + bad[3] == 0 ? "cr math ok" : "cr math MISMATCH");
+    printf("%s\n", bad[0] + bad[1] + bad[2] + bad[3] + bad[4] == 0 ? "cr math ok" : "cr math MISMATCH");
     return 0;
 }

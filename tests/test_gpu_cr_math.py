@@ -4,7 +4,9 @@ and '/' over their whole operand range (tests/native/cr_math_test.hip: every
 float in [2^-20, 2^40] for the square root, its reciprocal and the reciprocal
 of the float itself, 2e11 quotients over
 three full divisor binades and 2^26 random pairs), and the wave form of the
-Cephes arcsine gives asinf_cephes's bits on every float bit pattern."""
+Cephes arcsine gives asinf_cephes's bits on every float bit pattern, and the
+fetch position's clamp as v_med3_f32 equals the fminf(fmaxf()) clamp on every
+float."""
 import subprocess
 
 import pytest
