@@ -166,6 +166,24 @@ struct SatLaunch {
     SatDev j[2 * kBatch];
 };
 
+// Table entries leave the row builder as non-temporal stores: its tables are
+// written once, far larger than L2, and read by the next kernel.  Measured on
+// MI355X, 512x512x64, 2304 views, two-float: sinogram table 0.536 -> 0.502 ms;
+// the tiled builder's volume tables 0.155 -> 0.215 ms with them (not used
+// there).
+typedef float nt_f4 __attribute__((ext_vector_type(4)));
+typedef float nt_f2 __attribute__((ext_vector_type(2)));
+typedef double nt_d2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void store_nt(float4* p, float4 v) {
+    __builtin_nontemporal_store(nt_f4{v.x, v.y, v.z, v.w}, reinterpret_cast<nt_f4*>(p));
+}
+__device__ __forceinline__ void store_nt(float2* p, float2 v) {
+    __builtin_nontemporal_store(nt_f2{v.x, v.y}, reinterpret_cast<nt_f2*>(p));
+}
+__device__ __forceinline__ void store_nt(double2* p, double2 v) {
+    __builtin_nontemporal_store(nt_d2{v.x, v.y}, reinterpret_cast<nt_d2*>(p));
+}
+
 // Value of lane - d of the same wave (callers use it for lane >= d only).
 __device__ __forceinline__ float lane_up(float v, int d) { return __shfl_up(v, d); }
 __device__ __forceinline__ double lane_up(double v, int d) { return __shfl_up(v, d); }
@@ -259,7 +277,7 @@ __global__ __launch_bounds__(256) void k_sat_rows(const SatLaunch L) {
         E* dst = base + (((size_t)zc * jb.nrows + row) * jb.ZT + zt) * K;
         const Lt* pq = pre + (size_t)q * K;
         const float* vq = val + (size_t)q * K;
-        for (int k = threadIdx.x; k < K; k += 256) dst[k] = S::entry(pq[k], vq[k]);
+        for (int k = threadIdx.x; k < K; k += 256) store_nt(dst + k, S::entry(pq[k], vq[k]));
     }
 }
 
