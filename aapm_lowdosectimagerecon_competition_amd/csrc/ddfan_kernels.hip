@@ -295,7 +295,11 @@ __global__ __launch_bounds__(256) void k_sat_rows(const SatLaunch L) {
 // The chunk's samples (tin) are dead once scanned, before its entries (tout)
 // are written, so the two share LDS: 19.2 KB per block at G = 32 (two-float),
 // eight blocks per CU, with the register budget of eight waves per SIMD.
-template <int MODE, int G>
+// V4 (volumes whose slice count is a multiple of 4, 16-byte aligned): each
+// thread loads four slices of one position with one 16-byte load, the whole
+// chunk in one pass instead of four rounds of 4-byte loads; the same samples
+// reach LDS, so the tables are the same.
+template <int MODE, int G, bool V4>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
 void k_sat_tiles(const SatLaunch L) {
     using S = Scan<MODE>;
@@ -342,11 +346,37 @@ void k_sat_tiles(const SatLaunch L) {
             }
         }
     };
-    fetch(0);
+    constexpr int Q = G / 4;  // V4: slice quads per position; KC = 256 / Q positions
+    const int lq = tid % Q, lkq = tid / Q;
+    const bool qin = z0 + 4 * lq < jb.SLN;  // a quad lies wholly inside or outside
+    float4 n4;
+    uint8_t nm4;
+    auto fetch4 = [&](int k0) __attribute__((always_inline)) {
+        const int k = k0 + lkq;
+        n4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        nm4 = 1;
+        if (qin && k < jb.Nk) {
+            const size_t p = (size_t)row * jb.rs + (size_t)k * jb.ks;
+            n4 = *reinterpret_cast<const float4*>(jb.in + p * jb.SLN + z0 + 4 * lq);
+            if (jb.mask) nm4 = jb.mask[p];
+        }
+    };
+    if constexpr (V4) fetch4(0);
+    else fetch(0);
     for (int k0 = 0; k0 < K; k0 += KC) {
+        if constexpr (V4) {
+            const float m = (float)nm4;
+            float* tr = &tin[lkq][4 * lq];
+            tr[0] = jb.mask ? n4.x * m : n4.x;
+            tr[1] = jb.mask ? n4.y * m : n4.y;
+            tr[2] = jb.mask ? n4.z * m : n4.z;
+            tr[3] = jb.mask ? n4.w * m : n4.w;
+            if (k0 + KC < K) fetch4(k0 + KC);
+        } else {
 #pragma unroll
-        for (int r = 0; r < LP; ++r) tin[r * P + lk][lz] = jb.mask ? nv[r] * (float)nm[r] : nv[r];
-        if (k0 + KC < K) fetch(k0 + KC);
+            for (int r = 0; r < LP; ++r) tin[r * P + lk][lz] = jb.mask ? nv[r] * (float)nm[r] : nv[r];
+            if (k0 + KC < K) fetch(k0 + KC);
+        }
         __syncthreads();
         S acc;
         Lt pre[LP];
@@ -1672,10 +1702,17 @@ static hipError_t launch_sat_tile_group(int mode, int G, SatDev* t, int n, hipSt
     if (nb > 0x7fffff00LL) return hipErrorInvalidValue;
     L.nwork = (int)nb;
     const unsigned grid = (unsigned)xcd_grid((int)nb);
-    if (G == 32) {
-        DDF_MODE_SWITCH(mode, (k_sat_tiles<MC, 32><<<grid, 256, 0, st>>>(L)));
+    bool v4 = true;  // every table's volume: slices a multiple of 4, 16-byte aligned
+    for (int i = 0; i < m; ++i)
+        v4 = v4 && L.j[i].SLN % 4 == 0 && reinterpret_cast<uintptr_t>(L.j[i].in) % 16 == 0;
+    if (G == 32 && v4) {
+        DDF_MODE_SWITCH(mode, (k_sat_tiles<MC, 32, true><<<grid, 256, 0, st>>>(L)));
+    } else if (G == 32) {
+        DDF_MODE_SWITCH(mode, (k_sat_tiles<MC, 32, false><<<grid, 256, 0, st>>>(L)));
+    } else if (v4) {
+        DDF_MODE_SWITCH(mode, (k_sat_tiles<MC, 16, true><<<grid, 256, 0, st>>>(L)));
     } else {
-        DDF_MODE_SWITCH(mode, (k_sat_tiles<MC, 16><<<grid, 256, 0, st>>>(L)));
+        DDF_MODE_SWITCH(mode, (k_sat_tiles<MC, 16, false><<<grid, 256, 0, st>>>(L)));
     }
     return hipGetLastError();
 }
