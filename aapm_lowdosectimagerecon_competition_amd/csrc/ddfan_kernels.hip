@@ -1717,19 +1717,25 @@ static hipError_t launch_sat_tile_group(int mode, int G, SatDev* t, int n, hipSt
     return hipGetLastError();
 }
 
-// Slices per block of the tiled builder for a volume table of SLN slices: 32
-// or 16, or 0 for the row-per-block builder (tables of fewer slices, and
-// sinogram tables).  The two builders group a row's sums differently, so the
-// choice follows the volume's own slice count, never its padded count
+// Slices per block of the tiled builder for a volume table of SLN slices: 16
+// or 32, or 0 for the row-per-block builder (tables of fewer than 16 slices,
+// and sinogram tables).  The two builders group a row's sums differently, so
+// the choice follows the volume's own slice count, never its padded count
 // nzc * ZT, which depends on the slice chunk (13 slices in chunks of 2 or of
 // 4 would otherwise get different tables).  The tiled builder's sums do not
-// depend on its slices per block.  Measured on MI355X at 512x512x64, 2304
-// views (two-float tables): volume tables 0.236 ms row per block, 0.157 ms
-// tiled; the sinogram table 0.534 ms row per block, 0.601 ms tiled (both
-// builders then move 3.6-4.3 TB/s, most of it written entries).
-static int sat_tile_group(const SatDev& t) {
-    if (!t.tiled) return 0;
-    return t.SLN >= 32 ? 32 : t.SLN >= 16 ? 16 : 0;
+// depend on its slices per block, so that choice is free: 16 for 16-byte
+// entries (two-float, f64), whose rows then leave as 1 KiB runs; for f32
+// entries 32 unless 16 pads the slice count less.  Measured on MI355X
+// (both volume tables per call, 32 -> 16 slices per block): 512x512x64
+// two-float 0.150 -> 0.144 ms, f32 0.097 -> 0.107 ms; 512x512x200 in one-slice
+// chunks two-float 0.593 -> 0.535 ms, f32 0.375 -> 0.345 ms.  Row per block
+// vs tiled at 512x512x64, 2304 views (two-float): volume tables 0.236 vs
+// 0.157 ms; the sinogram table 0.534 vs 0.601 ms (both builders then move
+// 3.6-4.3 TB/s, most of it written entries).
+static int sat_tile_group(int mode, const SatDev& t) {
+    if (!t.tiled || t.SLN < 16) return 0;
+    if (mode == SAT_F32 && t.SLN >= 32 && (t.SLN + 31) / 32 * 32 <= (t.SLN + 15) / 16 * 16) return 32;
+    return 16;
 }
 
 // Up to 2*kBatch tables.  The builder and its slices per block fix how a row
@@ -1741,7 +1747,7 @@ static hipError_t launch_sat_tables(int mode, SatDev* t, int n, hipStream_t st) 
         SatDev grp[2 * kBatch];
         int m = 0;
         for (int i = 0; i < n; ++i)
-            if (sat_tile_group(t[i]) == G) grp[m++] = t[i];
+            if (sat_tile_group(mode, t[i]) == G) grp[m++] = t[i];
         const hipError_t e = launch_sat_tile_group(mode, G, grp, m, st);
         if (e != hipSuccess) return e;
     }
@@ -1749,12 +1755,12 @@ static hipError_t launch_sat_tables(int mode, SatDev* t, int n, hipStream_t st) 
         SatDev grp[2 * kBatch];
         int m = 0;
         for (int i = 0; i < n; ++i)
-            if (sat_tile_group(t[i]) == 0 && sat_group(mode, t[i].Nk + 1) == G) grp[m++] = t[i];
+            if (sat_tile_group(mode, t[i]) == 0 && sat_group(mode, t[i].Nk + 1) == G) grp[m++] = t[i];
         const hipError_t e = launch_sat_group(mode, G, grp, m, st);
         if (e != hipSuccess) return e;
     }
     for (int i = 0; i < n; ++i)
-        if (sat_tile_group(t[i]) == 0 && sat_group(mode, t[i].Nk + 1) == 0)
+        if (sat_tile_group(mode, t[i]) == 0 && sat_group(mode, t[i].Nk + 1) == 0)
             return hipErrorInvalidValue;
     return hipSuccess;
 }
