@@ -361,9 +361,14 @@ void k_sat_tiles(const SatLaunch L) {
             if (jb.mask) nm4 = jb.mask[p];
         }
     };
+    // A row of Nk samples has Nk + 1 entries.  When Nk is a whole number of
+    // chunks the last entry, the row total, would be a chunk of its own: it is
+    // written instead with the carry of the chunk before, by the expression
+    // that chunk would have evaluated (the carry plus an empty part, sample 0).
+    const int kend = jb.Nk % KC == 0 ? jb.Nk : K;
     if constexpr (V4) fetch4(0);
     else fetch(0);
-    for (int k0 = 0; k0 < K; k0 += KC) {
+    for (int k0 = 0; k0 < kend; k0 += KC) {
         if constexpr (V4) {
             const float m = (float)nm4;
             float* tr = &tin[lkq][4 * lq];
@@ -371,11 +376,11 @@ void k_sat_tiles(const SatLaunch L) {
             tr[1] = jb.mask ? n4.y * m : n4.y;
             tr[2] = jb.mask ? n4.z * m : n4.z;
             tr[3] = jb.mask ? n4.w * m : n4.w;
-            if (k0 + KC < K) fetch4(k0 + KC);
+            if (k0 + KC < kend) fetch4(k0 + KC);
         } else {
 #pragma unroll
             for (int r = 0; r < LP; ++r) tin[r * P + lk][lz] = jb.mask ? nv[r] * (float)nm[r] : nv[r];
-            if (k0 + KC < K) fetch(k0 + KC);
+            if (k0 + KC < kend) fetch(k0 + KC);
         }
         __syncthreads();
         S acc;
@@ -394,7 +399,16 @@ void k_sat_tiles(const SatLaunch L) {
 #pragma unroll
         for (int i = 0; i < LP; ++i) tout[sz][part * LP + i] = S::entry(S::plus(off, pre[i]), sv[i]);
         __syncthreads();  // every thread has read carry before it moves on
-        if (part == P - 1) carry[sz] = S::plus(off, acc.get());
+        if (part == P - 1) {
+            const Lt c = S::plus(off, acc.get());
+            carry[sz] = c;
+            const int zz = z0 + sz;
+            if (kend < K && k0 + KC == kend && zz < SLNp) {
+                const int zc = zz / jb.ZT, zt = zz - zc * jb.ZT;
+                base[(((size_t)zc * jb.nrows + row) * jb.ZT + zt) * K + jb.Nk] =
+                    S::entry(S::plus(c, S::zero()), 0.0f);
+            }
+        }
 #pragma unroll
         for (int j = 0; j < LP; ++j) {
             const int w = tid + 256 * j;

@@ -11,9 +11,10 @@ device for each precision.
   prefix length times the FP32 unit round-off, two-float and f64 at the FP64
   level): the error the two-float and f64 tables remove from the reference's
   FP32 sequential prefix (MultiSlicesProjGPU.cu:42-51).
-Both builders are covered: volume tables of 64 slices (32 slices per block),
-24 slices (16 per block) and 5 slices (one row per block), and sinogram
-tables (one row per block)."""
+Both builders are covered: volume tables of 64 slices (16 slices per block
+for 16-byte entries, 32 for f32), 24 and 40 slices and 5 slices (one row per
+block), rows of 512 samples (a whole number of the tiled builder's chunks)
+and of 200 and 130, and sinogram tables (one row per block)."""
 import numpy as np
 import pytest
 
@@ -49,11 +50,12 @@ def _unchunk(T, SLN):
     return A[:SLN], A[SLN:]
 
 
-@pytest.mark.parametrize("SLN", [64, 24, 5])
-def test_volume_tables_integer_exact_and_error(SLN):
+# (512, *): rows of a whole number of the tiled builder's sample chunks (the
+# row total folded into the last chunk); 200 and 130: rows ending inside a chunk.
+@pytest.mark.parametrize("XN,SLN", [(512, 64), (512, 24), (512, 5), (200, 24), (130, 40)])
+def test_volume_tables_integer_exact_and_error(XN, SLN):
     import aapm_lowdosectimagerecon_competition_amd as dd
     from helpers import per_slice_angles
-    XN = 512
     conf = small_conf(XN=XN, YN=XN, SLN=SLN, dx=1.0, DNU=64, bin=8)
     g = dd.detector_geometry(conf)
     hangs = per_slice_angles(16, SLN) if SLN == 5 else np.zeros((16, SLN), np.float32)
