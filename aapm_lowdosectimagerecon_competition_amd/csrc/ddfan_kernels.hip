@@ -289,12 +289,14 @@ __global__ __launch_bounds__(256) void k_sat_rows(const SatLaunch L) {
 // (its mask bytes are applied when it reaches LDS), transposed through LDS, scanned by 256 / G parts of 4 samples per slice
 // (each part sums its 4 samples in order, then adds the slice's running total
 // and the totals of the parts before it, in order), and the chunk's entries
-// leave through LDS as contiguous runs of each slice's row.  The summation
-// tree depends only on G, which the launcher picks from the table's own
-// slice count, so a table's entries do not depend on what shares its launch.
-// The chunk's samples (tin) are dead once scanned, before its entries (tout)
-// are written, so the two share LDS: 19.2 KB per block at G = 32 (two-float),
-// eight blocks per CU, with the register budget of eight waves per SIMD.
+// leave through LDS as contiguous runs of each slice's row.  The sums are one
+// sequential chain of 4-sample part totals per slice whatever G is (a chunk's
+// carry is the chain up to its end), and the builder is picked from the
+// table's own slice count, so a table's entries do not depend on G or on
+// what shares its launch.  The chunk's samples (tin) are dead once scanned,
+// before its entries (tout) are written, so the two share LDS: 19.2 KB per
+// block at G = 32, 18.8 KB at G = 16 (two-float), eight blocks per CU, with
+// the register budget of eight waves per SIMD.
 // V4 (volumes whose slice count is a multiple of 4, 16-byte aligned): each
 // thread loads four slices of one position with one 16-byte load, the whole
 // chunk in one pass instead of four rounds of 4-byte loads; the same samples
