@@ -12,9 +12,10 @@ device for each precision.
   level): the error the two-float and f64 tables remove from the reference's
   FP32 sequential prefix (MultiSlicesProjGPU.cu:42-51).
 Both builders are covered: volume tables of 64 slices (16 slices per block
-for 16-byte entries, 32 for f32), 24 and 40 slices and 5 slices (one row per
-block), rows of 512 samples (a whole number of the tiled builder's chunks)
-and of 200 and 130, and sinogram tables (one row per block)."""
+for 16-byte entries, 32 for f32), 24, 40, 19 and 63 slices (16-byte and 4-byte
+sample loads) and 5 slices (one row per block), rows of 512 samples (a whole
+number of the tiled builder's chunks) and of 200 and 130, and sinogram tables
+(one row per block)."""
 import numpy as np
 import pytest
 
@@ -52,7 +53,10 @@ def _unchunk(T, SLN):
 
 # (512, *): rows of a whole number of the tiled builder's sample chunks (the
 # row total folded into the last chunk); 200 and 130: rows ending inside a chunk.
-@pytest.mark.parametrize("XN,SLN", [(512, 64), (512, 24), (512, 5), (200, 24), (130, 40)])
+# Slice counts that are multiples of 4 take the tiled builder's 16-byte loads,
+# 19 and 63 its 4-byte loads (63: 32 slices per block for f32, 16 otherwise).
+@pytest.mark.parametrize("XN,SLN", [(512, 64), (512, 24), (512, 5), (200, 24), (130, 40),
+                                    (200, 19), (130, 63)])
 def test_volume_tables_integer_exact_and_error(XN, SLN):
     import aapm_lowdosectimagerecon_competition_amd as dd
     from helpers import per_slice_angles
