@@ -166,11 +166,16 @@ struct SatLaunch {
     SatDev j[2 * kBatch];
 };
 
-// Table entries leave the row builder as non-temporal stores: its tables are
-// written once, far larger than L2, and read by the next kernel.  Measured on
-// MI355X, 512x512x64, 2304 views, two-float: sinogram table 0.536 -> 0.502 ms;
-// the tiled builder's volume tables 0.155 -> 0.215 ms with them (not used
-// there).
+// Table entries leave the row builder as non-temporal stores when its launch
+// writes at least kSatNtBytes (the size of MI355X's last-level cache): such
+// tables are written once, too large to stay cached for the next kernel.
+// Measured on MI355X, 512x512x64, 2304 views, two-float: sinogram table
+// 0.536 -> 0.502 ms; the tiled builder's volume tables 0.155 -> 0.215 ms with
+// them (not used there).  Smaller tables keep ordinary stores, which leave
+// them cached for the kernel that reads them: 16 reconstructions of 256x256 x
+// 8-16 slices x 720 views one after another 0.384 s, 0.451 s with
+// non-temporal stores; OS-SART subset tables (72 MB) the same either way.
+constexpr size_t kSatNtBytes = size_t(256) << 20;
 typedef float nt_f4 __attribute__((ext_vector_type(4)));
 typedef float nt_f2 __attribute__((ext_vector_type(2)));
 typedef double nt_d2 __attribute__((ext_vector_type(2)));
@@ -196,7 +201,7 @@ __device__ __forceinline__ float2 lane_up(float2 v, int d) {
 // 256 / G segments give at most 32 samples (f32 tables, G = 1, K = 8064).
 constexpr int kSatSeg = 32;
 
-template <int MODE, int G>
+template <int MODE, int G, bool NT>
 __global__ __launch_bounds__(256) void k_sat_rows(const SatLaunch L) {
     using S = Scan<MODE>;
     using Lt = typename S::L;
@@ -277,7 +282,10 @@ __global__ __launch_bounds__(256) void k_sat_rows(const SatLaunch L) {
         E* dst = base + (((size_t)zc * jb.nrows + row) * jb.ZT + zt) * K;
         const Lt* pq = pre + (size_t)q * K;
         const float* vq = val + (size_t)q * K;
-        for (int k = threadIdx.x; k < K; k += 256) store_nt(dst + k, S::entry(pq[k], vq[k]));
+        for (int k = threadIdx.x; k < K; k += 256) {
+            if constexpr (NT) store_nt(dst + k, S::entry(pq[k], vq[k]));
+            else dst[k] = S::entry(pq[k], vq[k]);
+        }
     }
 }
 
@@ -1670,7 +1678,14 @@ static int sat_group(int mode, int K) {
 static hipError_t launch_sat_group(int mode, int G, SatDev* t, int n, hipStream_t st) {
     if (n == 0) return hipSuccess;
     int maxK = 0;
-    for (int i = 0; i < n; ++i) maxK = t[i].Nk + 1 > maxK ? t[i].Nk + 1 : maxK;
+    size_t bytes = 0;  // table entries the launch writes
+    for (int i = 0; i < n; ++i) {
+        maxK = t[i].Nk + 1 > maxK ? t[i].Nk + 1 : maxK;
+        if (t[i].nrows > 0)
+            bytes += (size_t)t[i].nrows * (size_t)(t[i].Nk + 1) * (size_t)(t[i].nzc * t[i].ZT) *
+                     (mode == SAT_F32 ? 8 : 16);
+    }
+    const bool nt = bytes >= kSatNtBytes;
     SatLaunch L{};
     long long nb = 0;
     int m = 0;
@@ -1689,8 +1704,12 @@ static hipError_t launch_sat_group(int mode, int G, SatDev* t, int n, hipStream_
     const unsigned grid = (unsigned)xcd_grid((int)nb);
     const size_t lsz = mode == SAT_F32 ? 4 : 8;
     const size_t lds = (size_t)G * maxK * (lsz + 4) + (size_t)256 * lsz;
-#define DDF_SATV(GG) \
-    DDF_MODE_SWITCH(mode, k_sat_rows<MC, GG><<<grid, 256, lds, st>>>(L))
+#define DDF_SATV(GG)                                                                   \
+    if (nt) {                                                                          \
+        DDF_MODE_SWITCH(mode, (k_sat_rows<MC, GG, true><<<grid, 256, lds, st>>>(L)));  \
+    } else {                                                                           \
+        DDF_MODE_SWITCH(mode, (k_sat_rows<MC, GG, false><<<grid, 256, lds, st>>>(L))); \
+    }
     switch (G) {
         case 4: DDF_SATV(4); break;
         case 2: DDF_SATV(2); break;
