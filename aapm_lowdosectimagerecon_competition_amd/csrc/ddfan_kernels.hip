@@ -163,6 +163,7 @@ struct SatDev {
 };
 struct SatLaunch {
     int njobs, nwork;
+    int nt;  // k_sat_tiles: entries leave as non-temporal stores
     SatDev j[2 * kBatch];
 };
 
@@ -170,12 +171,17 @@ struct SatLaunch {
 // writes at least kSatNtBytes (the size of MI355X's last-level cache): such
 // tables are written once, too large to stay cached for the next kernel.
 // Measured on MI355X, 512x512x64, 2304 views, two-float: sinogram table
-// 0.536 -> 0.502 ms; the tiled builder's volume tables 0.155 -> 0.215 ms with
-// them (not used there).  Smaller tables keep ordinary stores, which leave
+// 0.536 -> 0.502 ms.  Smaller tables keep ordinary stores, which leave
 // them cached for the kernel that reads them: 16 reconstructions of 256x256 x
 // 8-16 slices x 720 views one after another 0.384 s, 0.451 s with
 // non-temporal stores; OS-SART subset tables (72 MB) the same either way.
+// The tiled builder takes them from 1 GiB of entries per launch: both volume
+// tables of 512x512x200 in one-slice chunks (1.68 GB, two-float) 0.535 ->
+// 0.497 ms, and the projector reading them 2.26 -> 2.23 ms; at 512x512x64
+// (0.55 GB) 0.141 -> 0.166 ms (0.155 -> 0.215 ms at 32 slices per block), so
+// smaller launches keep ordinary stores.
 constexpr size_t kSatNtBytes = size_t(256) << 20;
+constexpr size_t kSatTileNtBytes = size_t(1) << 30;
 typedef float nt_f4 __attribute__((ext_vector_type(4)));
 typedef float nt_f2 __attribute__((ext_vector_type(2)));
 typedef double nt_d2 __attribute__((ext_vector_type(2)));
@@ -415,8 +421,10 @@ void k_sat_tiles(const SatLaunch L) {
             const int zz = z0 + sz;
             if (kend < K && k0 + KC == kend && zz < SLNp) {
                 const int zc = zz / jb.ZT, zt = zz - zc * jb.ZT;
-                base[(((size_t)zc * jb.nrows + row) * jb.ZT + zt) * K + jb.Nk] =
-                    S::entry(S::plus(c, S::zero()), 0.0f);
+                E* const dst = &base[(((size_t)zc * jb.nrows + row) * jb.ZT + zt) * K + jb.Nk];
+                const E e = S::entry(S::plus(c, S::zero()), 0.0f);
+                if (L.nt) store_nt(dst, e);
+                else *dst = e;
             }
         }
 #pragma unroll
@@ -426,7 +434,9 @@ void k_sat_tiles(const SatLaunch L) {
             const int zz = z0 + z, k = k0 + kk;
             if (k < K && zz < SLNp) {
                 const int zc = zz / jb.ZT, zt = zz - zc * jb.ZT;
-                base[(((size_t)zc * jb.nrows + row) * jb.ZT + zt) * K + k] = tout[z][kk];
+                E* const dst = &base[(((size_t)zc * jb.nrows + row) * jb.ZT + zt) * K + k];
+                if (L.nt) store_nt(dst, tout[z][kk]);
+                else *dst = tout[z][kk];
             }
         }
         __syncthreads();
@@ -1733,6 +1743,11 @@ static hipError_t launch_sat_tile_group(int mode, int G, SatDev* t, int n, hipSt
         L.j[m++] = t[i];
     }
     L.njobs = m;
+    size_t bytes = 0;  // table entries the launch writes
+    for (int i = 0; i < m; ++i)
+        bytes += (size_t)L.j[i].nrows * (size_t)(L.j[i].Nk + 1) * (size_t)(L.j[i].nzc * L.j[i].ZT) *
+                 (mode == SAT_F32 ? 8 : 16);
+    L.nt = bytes >= kSatTileNtBytes;
     if (nb == 0) return hipSuccess;
     if (nb > 0x7fffff00LL) return hipErrorInvalidValue;
     L.nwork = (int)nb;
